@@ -1,0 +1,38 @@
+"""Build libaerocfd_amd.so in-tree with hipcc for gfx950.
+
+    python -m aerocfd_amd.build
+"""
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+SOURCES = ["k_flux.hip", "k_implicit.hip", "k_sa.hip", "k_bc.hip", "context.hip"]
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+FLAGS = [
+    "--offload-arch=gfx950", "-O3", "-fPIC", "-shared", "-std=c++17",
+    # no packed FP32 instructions in any kernel
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+]
+
+
+def build(verbose=False):
+    out = os.path.join(HERE, "libaerocfd_amd.so")
+    tmp = out + ".tmp"
+    cmd = [HIPCC] + FLAGS + ["-I" + os.path.join(ROOT, "include")] + \
+        [os.path.join(HERE, "csrc", s) for s in SOURCES] + ["-o", tmp]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    msgs = [l for l in (r.stdout + r.stderr).splitlines()
+            if l.strip() and "packed-fp32-ops" not in l]
+    if r.returncode != 0:
+        raise RuntimeError("hipcc failed:\n" + "\n".join(msgs))
+    if verbose and msgs:
+        print("\n".join(msgs))
+    os.replace(tmp, out)
+    return out
+
+
+if __name__ == "__main__":
+    print(build(verbose=True))
+    sys.exit(0)

@@ -1,0 +1,345 @@
+"""Synthetic multi-block cases for the residual-and-update path.
+
+Blocks use the reference's overlap convention: neighbouring blocks share their
+interface node layer (node ni of one block is node 1 of the next), and each
+interface face carries two halo layers filled from the neighbour's nodes 2 and
+3 (sub_bc_updatebuffer.f90:57-135).  Subface records are the 11 integers of a
+bcinfo line (sub_init_readbcinfo.f90:96-98): f_no, face, is, ie, js, je, ks,
+ke, blk_t, subface_t, ori.  The orientation code ``ori`` of every interface is
+found by matching node identities through the same face pairing the halo
+exchange uses, so a generated topology is geometrically consistent by
+construction.
+
+Cases (the configurations the measurements are quoted on):
+  * ``vortex_box``  -- single block on [-5,5]^3, farfield on all faces,
+    isentropic vortex on a uniform freestream (Euler).
+  * ``tgv_box``     -- single block on [0,2 pi]^3 with node n == node 1 on every
+    face pair (self-connected interfaces), Taylor-Green vortex (laminar NS).
+  * ``split_box``   -- the same kinds of state on a box cut into several
+    blocks with interfaces of every orientation.
+"""
+import math
+
+import numpy as np
+
+GAMMA = 1.4
+
+FACE_AXIS = {1: 0, 2: 0, 3: 1, 4: 1, 5: 2, 6: 2}
+
+
+# ------------------------------------------------------------ face pairing
+def _pair_nodes(subs, dims, recv, send):
+    """Sender face node of every receiving face node for the given
+    receiving/sending subface records (face_t load, ori permutation, face
+    store of sub_bc_updatebuffer.f90:261-539)."""
+    f, t = recv, send
+    n11, n12, n13 = t[3] - t[2] + 1, t[5] - t[4] + 1, t[7] - t[6] + 1
+    l1, m1 = {1: (n13, n12), 2: (n12, n13), 3: (n11, n13), 4: (n13, n11), 5: (n12, n11), 6: (n11, n12)}[t[1]]
+
+    def ua(ll, mm):
+        ts, te, js, je, ks, ke = t[2], t[3], t[4], t[5], t[6], t[7]
+        return {1: (ts, js + mm - 1, ks + ll - 1), 2: (te, js + ll - 1, ks + mm - 1),
+                3: (ts + ll - 1, js, ks + mm - 1), 4: (ts + mm - 1, je, ks + ll - 1),
+                5: (ts + mm - 1, js + ll - 1, ks), 6: (ts + ll - 1, js + mm - 1, ke)}[t[1]]
+
+    ori = f[10]
+    tr = ori in (2, 4)
+    l2, m2 = (m1, l1) if tr else (l1, m1)
+    n1, n2, n3 = f[3] - f[2] + 1, f[5] - f[4] + 1, f[7] - f[6] + 1
+    ra, rb = {1: (n3, n2), 2: (n2, n3), 3: (n1, n3), 4: (n3, n1), 5: (n2, n1), 6: (n1, n2)}[f[1]]
+    if (ra, rb) != (l2, m2):
+        return None
+    out = []
+    for b in range(1, m2 + 1):
+        for a in range(1, l2 + 1):
+            ll, mm = {1: (a, m1 - b + 1), 2: (b, a), 3: (l1 - a + 1, b), 4: (l1 - b + 1, m1 - a + 1)}[ori]
+            s = ua(ll, mm)
+            fs, fe, gjs, gje, gks, gke = f[2], f[3], f[4], f[5], f[6], f[7]
+            r = {1: (fs, gjs + b - 1, gks + a - 1), 2: (fe, gjs + a - 1, gks + b - 1),
+                 3: (fs + a - 1, gjs, gks + b - 1), 4: (fs + b - 1, gje, gks + a - 1),
+                 5: (fs + b - 1, gjs + a - 1, gks), 6: (fs + a - 1, gjs + b - 1, gke)}[f[1]]
+            out.append((r, s))
+    return out
+
+
+def assign_orientations(dims, subfaces, gid):
+    """Choose ori of every interface so that receiving and sending face
+    nodes are the same node. gid(m, i, j, k) -> hashable global identity of
+    node (i,j,k) of block m (0-based m)."""
+    for m, subs in enumerate(subfaces):
+        for s in subs:
+            if s[8] <= 0:
+                continue
+            t = subfaces[s[8] - 1][s[9] - 1]
+            for ori in (1, 2, 3, 4):
+                s[10] = ori
+                pn = _pair_nodes(subfaces, dims, s, t)
+                if pn is None:
+                    continue
+                if all(gid(m, *r) == gid(s[8] - 1, *sn) for r, sn in pn):
+                    break
+            else:
+                raise ValueError("no orientation matches interface %s of block %d" % (s, m + 1))
+    return subfaces
+
+
+def _face_range(face, ni, nj, nk):
+    return {1: (1, 1, 1, nj, 1, nk), 2: (ni, ni, 1, nj, 1, nk), 3: (1, ni, 1, 1, 1, nk),
+            4: (1, ni, nj, nj, 1, nk), 5: (1, ni, 1, nj, 1, 1), 6: (1, ni, 1, nj, nk, nk)}[face]
+
+
+def box_subfaces(ni, nj, nk, kinds):
+    """Six whole-face subfaces; kinds[f-1] = blk_t for face f (physical codes
+    < 0) or (blk_t, subface_t) for an interface."""
+    subs = []
+    for f in range(1, 7):
+        kd = kinds[f - 1]
+        bt, st = (kd, 0) if isinstance(kd, int) else kd
+        subs.append([f, f, *_face_range(f, ni, nj, nk), bt, st, 0])
+    return subs
+
+
+def split_box(NI, NJ, NK, cuts=((), (), ()), phys=(-10,) * 6, periodic=(False, False, False),
+              permute=None):
+    """Cut a box of NI x NJ x NK nodes into blocks at the given node indices
+    (1-based global, shared by both sides).  phys: physical code per box face
+    where not periodic.  permute: optional per-block axis permutation
+    (a tuple of 3) applied to the block's local axes, to exercise every
+    orientation code.  Returns dims, subfaces, origins (global index of local
+    node (1,1,1) and the axis map of each block)."""
+    edges = []
+    for ax, n in enumerate((NI, NJ, NK)):
+        e = [1] + sorted(cuts[ax]) + [n]
+        edges.append(e)
+    nb = [len(e) - 1 for e in edges]
+    blocks = []
+    for bk in range(nb[2]):
+        for bj in range(nb[1]):
+            for bi in range(nb[0]):
+                lo = (edges[0][bi], edges[1][bj], edges[2][bk])
+                hi = (edges[0][bi + 1], edges[1][bj + 1], edges[2][bk + 1])
+                blocks.append((bi, bj, bk, lo, hi))
+    bid = {(b[0], b[1], b[2]): x for x, b in enumerate(blocks)}
+    perms = []
+    dims = []
+    for x, (bi, bj, bk, lo, hi) in enumerate(blocks):
+        p = tuple(permute[x]) if permute else (0, 1, 2)
+        perms.append(p)
+        ext = [hi[a] - lo[a] + 1 for a in range(3)]
+        dims.append(tuple(ext[p[a]] for a in range(3)))
+    # face of block x (global axis ax, side sd) in local face numbering
+    def local_face(x, ax, sd):
+        la = perms[x].index(ax)
+        return 2 * la + 1 + sd
+    subfaces = [[None] * 6 for _ in blocks]
+    for x, (bi, bj, bk, lo, hi) in enumerate(blocks):
+        idx = (bi, bj, bk)
+        for ax in range(3):
+            for sd in (0, 1):
+                nbr = list(idx)
+                nbr[ax] += -1 if sd == 0 else 1
+                per = False
+                if nbr[ax] < 0 or nbr[ax] >= nb[ax]:
+                    if periodic[ax]:
+                        nbr[ax] %= nb[ax]
+                        per = True
+                    else:
+                        nbr = None
+                f = local_face(x, ax, sd)
+                ni, nj, nk = dims[x]
+                rng = _face_range(f, ni, nj, nk)
+                if nbr is None:
+                    subfaces[x][f - 1] = [f, f, *rng, phys[2 * ax + sd], 0, 0]
+                else:
+                    y = bid[tuple(nbr)]
+                    fy = local_face(y, ax, 1 - sd)
+                    subfaces[x][f - 1] = [f, f, *rng, y + 1, fy, 0]
+    periods = (NI - 1, NJ - 1, NK - 1)
+
+    def gid(m, i, j, k):
+        loc = (i, j, k)
+        p = perms[m]
+        lo = blocks[m][3]
+        g = [0, 0, 0]
+        for a in range(3):
+            g[p[a]] = lo[p[a]] + loc[a] - 1
+        return tuple(g[a] % periods[a] if periodic[a] else g[a] for a in range(3))
+
+    assign_orientations(dims, subfaces, gid)
+    origins = [(b[3], perms[x]) for x, b in enumerate(blocks)]
+    return dims, subfaces, origins
+
+
+# ---------------------------------------------------------------- geometry
+def ranges_of(dims_m, subs_m):
+    """The 18 index ranges of sub_init_allocatememory.f90:73-144."""
+    ni, nj, nk = dims_m
+    r0 = [1, ni, 1, nj, 1, nk]
+    r = list(r0)
+    for s in subs_m:
+        if s[8] > 0:
+            f = s[1]
+            if f in (1, 3, 5):
+                r[f - 1] -= 2
+            else:
+                r[f - 1] += 2
+    r1 = [r0[a] + (1 if a % 2 == 0 else -1) if r0[a] == r[a] else r0[a] for a in range(6)]
+    return tuple(r + r0 + r1)
+
+
+def node_coords(ranges, origin, spacing, x0, perm=(0, 1, 2)):
+    """Cartesian coordinates (3, nk, nj, ni) of a uniform-grid block whose
+    local node (1,1,1) is global node ``origin``; halo nodes extrapolate the
+    lattice (the values the coordinate exchange delivers on a uniform grid
+    with periodic shifts)."""
+    is_, ie, js, je, ks, ke = ranges[:6]
+    li = np.arange(is_, ie + 1)
+    lj = np.arange(js, je + 1)
+    lk = np.arange(ks, ke + 1)
+    K, J, I = np.meshgrid(lk, lj, li, indexing="ij")
+    loc = (I, J, K)
+    xyz = np.zeros((3,) + I.shape)
+    for a in range(3):
+        ga = perm.index(a)  # local axis that carries global axis a
+        xyz[a] = x0[a] + spacing[a] * (origin[a] + loc[ga] - 2)
+    return xyz
+
+
+def uniform_metrics(ranges, spacing, perm=(0, 1, 2)):
+    """Metrics of a uniform Cartesian block: dxidx(3(d-1)+c) = d xi_d/d x_c,
+    inv_j = 1/J, and alpha/beta/gamma(c) = (grad xi_d . grad xi_c)/J -- the
+    values gridtransformation (sub_init_gridtransformation.f90:27-874) returns
+    on such a grid up to rounding."""
+    is_, ie, js, je, ks, ke = ranges[:6]
+    shp = (ke - ks + 1, je - js + 1, ie - is_ + 1)
+    h = [spacing[perm[a]] for a in range(3)]  # local-axis spacings
+    vol = h[0] * h[1] * h[2]
+    dxidx = np.zeros(shp + (9,))
+    for d in range(3):
+        dxidx[..., 3 * d + perm[d]] = 1.0 / h[d]
+    inv_j = np.full(shp, vol)
+    abg = []
+    for d in range(3):
+        a = np.zeros(shp + (3,))
+        a[..., d] = vol / (h[d] * h[d])
+        abg.append(a)
+    return dxidx, inv_j, abg[0], abg[1], abg[2]
+
+
+# ------------------------------------------------------- initial conditions
+def conservative(rho, u, v, w, p):
+    q = np.zeros(rho.shape + (5,))
+    q[..., 0] = rho
+    q[..., 1] = rho * u
+    q[..., 2] = rho * v
+    q[..., 3] = rho * w
+    q[..., 4] = p / (GAMMA - 1.0) + 0.5 * rho * (u * u + v * v + w * w)
+    return q
+
+
+def vortex_state(xyz, ma, beta=5.0, center=(0.0, 0.0)):
+    """Isentropic vortex (axis along z) on a uniform freestream u = 1 in the
+    reference's scaling (rho_inf = u_inf = T_inf = 1, p_inf = 1/(gamma ma^2)):
+    velocity perturbation scaled by c_inf/sqrt(gamma)."""
+    x = xyz[0] - center[0]
+    y = xyz[1] - center[1]
+    r2 = x * x + y * y
+    cinf = 1.0 / ma
+    s = cinf / math.sqrt(GAMMA)
+    du = -beta / (2 * math.pi) * np.exp(0.5 * (1 - r2)) * y * s
+    dv = beta / (2 * math.pi) * np.exp(0.5 * (1 - r2)) * x * s
+    dT = -(GAMMA - 1.0) * beta * beta / (8 * GAMMA * math.pi ** 2) * np.exp(1 - r2)
+    rho = (1.0 + dT) ** (1.0 / (GAMMA - 1.0))
+    p = 1.0 / (GAMMA * ma * ma) * rho ** GAMMA
+    return conservative(rho, 1.0 + du, dv, np.zeros_like(x), p)
+
+
+def tgv_state(xyz, ma):
+    """Taylor-Green vortex: u = sin x cos y cos z, v = -cos x sin y cos z,
+    w = 0, p = p_inf + (cos 2x + cos 2y)(cos 2z + 2)/16, isothermal T = 1
+    (rho = gamma ma^2 p)."""
+    x, y, z = xyz
+    u = np.sin(x) * np.cos(y) * np.cos(z)
+    v = -np.cos(x) * np.sin(y) * np.cos(z)
+    w = np.zeros_like(x)
+    p = 1.0 / (GAMMA * ma * ma) + (np.cos(2 * x) + np.cos(2 * y)) * (np.cos(2 * z) + 2.0) / 16.0
+    rho = GAMMA * ma * ma * p
+    return conservative(rho, u, v, w, p)
+
+
+# ------------------------------------------------------------------- cases
+class Case:
+    """Topology plus per-block input fields, ready for Solver.upload."""
+
+    def __init__(self, name, cfg, dims, subfaces):
+        self.name = name
+        self.cfg = cfg
+        self.dims = dims
+        self.subfaces = subfaces
+        self.ranges = [ranges_of(d, s) for d, s in zip(dims, subfaces)]
+        self.fields = [dict() for _ in dims]
+
+    @property
+    def totalnodes(self):
+        return int(sum(a * b * c for a, b, c in self.dims))
+
+    def shape(self, m):
+        r = self.ranges[m]
+        return (r[5] - r[4] + 1, r[3] - r[2] + 1, r[1] - r[0] + 1)
+
+    def upload(self, solver):
+        for m, fl in enumerate(self.fields):
+            for k, v in fl.items():
+                solver.upload(m + 1, k, v)
+
+
+def _common_fields(case, m, q, dxidx, inv_j, al, be, ga):
+    shp = case.shape(m)
+    fl = case.fields[m]
+    fl["q"] = q
+    fl["dxidx"] = dxidx
+    fl["inv_j"] = inv_j
+    amu = np.zeros(shp + (3,))
+    amu[..., 0] = 1.0
+    amu[..., 2] = 1.0
+    fl["amu"] = amu
+    fl["dt"] = np.full(shp, case.cfg.dt)
+    if case.cfg.iflag_turbulence == 2:
+        fl["aalpha"], fl["bbeta"], fl["ggamma"] = al, be, ga
+
+
+def vortex_box(n=64, cfg=None, blocks=((), (), ()), permute=None):
+    """Single (or cut) block n^3 on [-5,5]^3, farfield everywhere, isentropic
+    vortex; Euler, first-order implicit, dt = 0.5*(10/(n-1))/(1+2)."""
+    from .solver import FlowConfig
+    h = 10.0 / (n - 1)
+    if cfg is None:
+        cfg = FlowConfig(ma=0.5, re=1.0e6, iflag_solver=1, iflag_time=1, iflag_timeadvance=2,
+                         dt=0.5 * h / (1.0 + 2.0))
+    dims, subs, origins = split_box(n, n, n, blocks, phys=(-10,) * 6, permute=permute)
+    case = Case("vortex_box_%d" % n, cfg, dims, subs)
+    for m in range(len(dims)):
+        org, perm = origins[m]
+        xyz = node_coords(case.ranges[m], org, (h, h, h), (-5.0, -5.0, -5.0), perm)
+        q = vortex_state(xyz, cfg.ma)
+        _common_fields(case, m, q, *uniform_metrics(case.ranges[m], (h, h, h), perm))
+    return case
+
+
+def tgv_box(n=256, cfg=None, blocks=((), (), ()), permute=None):
+    """n^3 nodes on [0,2 pi]^3, node n == node 1 on every face pair (all
+    faces self-connected or connected through the cut blocks), Taylor-Green
+    vortex at Mach 0.1, Re = 1600, laminar NS, first-order implicit."""
+    from .solver import FlowConfig
+    h = 2.0 * math.pi / (n - 1)
+    if cfg is None:
+        cfg = FlowConfig(ma=0.1, re=1600.0, iflag_solver=0, iflag_time=1, iflag_timeadvance=2,
+                         dt=0.5 * h / (1.0 + 10.0), prl=0.72)
+    dims, subs, origins = split_box(n, n, n, blocks, periodic=(True, True, True), permute=permute)
+    case = Case("tgv_box_%d" % n, cfg, dims, subs)
+    for m in range(len(dims)):
+        org, perm = origins[m]
+        xyz = node_coords(case.ranges[m], org, (h, h, h), (0.0, 0.0, 0.0), perm)
+        q = tgv_state(xyz, cfg.ma)
+        _common_fields(case, m, q, *uniform_metrics(case.ranges[m], (h, h, h), perm))
+    return case

@@ -1,0 +1,84 @@
+// Shared device-side definitions for the aerocfd_amd kernels (gfx950).
+//
+// Device layout: every block is resident in HBM as structure-of-arrays,
+// field component c of node (i,j,k) at base[c*n + ix(i,j,k)], i fastest,
+// over the block's extended extents (is..ie, js..je, ks..ke, 1-based as in
+// the reference).  A wavefront of 64 lanes walks 64 consecutive i, so every
+// field load is one coalesced 512-byte transaction.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace acfd {
+
+// Index ranges of one block (module_block.f90:23-25) plus derived sizes.
+struct Ext {
+    int is, ie, js, je, ks, ke;
+    int is0, ie0, js0, je0, ks0, ke0;
+    int is1, ie1, js1, je1, ks1, ke1;
+    int ni, nj, nk;
+    long n;
+    __host__ __device__ inline long ix(int i, int j, int k) const {
+        return (long)(i - is) + (long)ni * ((long)(j - js) + (long)nj * (long)(k - ks));
+    }
+    __host__ __device__ inline long stride(int axis) const {
+        return axis == 0 ? 1 : axis == 1 ? (long)ni : (long)ni * nj;
+    }
+    __host__ __device__ inline int lo(int axis) const { return axis == 0 ? is : axis == 1 ? js : ks; }
+    __host__ __device__ inline int hi(int axis) const { return axis == 0 ? ie : axis == 1 ? je : ke; }
+    __host__ __device__ inline int lo0(int axis) const { return axis == 0 ? is0 : axis == 1 ? js0 : ks0; }
+    __host__ __device__ inline int hi0(int axis) const { return axis == 0 ? ie0 : axis == 1 ? je0 : ke0; }
+    __host__ __device__ inline int lo1(int axis) const { return axis == 0 ? is1 : axis == 1 ? js1 : ks1; }
+    __host__ __device__ inline int hi1(int axis) const { return axis == 0 ? ie1 : axis == 1 ? je1 : ke1; }
+};
+
+// Flow constants (sub_init_flowfield.f90:30-39, module_ns_consts.f90:31-33).
+struct Consts {
+    double gamma, cv, re, tinf, prl, prt;
+    double d0, u0, p0, c0, aoa;
+    int solver, itime, turb, des;
+};
+
+// Device pointers of one block.  Multi-component fields are SoA with
+// component stride n.
+struct Fields {
+    double *q, *pri, *rhsi, *rhsv, *rhs;
+    double *dxidx, *inv_j, *amu, *dt;
+    double *dudx, *tao, *vor;
+    double *aal, *bbe, *gga, *dist, *spacing, *length;
+    double *nut, *sa_rhs, *q0;
+    // work arrays
+    double *w_flux;   // 5n face fluxes / 12n viscous node fluxes (max)
+    double *w_dqs;    // 5n
+    double *w_dq;     // 5n
+    double *w_diag;   // n
+    double *w_r;      // 3n spectral radii
+    double *w_sa;     // 9n SA temporaries
+};
+
+// A box [i0,i1]x[j0,j1]x[k0,k1] of nodes covered by a (64,4,1) thread
+// block tiling: blockIdx.x over i, y over j, z over k.
+struct Box {
+    int i0, i1, j0, j1, k0, k1;
+};
+
+inline dim3 box_grid(const Box &b, int bx = 64, int by = 4) {
+    int ni = b.i1 - b.i0 + 1, nj = b.j1 - b.j0 + 1, nk = b.k1 - b.k0 + 1;
+    if (ni <= 0 || nj <= 0 || nk <= 0) return dim3(0, 0, 0);
+    return dim3((ni + bx - 1) / bx, (nj + by - 1) / by, nk);
+}
+
+__device__ inline bool box_node(const Box &b, int &i, int &j, int &k) {
+    i = b.i0 + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    j = b.j0 + (int)(blockIdx.y * blockDim.y + threadIdx.y);
+    k = b.k0 + (int)blockIdx.z;
+    return i <= b.i1 && j <= b.j1 && k <= b.k1;
+}
+
+}  // namespace acfd
+
+#define ACFD_CHECK_LAUNCH()                                                    \
+    do {                                                                       \
+        hipError_t e__ = hipGetLastError();                                    \
+        if (e__ != hipSuccess) return (int)e__;                                \
+    } while (0)

@@ -1,0 +1,1161 @@
+// Host side of the C ABI: the resident context (block table, device
+// arrays, halo copy lists, step orchestration) and the stateless per-routine
+// entry points.  The orchestration follows the reference time loop
+// (main.f90:343-573); the topology rules follow sub_init_allocatememory.f90
+// and sub_init_readbcinfo.f90.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/aerocfd_amd.h"
+#include "common.hpp"
+#include "kernels.hpp"
+
+using namespace acfd;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg)
+{
+    g_err = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                  \
+    do {                                                                               \
+        hipError_t e__ = (expr);                                                       \
+        if (e__ != hipSuccess)                                                         \
+            return fail(ACFD_EDEVICE, std::string(#expr ": ") + hipGetErrorString(e__)); \
+    } while (0)
+
+#define KRN_TRY(expr)                                                                          \
+    do {                                                                                       \
+        int r__ = (expr);                                                                      \
+        if (r__ != 0)                                                                          \
+            return fail(ACFD_EDEVICE, std::string(#expr ": ") + hipGetErrorString((hipError_t)r__)); \
+    } while (0)
+
+const int kNcomp[22] = {0, 5, 7, 5, 5, 5, 9, 6, 1, 3, 1, 9, 1, 3, 3, 3, 1, 1, 1, 1, 10, 1};
+
+struct Block {
+    Ext e;
+    Fields f;
+    double *arena = nullptr;
+    std::vector<acfd_subface> sub;
+};
+
+}  // namespace
+
+struct acfd_ctx {
+    acfd_config cfg;
+    Consts c;
+    hipStream_t stream = nullptr;
+    std::vector<Block> blk;
+    long totalnodes = 0;
+    bool topo = false;
+    // device scalars
+    double *d_cfl = nullptr;
+    unsigned long long *d_stats = nullptr;
+    unsigned int *d_nneg = nullptr;
+    double *d_res = nullptr;       // 6 sums
+    double *d_partial = nullptr;   // residual partials
+    int *d_tile_flags = nullptr;
+    long tile_flags_n = 0;
+    double *d_stage = nullptr;     // upload/download staging
+    long stage_n = 0;
+    // pointer tables for the multi-block copy kernels
+    double **d_qtab = nullptr, **d_nuttab = nullptr;
+    long *d_ntab = nullptr;
+    // halo copy list
+    int halo_n = 0;
+    int *d_h_sb = nullptr, *d_h_db = nullptr;
+    long *d_h_sn = nullptr, *d_h_dn = nullptr;
+    std::vector<int> h_sb, h_db;
+    std::vector<long> h_sn, h_dn;
+    std::vector<int> h_sub;   // receiving subface (global order) of every entry
+    // average list
+    AvgList avg{};
+    int *d_a_sb = nullptr, *d_a_gb = nullptr, *d_a_gs = nullptr;
+    long *d_a_sn = nullptr, *d_a_gd = nullptr;
+    double *d_a_buf = nullptr;
+    std::vector<BcTask> bc;
+    // timing: event pairs recorded on the stream without waiting; read
+    // back (and the pool recycled) when full or when the totals are asked for
+    bool timing = false;
+    double stage_ms[16] = {0};
+    long stage_calls[16] = {0};
+    std::vector<hipEvent_t> ev_pool;
+    std::vector<int> ev_stage;   // stage of each recorded pair
+    size_t ev_used = 0;          // pairs in use
+};
+
+namespace {
+
+void free_block(Block &b)
+{
+    if (b.arena) (void)hipFree(b.arena);
+    b.arena = nullptr;
+}
+
+template <typename T>
+int dev_upload(T **dst, const std::vector<T> &src)
+{
+    if (*dst) (void)hipFree(*dst);
+    *dst = nullptr;
+    if (src.empty()) return 0;
+    HIP_TRY(hipMalloc((void **)dst, src.size() * sizeof(T)));
+    HIP_TRY(hipMemcpy(*dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice));
+    return 0;
+}
+
+// Ranges of sub_init_allocatememory.f90:73-144 (3D, bufferLength = 2).
+void block_ranges(int ni, int nj, int nk, const acfd_subface *s, int ns, Ext &e)
+{
+    const int bl = 2;
+    e.is0 = 1; e.ie0 = ni; e.js0 = 1; e.je0 = nj; e.ks0 = 1; e.ke0 = nk;
+    e.is = e.is0; e.ie = e.ie0; e.js = e.js0; e.je = e.je0; e.ks = e.ks0; e.ke = e.ke0;
+    for (int x = 0; x < ns; x++) {
+        if (s[x].blk_t <= 0) continue;
+        switch (s[x].face) {
+        case 1: e.is = e.is0 - bl; break;
+        case 2: e.ie = e.ie0 + bl; break;
+        case 3: e.js = e.js0 - bl; break;
+        case 4: e.je = e.je0 + bl; break;
+        case 5: e.ks = e.ks0 - bl; break;
+        case 6: e.ke = e.ke0 + bl; break;
+        }
+    }
+    e.is1 = e.is0 == e.is ? e.is0 + 1 : e.is0;
+    e.ie1 = e.ie0 == e.ie ? e.ie0 - 1 : e.ie0;
+    e.js1 = e.js0 == e.js ? e.js0 + 1 : e.js0;
+    e.je1 = e.je0 == e.je ? e.je0 - 1 : e.je0;
+    e.ks1 = e.ks0 == e.ks ? e.ks0 + 1 : e.ks0;
+    e.ke1 = e.ke0 == e.ke ? e.ke0 - 1 : e.ke0;
+    e.ni = e.ie - e.is + 1;
+    e.nj = e.je - e.js + 1;
+    e.nk = e.ke - e.ks + 1;
+    e.n = (long)e.ni * e.nj * e.nk;
+}
+
+struct PairNode {
+    int ri, rj, rk, si, sj, sk;
+};
+
+// For the receiving subface f of block m, the sender's face node of every
+// receiving face node: the face_t load (sub_bc_updatebuffer.f90:265-354),
+// the ori permutation (:358-395) and the face store (:398-539), done on
+// coordinates.  Returns false when the faces do not match in size.
+bool pair_nodes(const std::vector<Block> &blk, const acfd_subface &f, std::vector<PairNode> &out)
+{
+    const acfd_subface &t = blk[f.blk_t - 1].sub[f.subface_t - 1];
+    int n11 = t.ie - t.is + 1, n12 = t.je - t.js + 1, n13 = t.ke - t.ks + 1;
+    int l1 = 0, m1 = 0;
+    switch (t.face) {
+    case 1: l1 = n13; m1 = n12; break;
+    case 2: l1 = n12; m1 = n13; break;
+    case 3: l1 = n11; m1 = n13; break;
+    case 4: l1 = n13; m1 = n11; break;
+    case 5: l1 = n12; m1 = n11; break;
+    case 6: l1 = n11; m1 = n12; break;
+    default: return false;
+    }
+    auto ua = [&](int ll, int mm, int *p) {
+        switch (t.face) {
+        case 1: p[0] = t.is; p[1] = t.js + mm - 1; p[2] = t.ks + ll - 1; break;
+        case 2: p[0] = t.ie; p[1] = t.js + ll - 1; p[2] = t.ks + mm - 1; break;
+        case 3: p[0] = t.is + ll - 1; p[1] = t.js; p[2] = t.ks + mm - 1; break;
+        case 4: p[0] = t.is + mm - 1; p[1] = t.je; p[2] = t.ks + ll - 1; break;
+        case 5: p[0] = t.is + mm - 1; p[1] = t.js + ll - 1; p[2] = t.ks; break;
+        case 6: p[0] = t.is + ll - 1; p[1] = t.js + mm - 1; p[2] = t.ke; break;
+        }
+    };
+    bool tr = f.ori == 2 || f.ori == 4;
+    int l2 = tr ? m1 : l1, m2 = tr ? l1 : m1;
+    int n1 = f.ie - f.is + 1, n2 = f.je - f.js + 1, n3 = f.ke - f.ks + 1;
+    // receiving face extents that the store loops walk (a over l2, b over m2)
+    int ra = 0, rb = 0;
+    switch (f.face) {
+    case 1: ra = n3; rb = n2; break;
+    case 2: ra = n2; rb = n3; break;
+    case 3: ra = n1; rb = n3; break;
+    case 4: ra = n3; rb = n1; break;
+    case 5: ra = n2; rb = n1; break;
+    case 6: ra = n1; rb = n2; break;
+    default: return false;
+    }
+    if (ra != l2 || rb != m2 || f.ori < 1 || f.ori > 4) return false;
+    out.clear();
+    for (int b = 1; b <= m2; b++)
+        for (int a = 1; a <= l2; a++) {
+            int ll = 0, mm = 0;
+            switch (f.ori) {
+            case 1: ll = a; mm = m1 - b + 1; break;
+            case 2: ll = b; mm = a; break;
+            case 3: ll = l1 - a + 1; mm = b; break;
+            case 4: ll = l1 - b + 1; mm = m1 - a + 1; break;
+            }
+            PairNode p;
+            int sp[3];
+            ua(ll, mm, sp);
+            p.si = sp[0]; p.sj = sp[1]; p.sk = sp[2];
+            switch (f.face) {
+            case 1: p.ri = f.is; p.rj = f.js + b - 1; p.rk = f.ks + a - 1; break;
+            case 2: p.ri = f.ie; p.rj = f.js + a - 1; p.rk = f.ks + b - 1; break;
+            case 3: p.ri = f.is + a - 1; p.rj = f.js; p.rk = f.ks + b - 1; break;
+            case 4: p.ri = f.is + b - 1; p.rj = f.je; p.rk = f.ks + a - 1; break;
+            case 5: p.ri = f.is + b - 1; p.rj = f.js + a - 1; p.rk = f.ks; break;
+            case 6: p.ri = f.is + a - 1; p.rj = f.js + b - 1; p.rk = f.ke; break;
+            }
+            out.push_back(p);
+        }
+    return true;
+}
+
+void inward(int face, int &di, int &dj, int &dk)
+{
+    int axis = (face - 1) / 2, s = (face & 1) ? 1 : -1;
+    di = axis == 0 ? s : 0;
+    dj = axis == 1 ? s : 0;
+    dk = axis == 2 ? s : 0;
+}
+
+int alloc_block(acfd_ctx *ctx, Block &b)
+{
+    const Ext &e = b.e;
+    const bool sa = ctx->c.turb == 2 && ctx->c.solver == 0;
+    const bool bdf = ctx->cfg.iflag_timeadvance == 3;
+    // field table: pointer slot and component count
+    struct Slot {
+        double **p;
+        int nc;
+    } slots[] = {
+        {&b.f.q, 5}, {&b.f.pri, 7}, {&b.f.rhsi, 5}, {&b.f.rhsv, 5}, {&b.f.rhs, 5}, {&b.f.dxidx, 9},
+        {&b.f.inv_j, 1}, {&b.f.amu, 3}, {&b.f.dt, 1}, {&b.f.dudx, 9}, {&b.f.tao, 6}, {&b.f.vor, 1},
+        {&b.f.aal, sa ? 3 : 0}, {&b.f.bbe, sa ? 3 : 0}, {&b.f.gga, sa ? 3 : 0}, {&b.f.dist, sa ? 1 : 0},
+        {&b.f.spacing, sa ? 1 : 0}, {&b.f.length, sa ? 1 : 0}, {&b.f.nut, 1}, {&b.f.sa_rhs, 1},
+        {&b.f.q0, bdf ? 10 : 0}, {&b.f.w_flux, 12}, {&b.f.w_dqs, 5}, {&b.f.w_dq, 5}, {&b.f.w_diag, 1},
+        {&b.f.w_r, 3}, {&b.f.w_sa, sa ? 9 : 0},
+    };
+    long tot = 0;
+    for (auto &s : slots) tot += (long)s.nc * e.n;
+    HIP_TRY(hipMalloc((void **)&b.arena, tot * sizeof(double)));
+    HIP_TRY(hipMemsetAsync(b.arena, 0, tot * sizeof(double), ctx->stream));
+    long off = 0;
+    for (auto &s : slots) {
+        *s.p = s.nc ? b.arena + off : nullptr;
+        off += (long)s.nc * e.n;
+    }
+    return 0;
+}
+
+double *field_ptr(Block &b, int field)
+{
+    switch (field) {
+    case ACFD_Q: return b.f.q;
+    case ACFD_PRI_V: return b.f.pri;
+    case ACFD_RHSI: return b.f.rhsi;
+    case ACFD_RHSV: return b.f.rhsv;
+    case ACFD_RHS: return b.f.rhs;
+    case ACFD_DUDX: return b.f.dudx;
+    case ACFD_TAO: return b.f.tao;
+    case ACFD_VOR: return b.f.vor;
+    case ACFD_AMU: return b.f.amu;
+    case ACFD_DT: return b.f.dt;
+    case ACFD_DXIDX: return b.f.dxidx;
+    case ACFD_INV_J: return b.f.inv_j;
+    case ACFD_AALPHA: return b.f.aal;
+    case ACFD_BBETA: return b.f.bbe;
+    case ACFD_GGAMMA: return b.f.gga;
+    case ACFD_DIST: return b.f.dist;
+    case ACFD_SPACING: return b.f.spacing;
+    case ACFD_NUT: return b.f.nut;
+    case ACFD_SA_RHS: return b.f.sa_rhs;
+    case ACFD_Q0: return b.f.q0;
+    case ACFD_LENGTH: return b.f.length;
+    }
+    return nullptr;
+}
+
+int ensure_stage(acfd_ctx *ctx, long n)
+{
+    if (ctx->stage_n >= n) return 0;
+    if (ctx->d_stage) (void)hipFree(ctx->d_stage);
+    ctx->d_stage = nullptr;
+    HIP_TRY(hipMalloc((void **)&ctx->d_stage, n * sizeof(double)));
+    ctx->stage_n = n;
+    return 0;
+}
+
+constexpr size_t kEventPairs = 2048;
+
+void timing_flush(acfd_ctx *ctx)
+{
+    if (!ctx->ev_used) return;
+    (void)hipStreamSynchronize(ctx->stream);
+    for (size_t x = 0; x < ctx->ev_used; x++) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, ctx->ev_pool[2 * x], ctx->ev_pool[2 * x + 1]) == hipSuccess) {
+            ctx->stage_ms[ctx->ev_stage[x]] += ms;
+            ctx->stage_calls[ctx->ev_stage[x]] += 1;
+        }
+    }
+    ctx->ev_used = 0;
+}
+
+// Records an event pair around a stage on the context's stream (no wait).
+struct StageTimer {
+    acfd_ctx *ctx;
+    size_t slot = 0;
+    bool on;
+    StageTimer(acfd_ctx *c, int s) : ctx(c), on(c->timing)
+    {
+        if (!on) return;
+        if (ctx->ev_pool.empty()) {
+            ctx->ev_pool.resize(2 * kEventPairs);
+            ctx->ev_stage.resize(kEventPairs);
+            for (auto &ev : ctx->ev_pool) (void)hipEventCreate(&ev);
+        }
+        if (ctx->ev_used == kEventPairs) timing_flush(ctx);
+        slot = ctx->ev_used++;
+        ctx->ev_stage[slot] = s;
+        (void)hipEventRecord(ctx->ev_pool[2 * slot], ctx->stream);
+    }
+    ~StageTimer()
+    {
+        if (on) (void)hipEventRecord(ctx->ev_pool[2 * slot + 1], ctx->stream);
+    }
+};
+
+int check_nneg(acfd_ctx *ctx)
+{
+    unsigned int h = 0;
+    HIP_TRY(hipMemcpyAsync(&h, ctx->d_nneg, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (h) {
+        HIP_TRY(hipMemsetAsync(ctx->d_nneg, 0, sizeof(unsigned int), ctx->stream));
+        return fail(ACFD_ENUMERIC, "negative density found at " + std::to_string(h) + " nodes");
+    }
+    return 0;
+}
+
+// ---------------------------------------------------------------- stages
+int st_getpri(acfd_ctx *ctx, int m)
+{
+    StageTimer t(ctx, ACFD_STAGE_GETPRI);
+    KRN_TRY(launch_getpri(ctx->stream, ctx->blk[m].e, ctx->blk[m].f, ctx->c, ctx->d_nneg));
+    return 0;
+}
+
+int st_inviscid(acfd_ctx *ctx, int m)
+{
+    StageTimer t(ctx, ACFD_STAGE_INVISCID);
+    KRN_TRY(launch_inviscid(ctx->stream, ctx->blk[m].e, ctx->blk[m].f, ctx->c, ctx->cfg.precision));
+    return 0;
+}
+
+int st_viscous(acfd_ctx *ctx, int m, int store_aux)
+{
+    StageTimer t(ctx, ACFD_STAGE_VISCOUS);
+    KRN_TRY(launch_viscous(ctx->stream, ctx->blk[m].e, ctx->blk[m].f, ctx->c, ctx->cfg.precision, store_aux));
+    return 0;
+}
+
+int st_find_dt(acfd_ctx *ctx, int m)
+{
+    StageTimer t(ctx, ACFD_STAGE_FIND_DT);
+    if (ctx->c.itime == 0) KRN_TRY(launch_reset_stats(ctx->stream, ctx->d_stats));
+    KRN_TRY(launch_find_dt(ctx->stream, ctx->blk[m].e, ctx->blk[m].f, ctx->c, ctx->d_cfl, ctx->d_stats));
+    return 0;
+}
+
+// sub-stage timing ids of the two LU-SGS sweeps
+constexpr int kTimeLuForward = 12, kTimeLuBackward = 13;
+
+int st_implicit(acfd_ctx *ctx, int m)
+{
+    StageTimer t(ctx, ACFD_STAGE_IMPLICIT1ST);
+    const Block &b = ctx->blk[m];
+    for (int ph = 0; ph < 4; ph++) {
+        StageTimer ts(ctx, ph == 1 ? kTimeLuForward : ph == 2 ? kTimeLuBackward : 15);
+        KRN_TRY(launch_lusgs_phase(ctx->stream, b.e, b.f, ctx->c, ctx->d_cfl, ctx->cfg.lusgs_kernel,
+                                   ctx->d_tile_flags, ph));
+    }
+    return 0;
+}
+
+int st_sa(acfd_ctx *ctx, int m)
+{
+    StageTimer t(ctx, ACFD_STAGE_SA);
+    KRN_TRY(launch_sa(ctx->stream, ctx->blk[m].e, ctx->blk[m].f, ctx->c, ctx->d_cfl));
+    return 0;
+}
+
+int st_crank(acfd_ctx *ctx, int m)
+{
+    StageTimer t(ctx, ACFD_STAGE_CRANK2ND);
+    KRN_TRY(launch_crank(ctx->stream, ctx->blk[m].e, ctx->blk[m].f, ctx->c));
+    return 0;
+}
+
+int st_update_q0(acfd_ctx *ctx, int m)
+{
+    StageTimer t(ctx, ACFD_STAGE_UPDATE_Q0);
+    KRN_TRY(launch_update_q0(ctx->stream, ctx->blk[m].e, ctx->blk[m].f));
+    return 0;
+}
+
+int st_physical_bc(acfd_ctx *ctx)
+{
+    StageTimer t(ctx, ACFD_STAGE_PHYSICAL_BC);
+    for (const BcTask &task : ctx->bc) {
+        Block &b = ctx->blk[task.block];
+        KRN_TRY(launch_bc(ctx->stream, b.e, b.f, ctx->c, task));
+    }
+    if (ctx->c.turb == 2 && ctx->c.solver == 0)
+        for (auto &b : ctx->blk) KRN_TRY(launch_sa_clamp(ctx->stream, b.e, b.f));
+    return 0;
+}
+
+int st_update_buffer(acfd_ctx *ctx)
+{
+    StageTimer t(ctx, ACFD_STAGE_UPDATE_BUFFER);
+    CopyList cl{ctx->halo_n, ctx->d_h_sb, ctx->d_h_db, ctx->d_h_sn, ctx->d_h_dn};
+    KRN_TRY(launch_halo_copy(ctx->stream, cl, ctx->d_qtab, ctx->d_nuttab, ctx->d_ntab, ctx->c.turb == 2));
+    return 0;
+}
+
+int st_average(acfd_ctx *ctx)
+{
+    StageTimer t(ctx, ACFD_STAGE_AVERAGE);
+    KRN_TRY(launch_average(ctx->stream, ctx->avg, ctx->d_qtab, ctx->d_nuttab, ctx->d_ntab, ctx->d_a_buf,
+                           ctx->c.turb == 2));
+    return 0;
+}
+
+int refresh(acfd_ctx *ctx)
+{
+    int rc = st_physical_bc(ctx);
+    if (!rc) rc = st_update_buffer(ctx);
+    if (!rc) rc = st_average(ctx);
+    return rc;
+}
+
+int residual_stage(acfd_ctx *ctx, int m, int store_aux)
+{
+    int rc = st_getpri(ctx, m);
+    if (!rc) rc = st_inviscid(ctx, m);
+    if (!rc && ctx->c.solver == 0) rc = st_viscous(ctx, m, store_aux);
+    if (!rc) rc = st_find_dt(ctx, m);
+    return rc;
+}
+
+int launch_residual_all(acfd_ctx *ctx)
+{
+    int parts = 0;
+    for (auto &b : ctx->blk) {
+        int p = residual_partials(b.e);
+        KRN_TRY(launch_residual(ctx->stream, b.e, b.f, ctx->d_partial + 6 * parts,
+                                ctx->c.turb == 2 && ctx->c.solver == 0));
+        parts += p;
+    }
+    KRN_TRY(launch_residual_final(ctx->stream, ctx->d_partial, parts, ctx->d_res));
+    return 0;
+}
+
+int one_step(acfd_ctx *ctx)
+{
+    const bool sa = ctx->c.turb == 2 && ctx->c.solver == 0;
+    const int aux = sa ? (ctx->c.des == 2 ? 2 : 1) : 0;
+    const int nb = (int)ctx->blk.size();
+    int rc = 0;
+    if (ctx->cfg.iflag_timeadvance == 2) {
+        for (int m = 0; m < nb && !rc; m++) {
+            rc = residual_stage(ctx, m, aux);
+            if (!rc) rc = st_implicit(ctx, m);
+            if (!rc && sa) rc = st_sa(ctx, m);
+        }
+        if (!rc) rc = refresh(ctx);
+        for (int m = 0; m < nb && !rc; m++) rc = st_getpri(ctx, m);
+    } else if (ctx->cfg.iflag_timeadvance == 3) {
+        for (int sub = 0; sub < 3 && !rc; sub++) {
+            rc = refresh(ctx);
+            for (int m = 0; m < nb && !rc; m++) {
+                rc = residual_stage(ctx, m, aux);
+                if (!rc) rc = st_crank(ctx, m);
+            }
+        }
+        for (int m = 0; m < nb && !rc; m++) rc = st_update_q0(ctx, m);
+        for (int m = 0; m < nb && !rc && sa; m++) rc = st_sa(ctx, m);
+        if (!rc) rc = refresh(ctx);
+        for (int m = 0; m < nb && !rc; m++) rc = st_getpri(ctx, m);
+    } else {
+        return fail(ACFD_EINVAL, "iflag_timeadvance must be 2 or 3");
+    }
+    if (!rc) rc = launch_residual_all(ctx) ? ACFD_EDEVICE : 0;
+    return rc;
+}
+
+}  // namespace
+
+// ===================================================================== C ABI
+extern "C" {
+
+const char *acfd_last_error(void) { return g_err.c_str(); }
+
+int acfd_create(const acfd_config *cfg, acfd_ctx **out)
+{
+    if (!cfg || !out) return fail(ACFD_EINVAL, "acfd_create: null argument");
+    if (cfg->ma <= 0.0) return fail(ACFD_EINVAL, "acfd_create: ma must be positive");
+    if (cfg->iflag_timeadvance != 2 && cfg->iflag_timeadvance != 3)
+        return fail(ACFD_EINVAL, "acfd_create: iflag_timeadvance must be 2 (1st implicit) or 3 (2nd crank)");
+    if (cfg->iflag_turbulence != 0 && cfg->iflag_turbulence != 2)
+        return fail(ACFD_EINVAL, "acfd_create: wrong turbulence model type");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
+        return fail(ACFD_EDEVICE, "acfd_create: no HIP device");
+    acfd_ctx *ctx = new acfd_ctx();
+    ctx->cfg = *cfg;
+    // non-dimensional freestream, sub_init_flowfield.f90:30-39
+    const double gamma = 1.4, pi = 3.1415926535897932;
+    Consts &c = ctx->c;
+    c.gamma = gamma;
+    c.d0 = 1.0;
+    c.u0 = 1.0;
+    c.p0 = 1.0 * 1.0 / gamma / cfg->ma / cfg->ma;
+    c.c0 = std::sqrt(1.0) / cfg->ma;
+    c.cv = 1.0 / gamma / (gamma - 1.0) / cfg->ma / cfg->ma;
+    c.re = cfg->re;
+    c.tinf = cfg->tinf;
+    c.prl = cfg->prl > 0 ? cfg->prl : 0.72;
+    c.prt = cfg->prt > 0 ? cfg->prt : 0.90;
+    c.aoa = cfg->aoa_deg * pi / 180.0;
+    c.solver = cfg->iflag_solver;
+    c.itime = cfg->iflag_time;
+    c.turb = cfg->iflag_turbulence;
+    c.des = cfg->iflag_des;
+    auto bad = [&](const std::string &m) {
+        acfd_destroy(ctx);
+        return fail(ACFD_EDEVICE, m);
+    };
+    if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) return bad("stream");
+    if (hipMalloc((void **)&ctx->d_cfl, sizeof(double)) != hipSuccess) return bad("alloc");
+    if (hipMalloc((void **)&ctx->d_stats, 2 * sizeof(unsigned long long)) != hipSuccess) return bad("alloc");
+    if (hipMalloc((void **)&ctx->d_nneg, sizeof(unsigned int)) != hipSuccess) return bad("alloc");
+    if (hipMalloc((void **)&ctx->d_res, 8 * sizeof(double)) != hipSuccess) return bad("alloc");
+    double cfl = cfg->cfl;
+    if (hipMemcpy(ctx->d_cfl, &cfl, sizeof(double), hipMemcpyHostToDevice) != hipSuccess) return bad("copy");
+    if (hipMemset(ctx->d_nneg, 0, sizeof(unsigned int)) != hipSuccess) return bad("memset");
+    if (hipMemset(ctx->d_res, 0, 8 * sizeof(double)) != hipSuccess) return bad("memset");
+    unsigned long long st[2] = {0ull, ~0ull};
+    if (hipMemcpy(ctx->d_stats, st, sizeof(st), hipMemcpyHostToDevice) != hipSuccess) return bad("copy");
+    *out = ctx;
+    return 0;
+}
+
+int acfd_destroy(acfd_ctx *ctx)
+{
+    if (!ctx) return 0;
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    for (auto &b : ctx->blk) free_block(b);
+    void *ptrs[] = {ctx->d_cfl, ctx->d_stats, ctx->d_nneg, ctx->d_res, ctx->d_partial, ctx->d_tile_flags,
+                    ctx->d_stage, ctx->d_qtab, ctx->d_nuttab, ctx->d_ntab, ctx->d_h_sb, ctx->d_h_db,
+                    ctx->d_h_sn, ctx->d_h_dn, ctx->d_a_sb, ctx->d_a_gb, ctx->d_a_gs, ctx->d_a_sn,
+                    ctx->d_a_gd, ctx->d_a_buf};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    for (auto &ev : ctx->ev_pool) (void)hipEventDestroy(ev);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return 0;
+}
+
+int acfd_set_topology(acfd_ctx *ctx, int nblk, const int *dims, const int *nsub, const acfd_subface *subs)
+{
+    if (!ctx || nblk <= 0 || !dims || !nsub) return fail(ACFD_EINVAL, "acfd_set_topology: bad arguments");
+    if (ctx->topo) return fail(ACFD_ESTATE, "acfd_set_topology: topology already set");
+    ctx->blk.resize(nblk);
+    long off = 0;
+    ctx->totalnodes = 0;
+    for (int m = 0; m < nblk; m++) {
+        int ni = dims[3 * m], nj = dims[3 * m + 1], nk = dims[3 * m + 2];
+        if (ni < 4 || nj < 4 || nk < 4) return fail(ACFD_EINVAL, "acfd_set_topology: blocks need >= 4 nodes per direction");
+        Block &b = ctx->blk[m];
+        b.sub.assign(subs + off, subs + off + nsub[m]);
+        off += nsub[m];
+        block_ranges(ni, nj, nk, b.sub.data(), nsub[m], b.e);
+        ctx->totalnodes += (long)ni * nj * nk;
+    }
+    // validate interfaces and faces (main.f90:655-690: no face mixes
+    // physical and interface subfaces)
+    for (int m = 0; m < nblk; m++) {
+        int phys[7] = {0}, intf[7] = {0};
+        const Ext &e = ctx->blk[m].e;
+        for (auto &s : ctx->blk[m].sub) {
+            if (s.face < 1 || s.face > 6) return fail(ACFD_EINVAL, "acfd_set_topology: face must be 1..6");
+            // the subface must lie on its face inside the physical range
+            bool ok = s.is >= e.is0 && s.is <= s.ie && s.ie <= e.ie0 && s.js >= e.js0 && s.js <= s.je &&
+                      s.je <= e.je0 && s.ks >= e.ks0 && s.ks <= s.ke && s.ke <= e.ke0;
+            switch (s.face) {
+            case 1: ok = ok && s.is == e.is0 && s.ie == e.is0; break;
+            case 2: ok = ok && s.is == e.ie0 && s.ie == e.ie0; break;
+            case 3: ok = ok && s.js == e.js0 && s.je == e.js0; break;
+            case 4: ok = ok && s.js == e.je0 && s.je == e.je0; break;
+            case 5: ok = ok && s.ks == e.ks0 && s.ke == e.ks0; break;
+            case 6: ok = ok && s.ks == e.ke0 && s.ke == e.ke0; break;
+            }
+            if (!ok)
+                return fail(ACFD_EINVAL, "acfd_set_topology: subface " + std::to_string(s.f_no) + " of block " +
+                                             std::to_string(m + 1) + " does not lie on its face");
+            if (s.blk_t < 0) phys[s.face] = 1;
+            if (s.blk_t > 0) {
+                intf[s.face] = 1;
+                if (s.blk_t > nblk || s.subface_t < 1 || s.subface_t > (int)ctx->blk[s.blk_t - 1].sub.size())
+                    return fail(ACFD_EINVAL, "acfd_set_topology: interface names a missing block/subface");
+            }
+        }
+        for (int f = 1; f <= 6; f++)
+            if (phys[f] && intf[f])
+                return fail(ACFD_EINVAL, "acfd_set_topology: block " + std::to_string(m + 1) +
+                                             " mixes physical and interface subfaces on one face");
+    }
+    for (auto &b : ctx->blk) {
+        int rc = alloc_block(ctx, b);
+        if (rc) return rc;
+    }
+    // halo copy list: every (sender inner layer n, receiver halo layer n)
+    // node pair, n = 1, 2 (sub_bc_updatebuffer.f90:57-135, :398-539)
+    std::vector<PairNode> pn;
+    ctx->h_sb.clear(); ctx->h_db.clear(); ctx->h_sn.clear(); ctx->h_dn.clear(); ctx->h_sub.clear();
+    std::vector<int> a_sb;
+    std::vector<long> a_sn;
+    std::vector<long> a_key;  // (block, node) of the receiving face node
+    std::vector<int> a_blk;
+    int gsub = 0;
+    for (int m = 0; m < nblk; m++) {
+        for (auto &s : ctx->blk[m].sub) {
+            int this_sub = gsub++;
+            if (s.blk_t <= 0) continue;
+            if (!pair_nodes(ctx->blk, s, pn))
+                return fail(ACFD_EINVAL, "acfd_set_topology: interface faces of block " + std::to_string(m + 1) +
+                                             " do not match in size for their ori");
+            const Block &sb = ctx->blk[s.blk_t - 1];
+            const acfd_subface &t = sb.sub[s.subface_t - 1];
+            int sdi, sdj, sdk, rdi, rdj, rdk;
+            inward(t.face, sdi, sdj, sdk);
+            inward(s.face, rdi, rdj, rdk);
+            const Ext &re = ctx->blk[m].e, &se = sb.e;
+            for (auto &p : pn) {
+                for (int nbuf = 1; nbuf <= 2; nbuf++) {
+                    ctx->h_sb.push_back(s.blk_t - 1);
+                    ctx->h_db.push_back(m);
+                    ctx->h_sn.push_back(se.ix(p.si + nbuf * sdi, p.sj + nbuf * sdj, p.sk + nbuf * sdk));
+                    ctx->h_dn.push_back(re.ix(p.ri - nbuf * rdi, p.rj - nbuf * rdj, p.rk - nbuf * rdk));
+                    ctx->h_sub.push_back(this_sub);
+                }
+                a_sb.push_back(s.blk_t - 1);
+                a_sn.push_back(se.ix(p.si, p.sj, p.sk));
+                a_blk.push_back(m);
+                a_key.push_back(re.ix(p.ri, p.rj, p.rk));
+            }
+        }
+    }
+    ctx->halo_n = (int)ctx->h_sb.size();
+    int rc = 0;
+    if ((rc = dev_upload(&ctx->d_h_sb, ctx->h_sb)) || (rc = dev_upload(&ctx->d_h_db, ctx->h_db)) ||
+        (rc = dev_upload(&ctx->d_h_sn, ctx->h_sn)) || (rc = dev_upload(&ctx->d_h_dn, ctx->h_dn)))
+        return rc;
+    // average list grouped by receiving node; entries keep subface order
+    std::vector<int> order(a_sb.size());
+    for (size_t x = 0; x < order.size(); x++) order[x] = (int)x;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+        if (a_blk[a] != a_blk[b]) return a_blk[a] < a_blk[b];
+        return a_key[a] < a_key[b];
+    });
+    std::vector<int> e_sb, g_blk, g_start;
+    std::vector<long> e_sn, g_dst;
+    for (size_t x = 0; x < order.size(); x++) {
+        int o = order[x];
+        if (x == 0 || a_blk[o] != g_blk.back() || a_key[o] != g_dst.back()) {
+            g_blk.push_back(a_blk[o]);
+            g_dst.push_back(a_key[o]);
+            g_start.push_back((int)x);
+        }
+        e_sb.push_back(a_sb[o]);
+        e_sn.push_back(a_sn[o]);
+    }
+    g_start.push_back((int)order.size());
+    if ((rc = dev_upload(&ctx->d_a_sb, e_sb)) || (rc = dev_upload(&ctx->d_a_sn, e_sn)) ||
+        (rc = dev_upload(&ctx->d_a_gb, g_blk)) || (rc = dev_upload(&ctx->d_a_gd, g_dst)) ||
+        (rc = dev_upload(&ctx->d_a_gs, g_start)))
+        return rc;
+    if (!order.empty()) HIP_TRY(hipMalloc((void **)&ctx->d_a_buf, order.size() * 6 * sizeof(double)));
+    ctx->avg.count = (int)order.size();
+    ctx->avg.src_blk = ctx->d_a_sb;
+    ctx->avg.src = ctx->d_a_sn;
+    ctx->avg.ngroups = (int)g_blk.size();
+    ctx->avg.grp_blk = ctx->d_a_gb;
+    ctx->avg.grp_dst = ctx->d_a_gd;
+    ctx->avg.grp_start = ctx->d_a_gs;
+    // physical boundary tasks in block/subface order (sub_bc.f90:26-62);
+    // farfield runs with the block's js in place of the subface's js
+    // (sub_bc.f90:57-59)
+    ctx->bc.clear();
+    for (int m = 0; m < nblk; m++)
+        for (auto &s : ctx->blk[m].sub) {
+            if (s.blk_t >= 0) continue;
+            if (s.blk_t != -10 && s.blk_t != -20 && s.blk_t != -30 && s.blk_t != -40 && s.blk_t != -50) continue;
+            BcTask t{m, s.face, s.blk_t, s.is, s.ie, s.js, s.je, s.ks, s.ke};
+            if (s.blk_t == -10) t.js = ctx->blk[m].e.js;
+            ctx->bc.push_back(t);
+        }
+    // pointer tables
+    std::vector<double *> qt(nblk), nt(nblk);
+    std::vector<long> ntab(nblk);
+    long maxn = 0;
+    int parts = 0;
+    long tiles = 0;
+    for (int m = 0; m < nblk; m++) {
+        qt[m] = ctx->blk[m].f.q;
+        nt[m] = ctx->blk[m].f.nut;
+        ntab[m] = ctx->blk[m].e.n;
+        maxn = std::max(maxn, ctx->blk[m].e.n);
+        parts += residual_partials(ctx->blk[m].e);
+        tiles = std::max(tiles, lusgs_tile_flags_needed(ctx->blk[m].e));
+    }
+    if ((rc = dev_upload(&ctx->d_qtab, qt)) || (rc = dev_upload(&ctx->d_nuttab, nt)) ||
+        (rc = dev_upload(&ctx->d_ntab, ntab)))
+        return rc;
+    HIP_TRY(hipMalloc((void **)&ctx->d_partial, (size_t)parts * 6 * sizeof(double)));
+    HIP_TRY(hipMalloc((void **)&ctx->d_tile_flags, (size_t)tiles * sizeof(int)));
+    HIP_TRY(hipMemsetAsync(ctx->d_tile_flags, 0, (size_t)tiles * sizeof(int), ctx->stream));
+    ctx->tile_flags_n = tiles;
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    ctx->topo = true;
+    return 0;
+}
+
+int acfd_block_ranges(acfd_ctx *ctx, int m, int *r)
+{
+    if (!ctx || !ctx->topo || m < 1 || m > (int)ctx->blk.size() || !r)
+        return fail(ACFD_EINVAL, "acfd_block_ranges: bad block");
+    const Ext &e = ctx->blk[m - 1].e;
+    int v[18] = {e.is, e.ie, e.js, e.je, e.ks, e.ke, e.is0, e.ie0, e.js0, e.je0, e.ks0, e.ke0,
+                 e.is1, e.ie1, e.js1, e.je1, e.ks1, e.ke1};
+    std::memcpy(r, v, sizeof(v));
+    return 0;
+}
+
+int acfd_upload(acfd_ctx *ctx, int m, int field, const double *host)
+{
+    if (!ctx || !ctx->topo || m < 1 || m > (int)ctx->blk.size() || !host || field < 1 || field > 21)
+        return fail(ACFD_EINVAL, "acfd_upload: bad arguments");
+    Block &b = ctx->blk[m - 1];
+    double *dst = field_ptr(b, field);
+    if (!dst) return fail(ACFD_EINVAL, "acfd_upload: field not allocated for this configuration");
+    int nc = kNcomp[field];
+    long tot = b.e.n * nc;
+    int rc = ensure_stage(ctx, tot);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpyAsync(ctx->d_stage, host, tot * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    KRN_TRY(launch_aos_to_soa(ctx->stream, ctx->d_stage, dst, b.e.n, nc));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+int acfd_download(acfd_ctx *ctx, int m, int field, double *host)
+{
+    if (!ctx || !ctx->topo || m < 1 || m > (int)ctx->blk.size() || !host || field < 1 || field > 21)
+        return fail(ACFD_EINVAL, "acfd_download: bad arguments");
+    Block &b = ctx->blk[m - 1];
+    double *src = field_ptr(b, field);
+    if (!src) return fail(ACFD_EINVAL, "acfd_download: field not allocated for this configuration");
+    int nc = kNcomp[field];
+    long tot = b.e.n * nc;
+    int rc = ensure_stage(ctx, tot);
+    if (rc) return rc;
+    KRN_TRY(launch_soa_to_aos(ctx->stream, src, ctx->d_stage, b.e.n, nc));
+    HIP_TRY(hipMemcpyAsync(host, ctx->d_stage, tot * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+int acfd_run_stage(acfd_ctx *ctx, int stage)
+{
+    if (!ctx || !ctx->topo) return fail(ACFD_ESTATE, "acfd_run_stage: no topology");
+    const int nb = (int)ctx->blk.size();
+    int rc = 0;
+    switch (stage) {
+    case ACFD_STAGE_GETPRI:
+        for (int m = 0; m < nb && !rc; m++) rc = st_getpri(ctx, m);
+        if (!rc) rc = check_nneg(ctx);
+        break;
+    case ACFD_STAGE_INVISCID:
+        for (int m = 0; m < nb && !rc; m++) rc = st_inviscid(ctx, m);
+        break;
+    case ACFD_STAGE_VISCOUS:
+        for (int m = 0; m < nb && !rc; m++) rc = st_viscous(ctx, m, 2);
+        break;
+    case ACFD_STAGE_FIND_DT:
+        for (int m = 0; m < nb && !rc; m++) rc = st_find_dt(ctx, m);
+        break;
+    case ACFD_STAGE_IMPLICIT1ST:
+        for (int m = 0; m < nb && !rc; m++) rc = st_implicit(ctx, m);
+        break;
+    case ACFD_STAGE_SA:
+        if (!(ctx->c.turb == 2 && ctx->c.solver == 0)) return fail(ACFD_EINVAL, "SA stage needs NS + SA");
+        for (int m = 0; m < nb && !rc; m++) rc = st_sa(ctx, m);
+        break;
+    case ACFD_STAGE_PHYSICAL_BC: rc = st_physical_bc(ctx); break;
+    case ACFD_STAGE_UPDATE_BUFFER: rc = st_update_buffer(ctx); break;
+    case ACFD_STAGE_AVERAGE: rc = st_average(ctx); break;
+    case ACFD_STAGE_CRANK2ND:
+        if (ctx->cfg.iflag_timeadvance != 3) return fail(ACFD_EINVAL, "crank stage needs iflag_timeadvance = 3");
+        for (int m = 0; m < nb && !rc; m++) rc = st_crank(ctx, m);
+        break;
+    case ACFD_STAGE_UPDATE_Q0:
+        if (ctx->cfg.iflag_timeadvance != 3) return fail(ACFD_EINVAL, "q0 stage needs iflag_timeadvance = 3");
+        for (int m = 0; m < nb && !rc; m++) rc = st_update_q0(ctx, m);
+        break;
+    default: return fail(ACFD_EINVAL, "acfd_run_stage: unknown stage");
+    }
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+int acfd_prepare(acfd_ctx *ctx)
+{
+    if (!ctx || !ctx->topo) return fail(ACFD_ESTATE, "acfd_prepare: no topology");
+    int rc = refresh(ctx);
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+int acfd_step(acfd_ctx *ctx, int nsteps)
+{
+    if (!ctx || !ctx->topo) return fail(ACFD_ESTATE, "acfd_step: no topology");
+    for (int s = 0; s < nsteps; s++) {
+        int rc = one_step(ctx);
+        if (rc) return rc;
+    }
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (ctx->cfg.lusgs_kernel == 1 && ctx->d_tile_flags) {
+        int err = 0;
+        HIP_TRY(hipMemcpy(&err, ctx->d_tile_flags + (ctx->tile_flags_n - 1), sizeof(int), hipMemcpyDeviceToHost));
+        if (err) return fail(ACFD_EDEVICE, "acfd_step: pipelined LU-SGS sweep timed out waiting for a tile");
+    }
+    return check_nneg(ctx);
+}
+
+int acfd_residual(acfd_ctx *ctx, double *out)
+{
+    if (!ctx || !ctx->topo || !out) return fail(ACFD_EINVAL, "acfd_residual: bad arguments");
+    int rc = launch_residual_all(ctx);
+    if (rc) return rc;
+    double s[6];
+    HIP_TRY(hipMemcpyAsync(s, ctx->d_res, sizeof(s), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    for (int v = 0; v < 6; v++) out[v] = std::sqrt(s[v] / (double)ctx->totalnodes);
+    out[6] = 0.0;
+    return 0;
+}
+
+int acfd_stats(acfd_ctx *ctx, double *out)
+{
+    if (!ctx || !out) return fail(ACFD_EINVAL, "acfd_stats: bad arguments");
+    unsigned long long st[2];
+    double cfl;
+    HIP_TRY(hipMemcpyAsync(st, ctx->d_stats, sizeof(st), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(&cfl, ctx->d_cfl, sizeof(cfl), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    double mx, mn;
+    std::memcpy(&mx, &st[0], 8);
+    std::memcpy(&mn, &st[1], 8);
+    if (ctx->c.itime == 0) {
+        out[0] = mx; out[1] = st[1] == ~0ull ? 1.0e10 : mn; out[2] = 0.0; out[3] = 1.0e10;
+    } else {
+        out[0] = 0.0; out[1] = 1.0e10; out[2] = cfl; out[3] = cfl;
+    }
+    out[4] = cfl;
+    return 0;
+}
+
+int acfd_sync(acfd_ctx *ctx)
+{
+    if (!ctx) return fail(ACFD_EINVAL, "acfd_sync: null context");
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+int acfd_timing(acfd_ctx *ctx, int enable, double *ms, long *calls)
+{
+    if (!ctx) return fail(ACFD_EINVAL, "acfd_timing: null context");
+    timing_flush(ctx);
+    if (ms)
+        for (int s = 0; s < 16; s++) ms[s] = ctx->stage_ms[s];
+    if (calls)
+        for (int s = 0; s < 16; s++) calls[s] = ctx->stage_calls[s];
+    if (enable >= 0) {
+        ctx->timing = enable != 0;
+        for (int s = 0; s < 16; s++) {
+            ctx->stage_ms[s] = 0;
+            ctx->stage_calls[s] = 0;
+        }
+    }
+    return 0;
+}
+
+int acfd_exchange_coordinates(acfd_ctx *ctx, double **xyz, const double *shift)
+{
+    if (!ctx || !ctx->topo || !xyz) return fail(ACFD_EINVAL, "acfd_exchange_coordinates: bad arguments");
+    // gather first (every pack precedes every unpack), then scatter
+    std::vector<double> val(ctx->h_sb.size() * 3);
+    for (size_t x = 0; x < ctx->h_sb.size(); x++) {
+        const Block &sb = ctx->blk[ctx->h_sb[x]];
+        const double *src = xyz[ctx->h_sb[x]];
+        for (int c = 0; c < 3; c++) val[3 * x + c] = src[c * sb.e.n + ctx->h_sn[x]];
+    }
+    for (size_t x = 0; x < ctx->h_db.size(); x++) {
+        const Block &db = ctx->blk[ctx->h_db[x]];
+        double *dst = xyz[ctx->h_db[x]];
+        for (int c = 0; c < 3; c++) {
+            double s = shift ? shift[3 * ctx->h_sub[x] + c] : 0.0;
+            dst[c * db.e.n + ctx->h_dn[x]] = val[3 * x + c] + s;
+        }
+    }
+    return 0;
+}
+
+}  // extern "C"
+
+// ======================================================= stateless routines
+namespace {
+
+// single-block context with explicit ranges and no interfaces
+int make_single(const acfd_config *cfg, const int *r, acfd_ctx **out)
+{
+    acfd_config c = cfg ? *cfg : acfd_config{};
+    if (!cfg) {
+        c.ma = 0.5; c.re = 1e6; c.tinf = 288.15; c.iflag_timeadvance = 2; c.iflag_time = 1;
+    }
+    if (c.iflag_timeadvance != 3) c.iflag_timeadvance = 2;
+    int rc = acfd_create(&c, out);
+    if (rc) return rc;
+    acfd_ctx *ctx = *out;
+    ctx->blk.resize(1);
+    Ext &e = ctx->blk[0].e;
+    e.is = r[0]; e.ie = r[1]; e.js = r[2]; e.je = r[3]; e.ks = r[4]; e.ke = r[5];
+    e.is0 = r[6]; e.ie0 = r[7]; e.js0 = r[8]; e.je0 = r[9]; e.ks0 = r[10]; e.ke0 = r[11];
+    e.is1 = r[12]; e.ie1 = r[13]; e.js1 = r[14]; e.je1 = r[15]; e.ks1 = r[16]; e.ke1 = r[17];
+    e.ni = e.ie - e.is + 1; e.nj = e.je - e.js + 1; e.nk = e.ke - e.ks + 1;
+    e.n = (long)e.ni * e.nj * e.nk;
+    if (e.ni < 3 || e.nj < 3 || e.nk < 3) {
+        acfd_destroy(ctx);
+        return fail(ACFD_EINVAL, "ranges: block too small");
+    }
+    rc = alloc_block(ctx, ctx->blk[0]);
+    if (rc) {
+        acfd_destroy(ctx);
+        return rc;
+    }
+    std::vector<double *> qt{ctx->blk[0].f.q}, nt{ctx->blk[0].f.nut};
+    std::vector<long> nn{e.n};
+    dev_upload(&ctx->d_qtab, qt);
+    dev_upload(&ctx->d_nuttab, nt);
+    dev_upload(&ctx->d_ntab, nn);
+    HIP_TRY(hipMalloc((void **)&ctx->d_partial, (size_t)residual_partials(e) * 6 * sizeof(double)));
+    ctx->tile_flags_n = lusgs_tile_flags_needed(e);
+    HIP_TRY(hipMalloc((void **)&ctx->d_tile_flags, (size_t)ctx->tile_flags_n * sizeof(int)));
+    HIP_TRY(hipMemset(ctx->d_tile_flags, 0, (size_t)ctx->tile_flags_n * sizeof(int)));
+    ctx->totalnodes = (long)(e.ie0 - e.is0 + 1) * (e.je0 - e.js0 + 1) * (e.ke0 - e.ks0 + 1);
+    ctx->topo = true;
+    return 0;
+}
+
+struct Single {
+    acfd_ctx *ctx = nullptr;
+    ~Single() { acfd_destroy(ctx); }
+};
+
+#define ST_TRY(x)            \
+    do {                     \
+        int r__ = (x);       \
+        if (r__) return r__; \
+    } while (0)
+
+int put(acfd_ctx *c, int f, const double *h)
+{
+    return h ? acfd_upload(c, 1, f, h) : 0;
+}
+
+int set_cfl(acfd_ctx *c, double v)
+{
+    HIP_TRY(hipMemcpy(c->d_cfl, &v, sizeof(v), hipMemcpyHostToDevice));
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int acfd_getpri(const int *r, double gamma, double cv, const double *q, double *pri_v)
+{
+    if (!r || !q || !pri_v) return fail(ACFD_EINVAL, "acfd_getpri: null argument");
+    (void)gamma;
+    acfd_config c{};
+    c.ma = std::sqrt(1.0 / (1.4 * 0.4 * cv));  // cv = 1/(gamma (gamma-1) ma^2)
+    c.re = 1e6; c.tinf = 288.15; c.iflag_timeadvance = 2; c.iflag_time = 1; c.iflag_solver = 1;
+    Single s;
+    ST_TRY(make_single(&c, r, &s.ctx));
+    s.ctx->c.cv = cv;
+    ST_TRY(put(s.ctx, ACFD_Q, q));
+    int rc = acfd_run_stage(s.ctx, ACFD_STAGE_GETPRI);
+    if (rc && rc != ACFD_ENUMERIC) return rc;
+    ST_TRY(acfd_download(s.ctx, 1, ACFD_PRI_V, pri_v));
+    return rc;
+}
+
+int acfd_inviscid_rhs(const acfd_config *cfg, const int *r, const double *q, const double *dxidx,
+                      const double *inv_j, double *rhsi)
+{
+    if (!r || !q || !dxidx || !inv_j || !rhsi) return fail(ACFD_EINVAL, "acfd_inviscid_rhs: null argument");
+    Single s;
+    ST_TRY(make_single(cfg, r, &s.ctx));
+    ST_TRY(put(s.ctx, ACFD_Q, q));
+    ST_TRY(put(s.ctx, ACFD_DXIDX, dxidx));
+    ST_TRY(put(s.ctx, ACFD_INV_J, inv_j));
+    ST_TRY(acfd_run_stage(s.ctx, ACFD_STAGE_INVISCID));
+    return acfd_download(s.ctx, 1, ACFD_RHSI, rhsi);
+}
+
+int acfd_viscous_rhs(const acfd_config *cfg, const int *r, const double *pri_v, double *amu,
+                     const double *dxidx, const double *inv_j, double *rhsv, double *vor, double *dudx,
+                     double *tao)
+{
+    if (!r || !pri_v || !amu || !dxidx || !inv_j || !rhsv) return fail(ACFD_EINVAL, "acfd_viscous_rhs: null argument");
+    Single s;
+    ST_TRY(make_single(cfg, r, &s.ctx));
+    s.ctx->c.solver = 0;
+    ST_TRY(put(s.ctx, ACFD_PRI_V, pri_v));
+    ST_TRY(put(s.ctx, ACFD_AMU, amu));
+    ST_TRY(put(s.ctx, ACFD_DXIDX, dxidx));
+    ST_TRY(put(s.ctx, ACFD_INV_J, inv_j));
+    ST_TRY(acfd_run_stage(s.ctx, ACFD_STAGE_VISCOUS));
+    ST_TRY(acfd_download(s.ctx, 1, ACFD_RHSV, rhsv));
+    ST_TRY(acfd_download(s.ctx, 1, ACFD_AMU, amu));
+    if (vor) ST_TRY(acfd_download(s.ctx, 1, ACFD_VOR, vor));
+    if (dudx) ST_TRY(acfd_download(s.ctx, 1, ACFD_DUDX, dudx));
+    if (tao) ST_TRY(acfd_download(s.ctx, 1, ACFD_TAO, tao));
+    return 0;
+}
+
+int acfd_find_dt(const acfd_config *cfg, const int *r, const double *pri_v, const double *amu,
+                 const double *dxidx, double *dt, double *cfl)
+{
+    if (!cfg || !r || !pri_v || !amu || !dxidx || !dt || !cfl) return fail(ACFD_EINVAL, "acfd_find_dt: null argument");
+    Single s;
+    ST_TRY(make_single(cfg, r, &s.ctx));
+    ST_TRY(put(s.ctx, ACFD_PRI_V, pri_v));
+    ST_TRY(put(s.ctx, ACFD_AMU, amu));
+    ST_TRY(put(s.ctx, ACFD_DXIDX, dxidx));
+    ST_TRY(put(s.ctx, ACFD_DT, dt));
+    ST_TRY(set_cfl(s.ctx, *cfl));
+    ST_TRY(acfd_run_stage(s.ctx, ACFD_STAGE_FIND_DT));
+    ST_TRY(acfd_download(s.ctx, 1, ACFD_DT, dt));
+    HIP_TRY(hipMemcpy(cfl, s.ctx->d_cfl, sizeof(double), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int acfd_lusgs(const acfd_config *cfg, const int *r, double *q, const double *dt, const double *pri_v,
+               const double *rhsi, const double *rhsv, const double *dxidx, const double *inv_j, double cfl,
+               double *rhs)
+{
+    if (!cfg || !r || !q || !dt || !pri_v || !rhsi || !rhsv || !dxidx || !inv_j)
+        return fail(ACFD_EINVAL, "acfd_lusgs: null argument");
+    Single s;
+    ST_TRY(make_single(cfg, r, &s.ctx));
+    ST_TRY(put(s.ctx, ACFD_Q, q));
+    ST_TRY(put(s.ctx, ACFD_DT, dt));
+    ST_TRY(put(s.ctx, ACFD_PRI_V, pri_v));
+    ST_TRY(put(s.ctx, ACFD_RHSI, rhsi));
+    ST_TRY(put(s.ctx, ACFD_RHSV, rhsv));
+    ST_TRY(put(s.ctx, ACFD_DXIDX, dxidx));
+    ST_TRY(put(s.ctx, ACFD_INV_J, inv_j));
+    ST_TRY(set_cfl(s.ctx, cfl));
+    ST_TRY(acfd_run_stage(s.ctx, ACFD_STAGE_IMPLICIT1ST));
+    if (cfg->lusgs_kernel == 1) {
+        int err = 0;
+        HIP_TRY(hipMemcpy(&err, s.ctx->d_tile_flags + (s.ctx->tile_flags_n - 1), sizeof(int), hipMemcpyDeviceToHost));
+        if (err) return fail(ACFD_EDEVICE, "acfd_lusgs: pipelined sweep timed out waiting for a tile");
+    }
+    ST_TRY(acfd_download(s.ctx, 1, ACFD_Q, q));
+    if (rhs) ST_TRY(acfd_download(s.ctx, 1, ACFD_RHS, rhs));
+    return 0;
+}
+
+int acfd_bdf2(const acfd_config *cfg, const int *r, double *q, const double *q0, const double *dt,
+              const double *pri_v, const double *rhsi, const double *rhsv, const double *dxidx,
+              const double *inv_j, double *rhs)
+{
+    if (!cfg || !r || !q || !q0 || !dt || !pri_v || !rhsi || !rhsv || !dxidx || !inv_j)
+        return fail(ACFD_EINVAL, "acfd_bdf2: null argument");
+    acfd_config c = *cfg;
+    c.iflag_timeadvance = 3;
+    Single s;
+    ST_TRY(make_single(&c, r, &s.ctx));
+    ST_TRY(put(s.ctx, ACFD_Q, q));
+    ST_TRY(put(s.ctx, ACFD_Q0, q0));
+    ST_TRY(put(s.ctx, ACFD_DT, dt));
+    ST_TRY(put(s.ctx, ACFD_PRI_V, pri_v));
+    ST_TRY(put(s.ctx, ACFD_RHSI, rhsi));
+    ST_TRY(put(s.ctx, ACFD_RHSV, rhsv));
+    ST_TRY(put(s.ctx, ACFD_DXIDX, dxidx));
+    ST_TRY(put(s.ctx, ACFD_INV_J, inv_j));
+    ST_TRY(acfd_run_stage(s.ctx, ACFD_STAGE_CRANK2ND));
+    ST_TRY(acfd_download(s.ctx, 1, ACFD_Q, q));
+    if (rhs) ST_TRY(acfd_download(s.ctx, 1, ACFD_RHS, rhs));
+    return 0;
+}
+
+int acfd_sa_step(const acfd_config *cfg, const int *r, double *nut, double *sa_rhs, const double *dt,
+                 const double *pri_v, const double *amu, const double *dudx, const double *vor,
+                 const double *dxidx, const double *inv_j, const double *aalpha, const double *bbeta,
+                 const double *ggamma, double *length, const double *dist, const double *spacing, double cfl)
+{
+    if (!cfg || !r || !nut || !sa_rhs || !dt || !pri_v || !amu || !dudx || !vor || !dxidx || !inv_j || !aalpha ||
+        !bbeta || !ggamma || !length || !dist || !spacing)
+        return fail(ACFD_EINVAL, "acfd_sa_step: null argument");
+    acfd_config c = *cfg;
+    c.iflag_solver = 0;
+    c.iflag_turbulence = 2;
+    Single s;
+    ST_TRY(make_single(&c, r, &s.ctx));
+    ST_TRY(put(s.ctx, ACFD_NUT, nut));
+    ST_TRY(put(s.ctx, ACFD_DT, dt));
+    ST_TRY(put(s.ctx, ACFD_PRI_V, pri_v));
+    ST_TRY(put(s.ctx, ACFD_AMU, amu));
+    ST_TRY(put(s.ctx, ACFD_DUDX, dudx));
+    ST_TRY(put(s.ctx, ACFD_VOR, vor));
+    ST_TRY(put(s.ctx, ACFD_DXIDX, dxidx));
+    ST_TRY(put(s.ctx, ACFD_INV_J, inv_j));
+    ST_TRY(put(s.ctx, ACFD_AALPHA, aalpha));
+    ST_TRY(put(s.ctx, ACFD_BBETA, bbeta));
+    ST_TRY(put(s.ctx, ACFD_GGAMMA, ggamma));
+    ST_TRY(put(s.ctx, ACFD_DIST, dist));
+    ST_TRY(put(s.ctx, ACFD_SPACING, spacing));
+    ST_TRY(set_cfl(s.ctx, cfl));
+    ST_TRY(acfd_run_stage(s.ctx, ACFD_STAGE_SA));
+    ST_TRY(acfd_download(s.ctx, 1, ACFD_NUT, nut));
+    ST_TRY(acfd_download(s.ctx, 1, ACFD_SA_RHS, sa_rhs));
+    return acfd_download(s.ctx, 1, ACFD_LENGTH, length);
+}
+
+}  // extern "C"

@@ -1,0 +1,319 @@
+// Spalart-Allmaras kernels: turbulent length scale, convection (first-order
+// upwind), diffusion, production/destruction with the diagonal Jacobian, and
+// the scalar LU-SGS update of nut.
+//
+// Reference: sub_tur_sa.f90:7-825, constants module_tursa.f90:4-11.
+#include "common.hpp"
+#include "kernels.hpp"
+
+namespace acfd {
+
+namespace sa {
+constexpr double dta = 2.0 / 3.0, kar = 0.41;
+constexpr double cb1 = 0.1355, cb2 = 0.622;
+constexpr double cw1 = cb1 / kar / kar + (1.0 + cb2) / dta;
+constexpr double cw2 = 0.3, cw3 = 2.0;
+constexpr double ct3 = 1.2, ct4 = 0.5;
+constexpr double cv1 = 7.1;
+constexpr double cdes = 0.65;
+}  // namespace sa
+
+// Work layout in f.w_sa (n each): 0-2 dnut/dxi,deta,dzeta; 3-5 alpha.grad,
+// beta.grad, gamma.grad; 6 nu + (1+cb2) nut; 7 diasd; 8 dnut*.
+// f.w_dqs holds contravariant velocities (0-2) and diag (3); f.w_dq dnut.
+
+// length (sub_tur_sa.f90:50-100) on the whole extended block
+__global__ void k_sa_length(Ext e, Consts c, const double *__restrict__ dist,
+                            const double *__restrict__ spac, const double *__restrict__ pri,
+                            const double *__restrict__ nut, const double *__restrict__ amu,
+                            const double *__restrict__ dudx, double *__restrict__ length)
+{
+    long n = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= e.n) return;
+    const long N = e.n;
+    double dist0 = dist[n];
+    if (c.des == 1) {
+        length[n] = fmin(dist0, sa::cdes * spac[n]);
+    } else if (c.des == 2) {
+        // dudx is declared dudx(3,is:ie,js:je,ks:ke,3) in sa_model
+        // (sub_tur_sa.f90:29) while the array holds dudx(9,...): element
+        // (a,n,b) of that view is flat element (a-1) + 3n + 3N(b-1) of the
+        // reference layout, i.e. component F%9 of node F/9.
+        auto du = [&](int a, int b) -> double {
+            long F = (long)(a - 1) + 3 * n + 3 * N * (long)(b - 1);
+            return dudx[(F % 9) * N + F / 9];
+        };
+        double uy = du(1, 2), uz = du(1, 3), vx = du(2, 1), vz = du(2, 3), wx = du(3, 1), wy = du(3, 2);
+        double d0 = pri[n], nut0 = nut[n];
+        double nu0 = amu[n] / d0;
+        double spac0 = spac[n];
+        double temp = uy * uy + uz * uz + vx * vx + vz * vz + wx * wx + wy * wy;
+        temp = sqrt(temp);
+        double rd = (nu0 + nut0) / (temp * c.re * (sa::kar * sa::kar) * (dist0 * dist0));
+        double x = 8.0 * rd;
+        double fd = 1.0 - tanh(x * x * x);
+        length[n] = dist0 - fd * fmax(0.0, dist0 - sa::cdes * spac0);
+    } else {
+        length[n] = dist0;
+    }
+}
+
+// one-sided at the physical ends, central inside; the operand is zero
+// outside the computed range when ZO (as the reference's zeroed temporaries)
+template <bool ZO>
+__device__ inline double sa_d(const Ext &e, const double *a, long n, long st, int x, int x0, int x1,
+                              int c0, int c1)
+{
+    auto val = [&](long m, int xm) -> double {
+        if (ZO && (xm < c0 || xm > c1)) return 0.0;
+        return a[m];
+    };
+    if (x == x0) return val(n + st, x + 1) - a[n];
+    if (x == x1) return a[n] - val(n - st, x - 1);
+    return (val(n + st, x + 1) - val(n - st, x - 1)) / 2.0;
+}
+
+// stage A on the computed range: convection (sub_tur_sa.f90:152-208) into
+// sa_rhs, nut gradient (:294-338), alpha/beta/gamma products (:340-356) and
+// nu + (1+cb2) nut (:405-413)
+__global__ void __launch_bounds__(256) k_sa_a(Ext e, Box b, const double *__restrict__ pri,
+                                              const double *__restrict__ dxidx,
+                                              const double *__restrict__ nut,
+                                              const double *__restrict__ amu,
+                                              const double *__restrict__ aal,
+                                              const double *__restrict__ bbe,
+                                              const double *__restrict__ gga,
+                                              double *__restrict__ sa_rhs, double *__restrict__ w)
+{
+    int i, j, k;
+    if (!box_node(b, i, j, k)) return;
+    const long N = e.n;
+    long n = e.ix(i, j, k);
+    const long st[3] = {1, (long)e.ni, (long)e.ni * e.nj};
+    double u = pri[N + n], v = pri[2 * N + n], ww = pri[3 * N + n];
+    double r[3];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        double m0 = dxidx[(3 * d) * N + n], m1 = dxidx[(3 * d + 1) * N + n], m2 = dxidx[(3 * d + 2) * N + n];
+        double uu = m0 * u + m1 * v + m2 * ww;
+        double nutp = nut[n + st[d]], nutc = nut[n], nutm = nut[n - st[d]];
+        r[d] = -(0.5 * (uu + fabs(uu)) * (nutc - nutm) + 0.5 * (uu - fabs(uu)) * (nutp - nutc));
+    }
+    sa_rhs[n] = r[0] + r[1] + r[2];
+    double g0 = sa_d<false>(e, nut, n, st[0], i, e.is0, e.ie0, 0, 0);
+    double g1 = sa_d<false>(e, nut, n, st[1], j, e.js0, e.je0, 0, 0);
+    double g2 = sa_d<false>(e, nut, n, st[2], k, e.ks0, e.ke0, 0, 0);
+    w[n] = g0;
+    w[N + n] = g1;
+    w[2 * N + n] = g2;
+    w[3 * N + n] = aal[n] * g0 + aal[N + n] * g1 + aal[2 * N + n] * g2;
+    w[4 * N + n] = bbe[n] * g0 + bbe[N + n] * g1 + bbe[2 * N + n] * g2;
+    w[5 * N + n] = gga[n] * g0 + gga[N + n] * g1 + gga[2 * N + n] * g2;
+    double nu0 = amu[n] / pri[n];
+    w[6 * N + n] = nu0 + (1.0 + sa::cb2) * nut[n];
+}
+
+// stage B on the computed range: diffusion (sub_tur_sa.f90:359-512) and
+// source with the diagonal Jacobian (:561-623)
+__global__ void __launch_bounds__(256) k_sa_b(Ext e, Box b, Consts c, const double *__restrict__ pri,
+                                              const double *__restrict__ dxidx,
+                                              const double *__restrict__ inv_j,
+                                              const double *__restrict__ nut,
+                                              const double *__restrict__ amu,
+                                              const double *__restrict__ vor,
+                                              const double *__restrict__ length,
+                                              double *__restrict__ sa_rhs, double *__restrict__ w)
+{
+    int i, j, k;
+    if (!box_node(b, i, j, k)) return;
+    const long N = e.n;
+    long n = e.ix(i, j, k);
+    const long st[3] = {1, (long)e.ni, (long)e.ni * e.nj};
+    const double re = c.re;
+    double tempa = sa_d<true>(e, w + 3 * N, n, st[0], i, e.is0, e.ie0, e.is1, e.ie1);
+    double tempb = sa_d<true>(e, w + 4 * N, n, st[1], j, e.js0, e.je0, e.js1, e.je1);
+    double tempc = sa_d<true>(e, w + 5 * N, n, st[2], k, e.ks0, e.ke0, e.ks1, e.ke1);
+    double txi = sa_d<true>(e, w + 6 * N, n, st[0], i, e.is0, e.ie0, e.is1, e.ie1);
+    double teta = sa_d<true>(e, w + 6 * N, n, st[1], j, e.js0, e.je0, e.js1, e.je1);
+    double tzeta = sa_d<true>(e, w + 6 * N, n, st[2], k, e.ks0, e.ke0, e.ks1, e.ke1);
+    double m[9];
+#pragma unroll
+    for (int x = 0; x < 9; x++) m[x] = dxidx[x * N + n];
+    double tx = txi * m[0] + teta * m[3] + tzeta * m[6];
+    double ty = txi * m[1] + teta * m[4] + tzeta * m[7];
+    double tz = txi * m[2] + teta * m[5] + tzeta * m[8];
+    double nut0 = nut[n];
+    double deltanut = (tempa + tempb + tempc) / inv_j[n];
+    double g0 = w[n], g1 = w[N + n], g2 = w[2 * N + n];
+    double dnutdx = g0 * m[0] + g1 * m[3] + g2 * m[6];
+    double dnutdy = g0 * m[1] + g1 * m[4] + g2 * m[7];
+    double dnutdz = g0 * m[2] + g1 * m[5] + g2 * m[8];
+    double temp = w[6 * N + n];
+    double diff = -(sa::cb2 / (re * sa::dta)) * nut0 * deltanut
+                + (1.0 / (re * sa::dta)) * (temp * deltanut + tx * dnutdx + ty * dnutdy + tz * dnutdz);
+    double rhs = sa_rhs[n] + diff;
+
+    // source (r = nut0 * div as in the reference, :579)
+    const double kar2 = sa::kar * sa::kar;
+    const double cv13 = sa::cv1 * sa::cv1 * sa::cv1;
+    const double cw36 = sa::cw3 * sa::cw3 * sa::cw3 * sa::cw3 * sa::cw3 * sa::cw3;
+    double d = pri[n];
+    double nu0 = amu[n] / d;
+    double omega = fabs(vor[n]);
+    double dist0 = length[n];
+    double leix = fmax(nut0 / nu0, 0.001);
+    double leix2 = leix * leix, leix3 = leix * leix * leix;
+    double fv1 = leix3 / (leix3 + cv13);
+    double fv2 = 1.0 - leix / (1.0 + leix * fv1);
+    double dist2 = dist0 * dist0;
+    double s = omega + nut0 * fv2 / (re * kar2 * dist2);
+    s = fmax(s, 0.3 * omega);
+    double div = s * re * kar2 * dist2;
+    div = fmax(div, 1.0e-6);
+    double r = nut0 * div;
+    r = fmin(r, 10.0);
+    double r6 = r * r * r * r * r * r;
+    double g = r + sa::cw2 * (r6 - r);
+    double gg6 = g * g * g * g * g * g;
+    double fw = g * pow((1.0 + cw36) / (gg6 + cw36), 1.0 / 6.0);
+    double ft2 = sa::ct3 * exp(-sa::ct4 * leix2);
+    double aaa = sa::cb1 * ((1.0 - ft2) * fv2 + ft2) / kar2;
+    double bbb = sa::cw1 * fw;
+    double ccc = nut0 / dist2;
+    double sd1 = (aaa - bbb) * ccc / re;
+    double sd = sd1 * nut0;
+    double sp = sa::cb1 * (1.0 - ft2) * omega * nut0;
+    sa_rhs[n] = rhs + sd + sp;
+    double leixn = 1.0 / nu0;
+    double den = leix3 + cv13;
+    double fv1n = (3.0 * leix2 * leixn * den - leix3 * (3.0 * leix2 * leixn)) / (den * den);
+    double den2 = 1.0 + leix * fv1;
+    double fv2n = (leixn * den2 - leix * (leixn * fv1 + leix * fv1n)) / (den2 * den2);
+    double ft2n = -2.0 * sa::ct4 * leix * leixn * ft2;
+    double sn = 1.0 / (re * kar2 * dist2) * (fv2 + fv2n * nut0);
+    div = (s * s) * (re * kar2 * dist2);
+    div = fmax(div, 1.0e-6);
+    double rn = (s - sn * nut0) / div;
+    double r5 = r * r * r * r * r;
+    double gn = (1.0 + sa::cw2 * 6.0 * r5 - sa::cw2) * rn;
+    const double cw3m6 = pow(sa::cw3, -6.0);
+    double fwn = -(1.0 / 6.0) * pow((pow(g, -6.0) + cw3m6) / (1.0 + cw3m6), -7.0 / 6.0);
+    fwn = fwn / (1.0 + cw3m6) * (-6.0 * pow(g, -7.0)) * gn;
+    double spn = sa::cb1 * (1.0 - ft2) * omega + sa::cb1 * (-ft2n) * omega * nut0;
+    double sdn = (1.0 / re) * (sa::cb1 * (fv2 - ft2 * fv2 + ft2) / kar2 - sa::cw1 * fw) * 2.0 * nut0 / dist2;
+    sdn = (1.0 / re) * (sa::cb1 * (fv2n - ft2n * fv2 - ft2 * fv2n + ft2n) / kar2 - sa::cw1 * fwn)
+              * ((nut0 * nut0) / dist2) + sdn;
+    w[7 * N + n] = fmin(spn + sdn, 0.0);
+}
+
+// diag and contravariant velocities of sa_lusgs (sub_tur_sa.f90:719-753)
+// on the ring; dnut* and dnut cleared there.
+__global__ void __launch_bounds__(256) k_sa_prep(Ext e, Box ring, Consts c, const double *__restrict__ pri,
+                                                 const double *__restrict__ dxidx,
+                                                 const double *__restrict__ nut,
+                                                 const double *__restrict__ dt, const double *cfl,
+                                                 double *__restrict__ w, double *__restrict__ cu,
+                                                 double *__restrict__ dnut)
+{
+    int i, j, k;
+    if (!box_node(ring, i, j, k)) return;
+    const long N = e.n;
+    long n = e.ix(i, j, k);
+    double phi = (*cfl >= 1.0e3 && c.itime == 0) ? 0.0 : 1.0;
+    double u = pri[N + n], v = pri[2 * N + n], ww = pri[3 * N + n];
+    double m[9];
+#pragma unroll
+    for (int x = 0; x < 9; x++) m[x] = dxidx[x * N + n];
+    double x2 = m[0] * m[0] + m[1] * m[1] + m[2] * m[2];
+    double c0 = m[0] * u + m[1] * v + m[2] * ww;
+    x2 = x2 + m[3] * m[3] + m[4] * m[4] + m[5] * m[5];
+    double c1 = m[3] * u + m[4] * v + m[5] * ww;
+    x2 = x2 + m[6] * m[6] + m[7] * m[7] + m[8] * m[8];
+    double c2 = m[6] * u + m[7] * v + m[8] * ww;
+    double tm = 4.0 * x2 * nut[n] / c.re;
+    bool inner = i >= e.is1 && i <= e.ie1 && j >= e.js1 && j <= e.je1 && k >= e.ks1 && k <= e.ke1;
+    double ds = inner ? w[7 * N + n] : 0.0;
+    cu[n] = c0;
+    cu[N + n] = c1;
+    cu[2 * N + n] = c2;
+    cu[3 * N + n] = 1.0 / dt[n] * phi + fabs(c0) + fabs(c1) + fabs(c2) + tm - ds;
+    w[8 * N + n] = 0.0;
+    dnut[n] = 0.0;
+}
+
+template <int DIR>
+__global__ void __launch_bounds__(256) k_sa_plane(Ext e, int l, int ja, int ka, const double *__restrict__ cu,
+                                                  const double *__restrict__ sa_rhs,
+                                                  double *__restrict__ dn1, double *__restrict__ dn)
+{
+    int lj = ja + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    int lk = ka + (int)(blockIdx.y * blockDim.y + threadIdx.y);
+    int Li = e.ie1 - e.is1 + 1, Lj = e.je1 - e.js1 + 1, Lk = e.ke1 - e.ks1 + 1;
+    int li = l - lj - lk;
+    if (lj >= Lj || lk >= Lk || li < 0 || li >= Li) return;
+    const long N = e.n;
+    const long sj = e.ni, sk = (long)e.ni * e.nj;
+    if (DIR > 0) {
+        long n = e.ix(e.is1 + li, e.js1 + lj, e.ks1 + lk);
+        double a = cu[n], b = cu[N + n], cc = cu[2 * N + n];
+        double ta = 0.5 * (a + fabs(a)) * dn1[n - 1];
+        double tb = 0.5 * (b + fabs(b)) * dn1[n - sj];
+        double tc = 0.5 * (cc + fabs(cc)) * dn1[n - sk];
+        dn1[n] = (sa_rhs[n] + (ta + tb + tc)) / cu[3 * N + n];
+    } else {
+        long n = e.ix(e.ie1 - li, e.je1 - lj, e.ke1 - lk);
+        double a = cu[n], b = cu[N + n], cc = cu[2 * N + n];
+        double ta = 0.5 * (a - fabs(a)) * dn[n + 1];
+        double tb = 0.5 * (b - fabs(b)) * dn[n + sj];
+        double tc = 0.5 * (cc - fabs(cc)) * dn[n + sk];
+        dn[n] = dn1[n] - (ta + tb + tc) / cu[3 * N + n];
+    }
+}
+
+__global__ void __launch_bounds__(256) k_add1(Ext e, Box b, double *__restrict__ x,
+                                              const double *__restrict__ d)
+{
+    int i, j, k;
+    if (!box_node(b, i, j, k)) return;
+    long n = e.ix(i, j, k);
+    x[n] = x[n] + d[n];
+}
+
+int launch_sa(hipStream_t s, const Ext &e, const Fields &f, const Consts &c, const double *cfl)
+{
+    int bs = 256;
+    // sa_rhs = 0 and zeroed temporaries (sub_tur_sa.f90:48, :276-291)
+    int rc = launch_fill(s, f.sa_rhs, e.n, 0.0);
+    if (!rc) rc = launch_fill(s, f.w_sa, 9 * e.n, 0.0);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_sa_length, dim3((unsigned)((e.n + bs - 1) / bs)), dim3(bs), 0, s, e, c, f.dist,
+                       f.spacing, f.pri, f.nut, f.amu, f.dudx, f.length);
+    Box b{e.is1, e.ie1, e.js1, e.je1, e.ks1, e.ke1};
+    dim3 g = box_grid(b), t(64, 4, 1);
+    if (g.x == 0) return 0;
+    hipLaunchKernelGGL(k_sa_a, g, t, 0, s, e, b, f.pri, f.dxidx, f.nut, f.amu, f.aal, f.bbe, f.gga, f.sa_rhs,
+                       f.w_sa);
+    hipLaunchKernelGGL(k_sa_b, g, t, 0, s, e, b, c, f.pri, f.dxidx, f.inv_j, f.nut, f.amu, f.vor, f.length,
+                       f.sa_rhs, f.w_sa);
+    Box ring{e.is1 - 1, e.ie1 + 1, e.js1 - 1, e.je1 + 1, e.ks1 - 1, e.ke1 + 1};
+    hipLaunchKernelGGL(k_sa_prep, box_grid(ring), t, 0, s, e, ring, c, f.pri, f.dxidx, f.nut, f.dt, cfl, f.w_sa,
+                       f.w_dqs, f.w_dq);
+    int Li = e.ie1 - e.is1 + 1, Lj = e.je1 - e.js1 + 1, Lk = e.ke1 - e.ks1 + 1;
+    int L = Li + Lj + Lk - 2;
+    double *dn1 = f.w_sa + 8 * e.n;
+    for (int dir = 0; dir < 2; dir++)
+        for (int l = 0; l < L; l++) {
+            int ja = max(0, l - (Li - 1) - (Lk - 1)), jb = min(Lj - 1, l);
+            int ka = max(0, l - (Li - 1) - (Lj - 1)), kb = min(Lk - 1, l);
+            dim3 gg((jb - ja + 64) / 64, (kb - ka + 4) / 4, 1);
+            if (dir == 0)
+                hipLaunchKernelGGL((k_sa_plane<1>), gg, t, 0, s, e, l, ja, ka, f.w_dqs, f.sa_rhs, dn1, f.w_dq);
+            else
+                hipLaunchKernelGGL((k_sa_plane<-1>), gg, t, 0, s, e, l, ja, ka, f.w_dqs, f.sa_rhs, dn1, f.w_dq);
+        }
+    hipLaunchKernelGGL(k_add1, g, t, 0, s, e, b, f.nut, f.w_dq);
+    ACFD_CHECK_LAUNCH();
+    return 0;
+}
+
+}  // namespace acfd

@@ -1,0 +1,67 @@
+// Host-side launchers of the device kernels (one translation unit each).
+#pragma once
+#include "common.hpp"
+
+namespace acfd {
+
+// k_flux.hip
+int launch_getpri(hipStream_t s, const Ext &e, const Fields &f, const Consts &c, unsigned int *nneg);
+int launch_fill(hipStream_t s, double *p, long n, double v);
+int launch_inviscid(hipStream_t s, const Ext &e, const Fields &f, const Consts &c, int precision);
+int launch_viscous(hipStream_t s, const Ext &e, const Fields &f, const Consts &c, int precision,
+                   int store_aux);
+int launch_find_dt(hipStream_t s, const Ext &e, const Fields &f, const Consts &c, double *cfl,
+                   unsigned long long *stats);
+int launch_reset_stats(hipStream_t s, unsigned long long *stats);
+
+// k_implicit.hip
+int launch_lusgs(hipStream_t s, const Ext &e, const Fields &f, const Consts &c, const double *cfl,
+                 int variant, int *tile_flags);
+int launch_lusgs_phase(hipStream_t s, const Ext &e, const Fields &f, const Consts &c, const double *cfl,
+                       int variant, int *tile_flags, int phase);
+int launch_crank(hipStream_t s, const Ext &e, const Fields &f, const Consts &c);
+int launch_update_q0(hipStream_t s, const Ext &e, const Fields &f);
+long lusgs_tile_flags_needed(const Ext &e);
+
+// k_sa.hip
+int launch_sa(hipStream_t s, const Ext &e, const Fields &f, const Consts &c, const double *cfl);
+
+// k_bc.hip
+struct BcTask {       // one physical subface
+    int block;        // 0-based
+    int face, kind;   // kind = blk_t (-10 .. -50)
+    int is, ie, js, je, ks, ke;  // loop range actually used (farfield js quirk applied)
+};
+int launch_bc(hipStream_t s, const Ext &e, const Fields &f, const Consts &c, const BcTask &t);
+int launch_sa_clamp(hipStream_t s, const Ext &e, const Fields &f);
+
+// halo copy lists (device arrays of node references)
+struct CopyList {
+    int count;
+    const int *src_blk, *dst_blk;     // 0-based block ids
+    const long *src, *dst;            // node indices in the blocks
+};
+int launch_halo_copy(hipStream_t s, const CopyList &cl, double *const *q_tab, double *const *nut_tab,
+                     const long *n_tab, int with_nut);
+// average: gather into buf (6 per entry), then apply in order per node
+struct AvgList {
+    int count;                        // entries
+    const int *src_blk;
+    const long *src;
+    int ngroups;                      // distinct destination nodes
+    const int *grp_blk;               // block of each group
+    const long *grp_dst;              // node index of each group
+    const int *grp_start;             // first entry of each group (ngroups+1)
+};
+int launch_average(hipStream_t s, const AvgList &al, double *const *q_tab, double *const *nut_tab,
+                   const long *n_tab, double *buf, int with_nut);
+
+int launch_residual(hipStream_t s, const Ext &e, const Fields &f, double *partial, int with_sa);
+int residual_partials(const Ext &e);
+int launch_residual_final(hipStream_t s, const double *partial, int count, double *out);
+
+// layout transposition between reference AoS and device SoA
+int launch_aos_to_soa(hipStream_t s, const double *aos, double *soa, long n, int ncomp);
+int launch_soa_to_aos(hipStream_t s, const double *soa, double *aos, long n, int ncomp);
+
+}  // namespace acfd

@@ -1,0 +1,187 @@
+"""Throughput of the residual-and-update path on one MI355X.
+
+Workload: Taylor-Green vortex, one block of 256^3 nodes on [0, 2 pi]^3 with
+every face pair self-connected (node 256 == node 1), Mach 0.1, Re 1600,
+laminar Navier-Stokes with the reference scheme (WENO3 + Roe inviscid flux,
+central viscous flux, first-order implicit LU-SGS), FP64.  One step is one
+pass of main.f90:343-447 plus the residual norm of that step.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 256]
+
+Prints one JSON line.  value = Mcell-updates/s = nodes * steps / seconds with
+the state resident in HBM; roofline = the dominant kernel's algorithmic bytes
+per launch over its HIP-event-timed average duration in the timed region;
+cpu_baseline = the reference's own Fortran (compiled from its sources into
+oracle/_ref) timing the same step on a bounded sample of the workload on one
+host core (or the C oracle when the compiled reference is absent).
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_HBM_GBS = 8000.0
+
+# Algorithmic bytes per computed node of each timed kernel/stage (FP64,
+# each array element counted once per read and once per write).
+BYTES_PER_NODE = {
+    "getpri": 40 + 56,                              # q -> pri_v
+    "inviscid": 40 + 72 + 8 + 40,                   # q, dxidx, inv_j -> rhsi
+    "viscous": 32 + 8 + 72 + 8 + 16 + 40,           # u,v,w,T, mu_t, dxidx, inv_j -> mu, rhsv
+    "lusgs_forward": 40 + 72 + 8 + 24 + 8 + 40 + 40,  # q, dxidx, inv_j, r_abc, diag, rhs -> dq*
+    "lusgs_backward": 40 + 72 + 8 + 24 + 8 + 40 + 40,  # q, dxidx, inv_j, r_abc, diag, dq* -> dq
+    "lusgs_other": 32 + 72 + 8 + 80 + 40 + 32 + 80 + 80,  # spectral radii/diag/rhs + q += dq
+}
+
+
+def cpu_baseline(n_sample=96, steps=8, timeout=300):
+    """Time the reference's own Fortran (or the oracle) on a TGV sample of
+    n_sample^3 nodes for `steps` steps in a child process (one core)."""
+    code = r"""
+import json, os, sys, time, tempfile
+sys.path.insert(0, %r)
+import numpy as np
+from aerocfd_amd import cases
+from oracle import refwrap as R, oracle as O
+from tests.helpers import ora_config, ora_states
+n, steps = %d, %d
+case = cases.tgv_box(n)
+kind = "reference" if R.available() else "port"
+if kind == "reference":
+    os.chdir(tempfile.mkdtemp())
+    c = case.cfg
+    R.setup(solver=0, itime=1, tadv=2, ma=c.ma, re=c.re, tinf=c.tinf, cfl=c.cfl, dt=c.dt)
+    exts = R.begin(case.dims, case.subfaces)
+    fl = case.fields[0]
+    for k in ("q", "dxidx", "inv_j", "amu", "dt"):
+        R.put(1, k, fl[k])
+    sc = R.scalars()
+    R.put(1, "pri_v", O.getpri(exts[0], fl["q"], sc["gamma"], sc["cv"])[0])
+    R.physical_bc(); R.update_buffer(); R.average()
+    R.step()
+    t0 = time.perf_counter()
+    for s in range(steps):
+        R.step()
+    dt = time.perf_counter() - t0
+else:
+    cfg = ora_config(case.cfg, case.totalnodes)
+    st = ora_states(case)
+    st[0]["pri_v"] = O.getpri(st[0].ext, st[0]["q"], cfg.gamma, cfg.cv)[0]
+    O.prepare(st, cfg)
+    O.step(st, cfg)
+    t0 = time.perf_counter()
+    for s in range(steps):
+        O.step(st, cfg)
+    dt = time.perf_counter() - t0
+print(json.dumps({"kind": kind, "seconds": dt, "nodes": case.totalnodes, "steps": steps}))
+""" % (ROOT, n_sample, steps)
+    env = dict(os.environ, OMP_NUM_THREADS="1")
+    try:
+        r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=timeout,
+                           env=env)
+        line = [l for l in r.stdout.splitlines() if l.startswith("{")]
+        if r.returncode != 0 or not line:
+            return {"value": None, "unit": "Mcell-updates/s", "cores": 1, "kind": None,
+                    "sample": "failed: " + (r.stderr or r.stdout)[-300:]}
+        d = json.loads(line[-1])
+    except subprocess.TimeoutExpired:
+        return {"value": None, "unit": "Mcell-updates/s", "cores": 1, "kind": None,
+                "sample": "timed out after %d s" % timeout}
+    return {"value": d["nodes"] * d["steps"] / d["seconds"] / 1e6, "unit": "Mcell-updates/s", "cores": 1,
+            "kind": d["kind"],
+            "sample": "TGV %d^3 periodic block, %d LU-SGS steps (same scheme and flags), %.2f s on one core"
+                      % (n_sample, d["steps"], d["seconds"])}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--precision", type=int, default=0, help="0 FP64, 1 FP32 flux / FP64 update")
+    ap.add_argument("--lusgs", type=int, default=1, help="0 hyperplane launches, 1 pipelined tile sweep")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU; every block of the workload is resident on it. "
+              "Sharding blocks over several GPUs is described in DESIGN.md but not built, so --gpus %d "
+              "is not supported." % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    from aerocfd_amd import Solver, cases
+
+    case = cases.tgv_box(args.n)
+    case.cfg.precision = args.precision
+    case.cfg.lusgs_kernel = args.lusgs
+    sol = Solver(case.cfg, case.dims, case.subfaces)
+    case.upload(sol)
+    del case.fields
+    sol.run_stage("physical_bc")
+    sol.run_stage("getpri")
+    sol.prepare()
+    for _ in range(args.warmup):
+        sol.step(1)
+    try:
+        import torch
+        sync = torch.cuda.synchronize if torch.cuda.is_available() else (lambda: None)
+    except Exception:
+        sync = lambda: None  # noqa: E731
+    sol.timing(True)
+    sol.sync()
+    sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        sol.step(1)
+    sol.sync()
+    sync()
+    t1 = time.perf_counter()
+    tm = sol.timing(None)
+    res = sol.residual()
+    sol.close()
+    secs = t1 - t0
+    nodes = case.totalnodes
+    value = nodes * args.steps / secs / 1e6
+    # dominant kernel among the timed ones (per launch averages)
+    comp = (args.n) ** 3  # computed nodes (all faces are interfaces)
+    cand = {k: v for k, v in tm.items() if k in BYTES_PER_NODE and v[1] > 0}
+    dom = max(cand, key=lambda k: cand[k][0])
+    avg_ms = cand[dom][0] / cand[dom][1]
+    bytes_launch = BYTES_PER_NODE[dom] * comp
+    achieved = bytes_launch / (avg_ms * 1e-3) / 1e9
+    out = {
+        "metric": "Mcell-updates/s",
+        "value": round(value, 3),
+        "unit": "Mcell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * secs / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.precision == 0 else "f32-flux/f64-update",
+        "data": "synthetic",
+        "config": {"workload": "tgv_%d^3_periodic_laminar_ns_weno3_roe_lusgs" % args.n, "nodes": nodes,
+                   "blocks": 1, "mach": 0.1, "re": 1600, "scheme": "WENO3+Roe, central viscous, LU-SGS",
+                   "lusgs_kernel": "pipelined-tiles" if args.lusgs == 1 else "hyperplanes"},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": PEAK_HBM_GBS,
+                     "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None,
+                     "bytes_per_node": BYTES_PER_NODE[dom], "avg_launch_ms": round(avg_ms, 4)},
+        "stage_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in tm.items() if v[1] > 0},
+        "residual": [float("%.6e" % x) for x in res[:5]],
+    }
+    if not args.no_cpu:
+        cb = cpu_baseline()
+        out["cpu_baseline"] = cb
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,176 @@
+/*
+ * aerocfd_amd -- MI355X (gfx950) implementation of aerocfd's per-block
+ * residual-and-update path behind a plain C ABI.
+ *
+ * The reference (jiamin0329/aerocfd, Fortran 90 + MPI) exposes this path as
+ * explicit-shape external subroutines called from its time loop
+ * (main.f90:342-613) and as argument-less routines that walk the block table
+ * (physical_bc, UpdateBuffer, average1/2, residual).  This header replaces
+ * both with:
+ *   - a resident context that owns every block on one GPU (flags and
+ *     constants explicit, no globals), and
+ *   - stateless per-routine entry points for unit parity.
+ * All arrays crossing the boundary are FP64 in the reference's own layout:
+ * Fortran column-major, component first, e.g. q(5,is:ie,js:je,ks:ke).
+ * Indices are 1-based as in the reference.  Every function returns 0 on
+ * success and a negative code on error; acfd_last_error() returns the message.
+ * Nothing here aborts the host.  A context is used by one host thread.
+ */
+#ifndef AEROCFD_AMD_H
+#define AEROCFD_AMD_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Error codes */
+#define ACFD_OK            0
+#define ACFD_EINVAL       -1   /* bad argument or configuration */
+#define ACFD_EDEVICE      -2   /* HIP runtime error */
+#define ACFD_ESTATE       -3   /* call out of order */
+#define ACFD_ENUMERIC     -4   /* negative density found (sub_getpri_v.f90:54-56) */
+
+/* Flags and constants.  Replaces the module state of module_flags.f90:4-27,
+ * module_ns_consts.f90:4-34 and the control file read by
+ * sub_init_readcontrolfile.f90:27-47.  Codes are the reference's. */
+typedef struct acfd_config {
+    double ma, re, tinf, aoa_deg;   /* freestream: Mach, Reynolds, T_inf [K], angle of attack [deg] */
+    double cfl;                     /* CFL number (steady local dt) */
+    double dt;                      /* time step (unsteady) */
+    double prl, prt;                /* laminar / turbulent Prandtl numbers (reference: 0.72 / 0.90) */
+    int iflag_solver;               /* 0 = Navier-Stokes, 1 = Euler */
+    int iflag_time;                 /* 0 = steady (local dt), 1 = unsteady */
+    int iflag_turbulence;           /* 0 = laminar, 2 = Spalart-Allmaras */
+    int iflag_des;                  /* 0 = RANS, 1 = DES97, 2 = DDES */
+    int iflag_timeadvance;          /* 2 = first-order implicit (LU-SGS), 3 = BDF2 sub-iterations */
+    int precision;                  /* 0 = FP64 everywhere; 1 = FP32 reconstruction/splitting/viscous, FP64 update */
+    int lusgs_kernel;               /* 0 = hyperplane launches, 1 = pipelined tile sweep (same results) */
+    int reserved[5];
+} acfd_config;
+
+/* One subface record, as read from bcinfo (sub_init_readbcinfo.f90:96-98):
+ * face 1..6 = i-,i+,j-,j+,k-,k+; blk_t > 0 names the neighbour block
+ * (1-based) and subface_t its subface; blk_t < 0 is a physical condition
+ * (-10 farfield, -20 wall, -30 symmetry, -40 outflow, -50 inflow); ori 1..4
+ * is the face orientation code of sub_bc_updatebuffer.f90:358-395. */
+typedef struct acfd_subface {
+    int f_no, face;
+    int is, ie, js, je, ks, ke;
+    int blk_t, subface_t, ori;
+} acfd_subface;
+
+/* Field identifiers for upload/download (module_block.f90:27-88). */
+enum acfd_field {
+    ACFD_Q = 1, ACFD_PRI_V = 2, ACFD_RHSI = 3, ACFD_RHSV = 4, ACFD_RHS = 5, ACFD_DUDX = 6,
+    ACFD_TAO = 7, ACFD_VOR = 8, ACFD_AMU = 9, ACFD_DT = 10, ACFD_DXIDX = 11, ACFD_INV_J = 12,
+    ACFD_AALPHA = 13, ACFD_BBETA = 14, ACFD_GGAMMA = 15, ACFD_DIST = 16, ACFD_SPACING = 17,
+    ACFD_NUT = 18, ACFD_SA_RHS = 19, ACFD_Q0 = 20, ACFD_LENGTH = 21
+};
+
+/* Stages of one time step, for acfd_run_stage (reference routine in brackets). */
+enum acfd_stage {
+    ACFD_STAGE_GETPRI = 1,        /* get_primitivevariables, sub_getpri_v.f90:19 */
+    ACFD_STAGE_INVISCID = 2,      /* inviscid_flux_1, sub_inviscid1.f90:8 */
+    ACFD_STAGE_VISCOUS = 3,       /* viscous_flux, sub_viscous.f90:9 */
+    ACFD_STAGE_FIND_DT = 4,       /* find_dt, sub_timeadvancement_finddt.f90:24 */
+    ACFD_STAGE_IMPLICIT1ST = 5,   /* implicit1st, sub_timeadvancement_1stimplicit.f90:7 */
+    ACFD_STAGE_SA = 6,            /* sa_model, sub_tur_sa.f90:7 */
+    ACFD_STAGE_PHYSICAL_BC = 7,   /* physical_bc, sub_bc.f90:9 */
+    ACFD_STAGE_UPDATE_BUFFER = 8, /* UpdateBuffer, sub_bc_updatebuffer.f90:10 */
+    ACFD_STAGE_AVERAGE = 9,       /* average1 + average2, sub_bc_updatebufferaverage.f90:10,442 */
+    ACFD_STAGE_CRANK2ND = 10,     /* crank2nd, sub_timeadvancement_2ndcrank.f90:7 */
+    ACFD_STAGE_UPDATE_Q0 = 11     /* update_q0, main.f90:692 */
+};
+
+typedef struct acfd_ctx acfd_ctx;
+
+const char *acfd_last_error(void);
+
+/* ---------------------------------------------------------------- context */
+/* Create a context on the current HIP device with one stream. */
+int acfd_create(const acfd_config *cfg, acfd_ctx **out);
+int acfd_destroy(acfd_ctx *ctx);
+
+/* Block table and interfaces.  dims[3*m..3*m+2] = ni,nj,nk of block m+1;
+ * nsub[m] subfaces of block m+1 follow one another in subs.  Index ranges
+ * are derived as in sub_init_allocatememory.f90:73-144 (2 halo layers on
+ * interface faces) and the halo copy lists are built once.  Replaces
+ * init_readbcinfo / init_allocatememory / init_readbcinfo2. */
+int acfd_set_topology(acfd_ctx *ctx, int nblk, const int *dims, const int *nsub,
+                      const acfd_subface *subs);
+/* ranges[18] = is,ie,js,je,ks,ke, is0..ke0, is1..ke1 of block m (1-based) */
+int acfd_block_ranges(acfd_ctx *ctx, int m, int *ranges);
+
+/* Copy one field of block m between host (reference AoS layout, extended
+ * extents) and the device-resident state. */
+int acfd_upload(acfd_ctx *ctx, int m, int field, const double *host);
+int acfd_download(acfd_ctx *ctx, int m, int field, double *host);
+
+/* Run one stage over all blocks, in the reference's block order. */
+int acfd_run_stage(acfd_ctx *ctx, int stage);
+/* The boundary refresh before the time loop (main.f90:309-318). */
+int acfd_prepare(acfd_ctx *ctx);
+/* nsteps time steps of main.f90:343-573 (iflag_timeadvance 2 or 3). */
+int acfd_step(acfd_ctx *ctx, int nsteps);
+/* RMS residual of sub_output_residual.f90:24-56: out[0..4] main equations,
+ * out[5] SA, out[6] = 0. */
+int acfd_residual(acfd_ctx *ctx, double *out);
+/* out = dtmax, dtmin, cflmax, cflmin of the last find_dt call and the
+ * current cfl (main.f90:589-592). */
+int acfd_stats(acfd_ctx *ctx, double *out);
+/* Wait for all work of the context. */
+int acfd_sync(acfd_ctx *ctx);
+/* Per-stage GPU time accumulated with HIP events on the context's stream
+ * (enable != 0 turns recording on and clears the totals).  ms[stage] and
+ * calls[stage] for stage 1..11; index 0 unused. */
+int acfd_timing(acfd_ctx *ctx, int enable, double *ms, long *calls);
+
+/* ---------------------------------------------------- stateless routines */
+/* Each runs one reference routine on one block given its 18 ranges
+ * (is..ke, is0..ke0, is1..ke1), copying inputs in and outputs out. */
+
+/* get_primitivevariables (sub_getpri_v.f90:19-63); returns ACFD_ENUMERIC
+ * after filling pri_v when a density is negative. */
+int acfd_getpri(const int *ranges, double gamma, double cv, const double *q, double *pri_v);
+/* inviscid_flux_1 (sub_inviscid1.f90:8-189) */
+int acfd_inviscid_rhs(const acfd_config *cfg, const int *ranges, const double *q,
+                      const double *dxidx, const double *inv_j, double *rhsi);
+/* viscous_flux (sub_viscous.f90:9-378); amu is read (component 2) and written
+ * (components 1 and 3). */
+int acfd_viscous_rhs(const acfd_config *cfg, const int *ranges, const double *pri_v,
+                     double *amu, const double *dxidx, const double *inv_j, double *rhsv,
+                     double *vor, double *dudx, double *tao);
+/* find_dt (sub_timeadvancement_finddt.f90:24-130); *cfl is read (steady) or
+ * overwritten (unsteady) as the reference does. */
+int acfd_find_dt(const acfd_config *cfg, const int *ranges, const double *pri_v,
+                 const double *amu, const double *dxidx, double *dt, double *cfl);
+/* implicit1st (sub_timeadvancement_1stimplicit.f90:7-585): q updated in place. */
+int acfd_lusgs(const acfd_config *cfg, const int *ranges, double *q, const double *dt,
+               const double *pri_v, const double *rhsi, const double *rhsv, const double *dxidx,
+               const double *inv_j, double cfl, double *rhs);
+/* crank2nd (sub_timeadvancement_2ndcrank.f90:7-467): q updated in place. */
+int acfd_bdf2(const acfd_config *cfg, const int *ranges, double *q, const double *q0,
+              const double *dt, const double *pri_v, const double *rhsi, const double *rhsv,
+              const double *dxidx, const double *inv_j, double *rhs);
+/* sa_model (sub_tur_sa.f90:7-122): nut updated in place. */
+int acfd_sa_step(const acfd_config *cfg, const int *ranges, double *nut, double *sa_rhs,
+                 const double *dt, const double *pri_v, const double *amu, const double *dudx,
+                 const double *vor, const double *dxidx, const double *inv_j,
+                 const double *aalpha, const double *bbeta, const double *ggamma,
+                 double *length, const double *dist, const double *spacing, double cfl);
+
+/* ------------------------------------------------------------- grid setup */
+/* Halo coordinate exchange over the context's topology, as
+ * UpdateBufferCoordinate (sub_bc_updatebuffercoordinate.f90:10-473), with an
+ * optional per-subface translation (period) added to the received
+ * coordinates: shift[3*s..] for the s-th subface in the order given to
+ * acfd_set_topology (NULL = none).  xyz[m] points to 3 consecutive extended
+ * scalar fields x,y,z of block m+1, updated in place on the host. */
+int acfd_exchange_coordinates(acfd_ctx *ctx, double **xyz, const double *shift);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

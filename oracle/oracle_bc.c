@@ -223,7 +223,7 @@ void ora_physical_bc(int nblk, ora_block *blk, const ora_config *c)
                 double nut = b->nut[n];
                 double leix = nut / b->amu[3 * n] * d;
                 leix = fmax(leix, 0.001);
-                double l3 = leix * (leix * leix);
+                double l3 = leix * leix * leix;
                 double fv1 = l3 / (l3 + cv1 * cv1 * cv1);
                 /* max(nut,1e-4) is overwritten by min(nut,1e6) of the old
                  * value (sub_bc.f90:80-81) */

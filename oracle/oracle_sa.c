@@ -153,7 +153,7 @@ static void sa_rhs3(const ora_ext *e, double *sa_rhs, const double *nut, const d
                 double omega = fabs(vor[n]);
                 double dist0 = length[n];
                 double leix = fmax(nut0 / nu0, 0.001);
-                double leix2 = leix * leix, leix3 = leix * (leix * leix);
+                double leix2 = leix * leix, leix3 = leix * leix * leix;
                 double fv1 = leix3 / (leix3 + cv13);
                 double fv2 = 1.0 - leix / (1.0 + leix * fv1);
                 double dist2 = dist0 * dist0;
@@ -163,11 +163,10 @@ static void sa_rhs3(const ora_ext *e, double *sa_rhs, const double *nut, const d
                 div = fmax(div, 1.0e-6);
                 double r = nut0 * div;
                 r = fmin(r, 10.0);
-                double r2 = r * r, r4 = r2 * r2;
-                double r6 = r2 * r4;
+                double r2 = r * r, r4 = r * r * r * r;
+                double r6 = r * r * r * r * r * r;
                 double g = r + cw2 * (r6 - r);
-                double g2 = g * g, g4 = g2 * g2;
-                double g6 = g2 * g4;
+                double g6 = g * g * g * g * g * g;
                 double fw = g * pow((1.0 + cw36) / (g6 + cw36), 1.0 / 6.0);
                 double ft2 = ct3 * exp(-ct4 * leix2);
                 double aaa = cb1 * ((1.0 - ft2) * fv2 + ft2) / kar2;
@@ -187,7 +186,7 @@ static void sa_rhs3(const ora_ext *e, double *sa_rhs, const double *nut, const d
                 div = (s * s) * (re * kar2 * dist2);
                 div = fmax(div, 1.0e-6);
                 double rn = (s - sn * nut0) / div;
-                double r5 = r4 * r;
+                double r5 = r * r * r * r * r;
                 double gn = (1.0 + cw2 * 6.0 * r5 - cw2) * rn;
                 double cw3m6 = pow(cw3, -6.0);
                 double fwn = -(1.0 / 6.0) * pow((pow(g, -6.0) + cw3m6) / (1.0 + cw3m6), -7.0 / 6.0);
@@ -287,7 +286,7 @@ void ora_sa_model(const ora_ext *e, double *nut, double *sa_rhs, const double *d
             temp = sqrt(temp);
             double rd = (nu0 + nut0) / (temp * re * (kar * kar) * (dist0 * dist0));
             double x = 8.0 * rd;
-            double fd = 1.0 - tanh(x * (x * x));
+            double fd = 1.0 - tanh(x * x * x);
             length[n] = dist0 - fd * fmax(0.0, dist0 - cdes * spac0);
         }
 #undef DU

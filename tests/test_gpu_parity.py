@@ -1,0 +1,376 @@
+"""GPU parity: every stage of the residual-and-update path on the HIP path
+(through the C ABI) against the CPU oracle on the same inputs.
+
+Tolerances: FP64 stages agree norm-wise per conserved variable to <= 1e-12
+after one evaluation and <= 1e-10 after many steps (the accuracy bar of the
+path); the GPU kernels use fused multiply-adds and a different division
+sequence, so agreement is to rounding, not bitwise.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from aerocfd_amd import FlowConfig, Solver, cases
+from oracle import oracle as O
+from tests.helpers import ora_config, ora_states, perturb_case, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+TOL1 = 1e-12
+TOLN = 1e-10
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def two_block_case(ns=False, sa=False, seed=0, tadv=2, itime=1):
+    cfg = FlowConfig(ma=0.5, re=2.0e4 if ns else 1.0e6, iflag_solver=0 if ns else 1,
+                     iflag_turbulence=2 if sa else 0, iflag_time=itime, iflag_timeadvance=tadv,
+                     cfl=0.8, dt=0.02)
+    dims, subs, origins = cases.split_box(12, 9, 8, ((6,), (), ()), phys=(-10, -40, -20, -30, -50, -10))
+    case = cases.Case("two_block", cfg, dims, subs)
+    h = 0.1
+    for m in range(len(dims)):
+        org, perm = origins[m]
+        xyz = cases.node_coords(case.ranges[m], org, (h, h, h), (0, 0, 0), perm)
+        q = cases.vortex_state(xyz * 10 - 5, cfg.ma)
+        dxidx, inv_j, al, be, ga = cases.uniform_metrics(case.ranges[m], (h, h, h), perm)
+        cases._common_fields(case, m, q, dxidx, inv_j, al, be, ga)
+        if sa:
+            case.fields[m]["aalpha"], case.fields[m]["bbeta"], case.fields[m]["ggamma"] = al, be, ga
+    perturb_case(case, seed=seed, sa=sa)
+    rng = np.random.default_rng(seed + 100)
+    for m in range(len(dims)):
+        # a non-uniform metric exercises every metric term
+        fl = case.fields[m]
+        fl["dxidx"] = fl["dxidx"] * (1 + 0.05 * rng.standard_normal(fl["dxidx"].shape))
+        fl["inv_j"] = fl["inv_j"] * (1 + 0.05 * rng.standard_normal(fl["inv_j"].shape))
+        if tadv == 3:
+            q = fl["q"]
+            fl["q0"] = np.ascontiguousarray(np.stack([q * 0.999, q * 1.0005], axis=-1).reshape(q.shape[:-1] + (10,)))
+    return case
+
+
+def upload_all(sol, case, states):
+    case.upload(sol)
+    # the oracle state was built from the same fields
+    for m in range(len(case.dims)):
+        for k in ("pri_v",):
+            sol.upload(m + 1, k, states[m][k])
+
+
+def prime(case, states, cfg):
+    """pri_v as the reference has it before the loop (getpri after the
+    initial boundary refresh)."""
+    for st in states:
+        st["pri_v"] = O.getpri(st.ext, st["q"], cfg.gamma, cfg.cv)[0]
+
+
+def compare(sol, states, names, tol, tag):
+    worst = 0.0
+    for m, st in enumerate(states):
+        for k in names:
+            g = sol.download(m + 1, k)
+            o = st[k]
+            e = rel_l2(g, o)
+            worst = max(worst, float(np.max(e)))
+            assert np.all(e <= tol), "%s: block %d field %s rel err %s" % (tag, m + 1, k, e)
+    return worst
+
+
+@pytest.mark.parametrize("ns", [False, True])
+def test_stage_residual(ns):
+    say("test_stage_residual ns=%s two_block 12x9x8 split" % ns)
+    case = two_block_case(ns=ns, seed=1)
+    cfg = ora_config(case.cfg, case.totalnodes)
+    states = ora_states(case)
+    prime(case, states, cfg)
+    sol = Solver(case.cfg, case.dims, case.subfaces)
+    try:
+        upload_all(sol, case, states)
+        sol.run_stage("getpri")
+        for st in states:
+            st["pri_v"] = O.getpri(st.ext, st["q"], cfg.gamma, cfg.cv)[0]
+        compare(sol, states, ["pri_v"], TOL1, "getpri")
+        sol.run_stage("inviscid")
+        for st in states:
+            st["rhsi"] = O.inviscid(st.ext, st["q"], st["dxidx"], st["inv_j"], cfg.gamma)
+        compare(sol, states, ["rhsi"], TOL1, "inviscid")
+        if ns:
+            sol.run_stage("viscous")
+            for st in states:
+                rhsv, amu, vor, dudx, tao = O.viscous(st.ext, st["pri_v"], st["amu"], st["dxidx"], st["inv_j"], cfg)
+                st["rhsv"], st["amu"], st["vor"], st["dudx"], st["tao"] = rhsv, amu, vor, dudx, tao
+            compare(sol, states, ["rhsv", "amu", "vor", "dudx", "tao"], TOL1, "viscous")
+    finally:
+        sol.close()
+
+
+@pytest.mark.parametrize("itime", [0, 1])
+def test_stage_find_dt(itime):
+    say("test_stage_find_dt itime=%d" % itime)
+    case = two_block_case(ns=True, seed=2, itime=itime)
+    cfg = ora_config(case.cfg, case.totalnodes)
+    states = ora_states(case)
+    prime(case, states, cfg)
+    sol = Solver(case.cfg, case.dims, case.subfaces)
+    try:
+        upload_all(sol, case, states)
+        sol.run_stage("find_dt")
+        cfl = case.cfg.cfl
+        for st in states:
+            dt, cfl, _ = O.find_dt(st.ext, st["dt"], st["pri_v"], st["amu"], st["dxidx"], cfg.re, cfl, itime)
+            st["dt"] = dt
+        compare(sol, states, ["dt"], TOL1, "find_dt")
+        s = sol.stats()
+        if itime == 1:
+            assert abs(s[4] - cfl) <= 1e-13 * abs(cfl)
+    finally:
+        sol.close()
+
+
+@pytest.mark.parametrize("variant", [0, 1])
+def test_stage_lusgs(variant):
+    say("test_stage_lusgs variant=%d" % variant)
+    case = two_block_case(ns=True, seed=3)
+    case.cfg.lusgs_kernel = variant
+    cfg = ora_config(case.cfg, case.totalnodes)
+    states = ora_states(case)
+    prime(case, states, cfg)
+    rng = np.random.default_rng(33)
+    for m, st in enumerate(states):
+        st["rhsi"] = 0.01 * rng.standard_normal(st["rhsi"].shape)
+        st["rhsv"] = 0.001 * rng.standard_normal(st["rhsv"].shape)
+    sol = Solver(case.cfg, case.dims, case.subfaces)
+    try:
+        upload_all(sol, case, states)
+        for m, st in enumerate(states):
+            sol.upload(m + 1, "rhsi", st["rhsi"])
+            sol.upload(m + 1, "rhsv", st["rhsv"])
+        sol.run_stage("implicit1st")
+        for st in states:
+            q, rhs = O.implicit1st(st.ext, st["q"], st["dt"], st["pri_v"], st["rhsi"], st["rhsv"],
+                                   st["dxidx"], st["inv_j"], cfg.gamma, case.cfg.cfl)
+            dq = q - st["q"]
+            st["q"], st["rhs"] = q, rhs
+            g = sol.download(states.index(st) + 1, "q") - (q - dq)
+            e = rel_l2(g, dq)
+            assert np.all(e <= 1e-11), "lusgs dq rel err %s" % e
+        compare(sol, states, ["q"], TOL1, "implicit1st")
+    finally:
+        sol.close()
+
+
+def test_stage_crank():
+    say("test_stage_crank")
+    case = two_block_case(ns=True, seed=4, tadv=3)
+    cfg = ora_config(case.cfg, case.totalnodes)
+    states = ora_states(case)
+    prime(case, states, cfg)
+    rng = np.random.default_rng(44)
+    for st in states:
+        st["rhsi"] = 0.01 * rng.standard_normal(st["rhsi"].shape)
+        st["rhsv"] = 0.001 * rng.standard_normal(st["rhsv"].shape)
+    sol = Solver(case.cfg, case.dims, case.subfaces)
+    try:
+        upload_all(sol, case, states)
+        for m, st in enumerate(states):
+            sol.upload(m + 1, "rhsi", st["rhsi"])
+            sol.upload(m + 1, "rhsv", st["rhsv"])
+        sol.run_stage("crank2nd")
+        for st in states:
+            q, rhs = O.crank2nd(st.ext, st["q"], st["q0"], st["dt"], st["pri_v"], st["rhsi"], st["rhsv"],
+                                st["dxidx"], st["inv_j"], cfg.gamma)
+            st["q"], st["rhs"] = q, rhs
+        compare(sol, states, ["q", "rhs"], TOL1, "crank2nd")
+        sol.run_stage("update_q0")
+        for st in states:
+            q0 = st["q0"].copy().reshape(st["q"].shape[:-1] + (5, 2))
+            e = st.ext
+            sl = (slice(e.ks0 - e.ks, e.ke0 - e.ks + 1), slice(e.js0 - e.js, e.je0 - e.js + 1),
+                  slice(e.is0 - e.is_, e.ie0 - e.is_ + 1))
+            q0[sl + (slice(None), 0)] = q0[sl + (slice(None), 1)]
+            q0[sl + (slice(None), 1)] = st["q"][sl]
+            st["q0"] = q0.reshape(st["q0"].shape)
+        compare(sol, states, ["q0"], TOL1, "update_q0")
+    finally:
+        sol.close()
+
+
+def test_stage_sa():
+    say("test_stage_sa")
+    case = two_block_case(ns=True, sa=True, seed=5, itime=0)
+    cfg = ora_config(case.cfg, case.totalnodes)
+    states = ora_states(case)
+    prime(case, states, cfg)
+    rng = np.random.default_rng(55)
+    for st in states:
+        st["vor"] = np.abs(rng.standard_normal(st["vor"].shape))
+        st["dt"] = 0.01 * (1 + rng.uniform(0, 1, st["dt"].shape))
+    sol = Solver(case.cfg, case.dims, case.subfaces)
+    try:
+        upload_all(sol, case, states)
+        for m, st in enumerate(states):
+            for k in ("vor", "dt", "nut", "dist", "spacing", "aalpha", "bbeta", "ggamma"):
+                sol.upload(m + 1, k, st[k])
+        sol.run_stage("sa")
+        for st in states:
+            nut, sa_rhs, length = O.sa_model(st.ext, st["nut"], st["dt"], st["pri_v"], st["amu"], st["dudx"],
+                                             st["vor"], st["dxidx"], st["inv_j"], st["aalpha"], st["bbeta"],
+                                             st["ggamma"], st["dist"], st["spacing"], cfg.re, case.cfg.cfl, 0, 0)
+            st["nut"], st["sa_rhs"], st["length"] = nut, sa_rhs, length
+        compare(sol, states, ["sa_rhs", "nut", "length"], 1e-11, "sa_model")
+    finally:
+        sol.close()
+
+
+@pytest.mark.parametrize("sa", [False, True])
+def test_stage_boundaries(sa):
+    say("test_stage_boundaries sa=%s" % sa)
+    case = two_block_case(ns=True, sa=sa, seed=6, itime=0 if sa else 1)
+    cfg = ora_config(case.cfg, case.totalnodes)
+    states = ora_states(case)
+    prime(case, states, cfg)
+    sol = Solver(case.cfg, case.dims, case.subfaces)
+    try:
+        upload_all(sol, case, states)
+        if sa:
+            for m, st in enumerate(states):
+                sol.upload(m + 1, "nut", st["nut"])
+        sol.run_stage("physical_bc")
+        O.physical_bc(states, cfg)
+        compare(sol, states, ["q", "nut", "amu"], TOL1, "physical_bc")
+        sol.run_stage("update_buffer")
+        O.update_buffer(states, cfg)
+        # the boundary values already differ by rounding (pow on the device)
+        compare(sol, states, ["q", "nut"], TOL1, "update_buffer")
+        sol.run_stage("average")
+        O.average(states, cfg)
+        compare(sol, states, ["q", "nut"], TOL1, "average")
+    finally:
+        sol.close()
+
+
+@pytest.mark.parametrize("ori", [1, 3, 4])
+def test_halo_orientations(ori):
+    """Interfaces with every orientation code on square faces (not a
+    geometric pairing: the copy rule itself is what is compared)."""
+    say("test_halo_orientations ori=%d" % ori)
+    n = 7
+    cfg = FlowConfig(ma=0.5, iflag_solver=1, iflag_turbulence=2)
+    dims = [(n, n, n), (n, n, n)]
+    s1 = cases.box_subfaces(n, n, n, [-10, (2, 3), -10, (2, 6), -30, (2, 1)])
+    s2 = cases.box_subfaces(n, n, n, [(1, 6), -10, (1, 2), -10, -20, (1, 4)])
+    for s in s1 + s2:
+        if s[8] > 0:
+            s[10] = ori
+    case = cases.Case("ori", cfg, dims, [s1, s2])
+    rng = np.random.default_rng(ori)
+    states = []
+    sol = Solver(cfg, dims, case.subfaces)
+    try:
+        for m in range(2):
+            st = O.BlockState(case.ranges[m], case.subfaces[m])
+            st["q"] = 1 + rng.uniform(0, 1, st["q"].shape)
+            st["nut"] = rng.uniform(0, 1, st["nut"].shape)
+            sol.upload(m + 1, "q", st["q"])
+            sol.upload(m + 1, "nut", st["nut"])
+            states.append(st)
+        c = ora_config(cfg, 2 * n ** 3)
+        c.iflag_turbulence = 2
+        sol.run_stage("update_buffer")
+        O.update_buffer(states, c)
+        compare(sol, states, ["q", "nut"], 0.0, "update_buffer ori")
+        sol.run_stage("average")
+        O.average(states, c)
+        compare(sol, states, ["q", "nut"], 0.0, "average ori")
+    finally:
+        sol.close()
+
+
+def run_steps(case, nsteps, sa=False):
+    cfg = ora_config(case.cfg, case.totalnodes)
+    states = ora_states(case)
+    sol = Solver(case.cfg, case.dims, case.subfaces)
+    try:
+        case.upload(sol)
+        if sa:
+            for m, st in enumerate(states):
+                for k in ("nut", "dist", "spacing", "aalpha", "bbeta", "ggamma"):
+                    sol.upload(m + 1, k, st[k])
+        # initial pri_v and boundary refresh (main.f90:249-257, :309-318)
+        O.physical_bc(states, cfg)
+        for st in states:
+            st["pri_v"] = O.getpri(st.ext, st["q"], cfg.gamma, cfg.cv)[0]
+        sol.run_stage("physical_bc")
+        sol.run_stage("getpri")
+        O.prepare(states, cfg)
+        sol.prepare()
+        res_o = []
+        res_g = []
+        for s in range(nsteps):
+            O.step(states, cfg)
+            res_o.append(O.residual(states, case.totalnodes))
+        sol.step(nsteps)
+        res_g = sol.residual()
+        return sol, states, res_o, res_g
+    except Exception:
+        sol.close()
+        raise
+
+
+@pytest.mark.parametrize("kind,nsteps", [("vortex", 20), ("tgv", 10), ("vortex_bdf2", 3),
+                                         ("sa_blocks", 5)])
+def test_steps(kind, nsteps):
+    say("test_steps %s n=%d" % (kind, nsteps))
+    sa = False
+    if kind == "vortex":
+        case = cases.vortex_box(20)
+    elif kind == "vortex_bdf2":
+        case = cases.vortex_box(16)
+        case.cfg.iflag_timeadvance = 3
+        for fl in case.fields:
+            q = fl["q"]
+            fl["q0"] = np.ascontiguousarray(np.stack([q, q], axis=-1).reshape(q.shape[:-1] + (10,)))
+    elif kind == "tgv":
+        case = cases.tgv_box(16, blocks=((8,), (), ()))
+    else:
+        sa = True
+        case = two_block_case(ns=True, sa=True, seed=7, itime=0)
+    sol, states, res_o, res_g = run_steps(case, nsteps, sa=sa)
+    try:
+        for m, st in enumerate(states):
+            g = sol.download(m + 1, "q")
+            e = rel_l2(g, st["q"])
+            say("  block", m + 1, "q rel err", e)
+            assert np.all(e <= TOLN), "q after %d steps: %s" % (nsteps, e)
+            if sa:
+                e = rel_l2(sol.download(m + 1, "nut"), st["nut"])
+                assert e <= TOLN, "nut after steps: %s" % e
+        r = res_o[-1]
+        assert np.allclose(res_g[:6], r[:6], rtol=1e-9, atol=1e-300), (res_g, r)
+    finally:
+        sol.close()
+
+
+def test_lusgs_variants_identical():
+    """The pipelined tile sweep computes each node from the same operands as
+    the hyperplane sweep: results must agree to the last bit."""
+    say("test_lusgs_variants_identical tgv 40^3")
+    outs = []
+    for v in (0, 1):
+        case = cases.tgv_box(40)
+        case.cfg.lusgs_kernel = v
+        sol = Solver(case.cfg, case.dims, case.subfaces)
+        try:
+            case.upload(sol)
+            sol.run_stage("getpri")
+            sol.prepare()
+            sol.step(3)
+            outs.append(sol.download(1, "q"))
+        finally:
+            sol.close()
+    assert np.array_equal(outs[0], outs[1])
