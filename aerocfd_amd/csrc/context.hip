@@ -8,6 +8,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -48,8 +49,39 @@ struct Block {
     Ext e;
     Fields f;
     double *arena = nullptr;
+    double *uc = nullptr;   // dq*/dq in uncached memory (tile sweeps)
+    int tile_shape = -1;    // chosen by the autotuner on first use
     std::vector<acfd_subface> sub;
 };
+
+// Uncached device memory for the tile sweeps' exchange arrays and counters.
+// Memory of this type that a kernel has read is never returned to the
+// runtime before the process ends; chunks are recycled between contexts.
+struct UcChunk {
+    void *p;
+    size_t n;
+    bool used;
+};
+std::vector<UcChunk> g_uc;
+
+void *uc_acquire(size_t n)
+{
+    for (auto &c : g_uc)
+        if (!c.used && c.n >= n) {
+            c.used = true;
+            return c.p;
+        }
+    void *p = nullptr;
+    if (hipExtMallocWithFlags(&p, n, hipDeviceMallocUncached) != hipSuccess) return nullptr;
+    g_uc.push_back({p, n, true});
+    return p;
+}
+
+void uc_release(void *p)
+{
+    for (auto &c : g_uc)
+        if (c.p == p) c.used = false;
+}
 
 }  // namespace
 
@@ -66,8 +98,9 @@ struct acfd_ctx {
     unsigned int *d_nneg = nullptr;
     double *d_res = nullptr;       // 6 sums
     double *d_partial = nullptr;   // residual partials
-    int *d_tile_flags = nullptr;
+    int *d_tile_flags = nullptr;   // [0] error word, [1..] tile counters
     long tile_flags_n = 0;
+    bool flags_uc = false;
     double *d_stage = nullptr;     // upload/download staging
     long stage_n = 0;
     // pointer tables for the multi-block copy kernels
@@ -102,6 +135,8 @@ void free_block(Block &b)
 {
     if (b.arena) (void)hipFree(b.arena);
     b.arena = nullptr;
+    if (b.uc) uc_release(b.uc);
+    b.uc = nullptr;
 }
 
 template <typename T>
@@ -231,6 +266,8 @@ int alloc_block(acfd_ctx *ctx, Block &b)
     const Ext &e = b.e;
     const bool sa = ctx->c.turb == 2 && ctx->c.solver == 0;
     const bool bdf = ctx->cfg.iflag_timeadvance == 3;
+    // the tile sweeps exchange dq* and dq through uncached memory
+    const bool uc = ctx->cfg.lusgs_kernel == 1;
     // field table: pointer slot and component count
     struct Slot {
         double **p;
@@ -240,8 +277,8 @@ int alloc_block(acfd_ctx *ctx, Block &b)
         {&b.f.inv_j, 1}, {&b.f.amu, 3}, {&b.f.dt, 1}, {&b.f.dudx, 9}, {&b.f.tao, 6}, {&b.f.vor, 1},
         {&b.f.aal, sa ? 3 : 0}, {&b.f.bbe, sa ? 3 : 0}, {&b.f.gga, sa ? 3 : 0}, {&b.f.dist, sa ? 1 : 0},
         {&b.f.spacing, sa ? 1 : 0}, {&b.f.length, sa ? 1 : 0}, {&b.f.nut, 1}, {&b.f.sa_rhs, 1},
-        {&b.f.q0, bdf ? 10 : 0}, {&b.f.w_flux, 12}, {&b.f.w_dqs, 5}, {&b.f.w_dq, 5}, {&b.f.w_diag, 1},
-        {&b.f.w_r, 3}, {&b.f.w_sa, sa ? 9 : 0},
+        {&b.f.q0, bdf ? 10 : 0}, {&b.f.w_flux, 12}, {&b.f.w_dqs, uc ? 0 : 5}, {&b.f.w_dq, uc ? 0 : 5},
+        {&b.f.w_diag, 1}, {&b.f.w_r, 3}, {&b.f.w_sa, sa ? 9 : 0},
     };
     long tot = 0;
     for (auto &s : slots) tot += (long)s.nc * e.n;
@@ -251,6 +288,13 @@ int alloc_block(acfd_ctx *ctx, Block &b)
     for (auto &s : slots) {
         *s.p = s.nc ? b.arena + off : nullptr;
         off += (long)s.nc * e.n;
+    }
+    if (uc) {
+        b.uc = (double *)uc_acquire((size_t)10 * e.n * sizeof(double));
+        if (!b.uc) return fail(ACFD_EDEVICE, "uncached device memory for the tile sweep is unavailable");
+        HIP_TRY(hipMemsetAsync(b.uc, 0, (size_t)10 * e.n * sizeof(double), ctx->stream));
+        b.f.w_dqs = b.uc;
+        b.f.w_dq = b.uc + 5 * e.n;
     }
     return 0;
 }
@@ -290,6 +334,34 @@ int ensure_stage(acfd_ctx *ctx, long n)
     ctx->d_stage = nullptr;
     HIP_TRY(hipMalloc((void **)&ctx->d_stage, n * sizeof(double)));
     ctx->stage_n = n;
+    return 0;
+}
+
+// Progress counters of the tile sweeps (uncached memory) or of nothing
+// (hyperplane sweeps).  The counters must start at or below the current
+// sweep epoch: fresh uncached memory is not zeroed and a recycled chunk may
+// have held other data, so the whole array is cleared here.
+int alloc_tile_flags(acfd_ctx *ctx, long n)
+{
+    ctx->tile_flags_n = n;
+    if (ctx->cfg.lusgs_kernel == 1) {
+        ctx->d_tile_flags = (int *)uc_acquire((size_t)n * sizeof(int));
+        if (!ctx->d_tile_flags) return fail(ACFD_EDEVICE, "uncached device memory for the tile counters is unavailable");
+        ctx->flags_uc = true;
+        HIP_TRY(hipMemset(ctx->d_tile_flags, 0, (size_t)n * sizeof(int)));
+    } else {
+        HIP_TRY(hipMalloc((void **)&ctx->d_tile_flags, (size_t)n * sizeof(int)));
+        HIP_TRY(hipMemset(ctx->d_tile_flags, 0, (size_t)n * sizeof(int)));
+    }
+    return 0;
+}
+
+int tile_error(acfd_ctx *ctx)
+{
+    if (ctx->cfg.lusgs_kernel != 1 || !ctx->d_tile_flags) return 0;
+    int err = 0;
+    HIP_TRY(hipMemcpy(&err, ctx->d_tile_flags, sizeof(int), hipMemcpyDeviceToHost));
+    if (err) return fail(ACFD_EDEVICE, "pipelined LU-SGS sweep timed out waiting for a neighbouring tile");
     return 0;
 }
 
@@ -378,14 +450,58 @@ int st_find_dt(acfd_ctx *ctx, int m)
 // sub-stage timing ids of the two LU-SGS sweeps
 constexpr int kTimeLuForward = 12, kTimeLuBackward = 13;
 
+// Autotuner of the tile sweep: on a block's first sweep, time the forward
+// sweep once per tile shape (the sweep recomputes dq* from unchanged inputs,
+// so repeating it changes nothing) and keep the fastest shape.
+int tune_tiles(acfd_ctx *ctx, Block &b)
+{
+    if (const char *fix = std::getenv("ACFD_TILE_SHAPE")) {
+        int s = std::atoi(fix);
+        if (s >= 0 && s < lusgs_tile_shapes()) {
+            b.tile_shape = s;
+            return 0;
+        }
+    }
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    float best = 1e30f;
+    int best_s = 0;
+    for (int s = 0; s < lusgs_tile_shapes(); s++) {
+        if (lusgs_tiles(b.e, s) <= 0) continue;
+        (void)hipEventRecord(e0, ctx->stream);
+        int rc = launch_lusgs_phase(ctx->stream, b.e, b.f, ctx->c, ctx->d_cfl, 1, ctx->d_tile_flags, 1, s);
+        (void)hipEventRecord(e1, ctx->stream);
+        if (rc) return fail(ACFD_EDEVICE, "tile sweep launch failed during tuning");
+        HIP_TRY(hipEventSynchronize(e1));
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        if (std::getenv("ACFD_VERBOSE"))
+            std::fprintf(stderr, "acfd: tile sweep shape %d (%ld tiles): %.3f ms\n", s, lusgs_tiles(b.e, s), ms);
+        if (ms < best) {
+            best = ms;
+            best_s = s;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    b.tile_shape = best_s;
+    return 0;
+}
+
 int st_implicit(acfd_ctx *ctx, int m)
 {
     StageTimer t(ctx, ACFD_STAGE_IMPLICIT1ST);
-    const Block &b = ctx->blk[m];
+    Block &b = ctx->blk[m];
+    const int variant = ctx->cfg.lusgs_kernel;
     for (int ph = 0; ph < 4; ph++) {
+        if (ph == 1 && variant == 1 && b.tile_shape < 0) {
+            int rc = tune_tiles(ctx, b);
+            if (rc) return rc;
+        }
         StageTimer ts(ctx, ph == 1 ? kTimeLuForward : ph == 2 ? kTimeLuBackward : 15);
-        KRN_TRY(launch_lusgs_phase(ctx->stream, b.e, b.f, ctx->c, ctx->d_cfl, ctx->cfg.lusgs_kernel,
-                                   ctx->d_tile_flags, ph));
+        KRN_TRY(launch_lusgs_phase(ctx->stream, b.e, b.f, ctx->c, ctx->d_cfl, variant, ctx->d_tile_flags, ph,
+                                   b.tile_shape < 0 ? 0 : b.tile_shape));
     }
     return 0;
 }
@@ -564,7 +680,11 @@ int acfd_destroy(acfd_ctx *ctx)
     if (!ctx) return 0;
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     for (auto &b : ctx->blk) free_block(b);
-    void *ptrs[] = {ctx->d_cfl, ctx->d_stats, ctx->d_nneg, ctx->d_res, ctx->d_partial, ctx->d_tile_flags,
+    if (ctx->d_tile_flags) {
+        if (ctx->flags_uc) uc_release(ctx->d_tile_flags);
+        else (void)hipFree(ctx->d_tile_flags);
+    }
+    void *ptrs[] = {ctx->d_cfl, ctx->d_stats, ctx->d_nneg, ctx->d_res, ctx->d_partial,
                     ctx->d_stage, ctx->d_qtab, ctx->d_nuttab, ctx->d_ntab, ctx->d_h_sb, ctx->d_h_db,
                     ctx->d_h_sn, ctx->d_h_dn, ctx->d_a_sb, ctx->d_a_gb, ctx->d_a_gs, ctx->d_a_sn,
                     ctx->d_a_gd, ctx->d_a_buf};
@@ -733,9 +853,8 @@ int acfd_set_topology(acfd_ctx *ctx, int nblk, const int *dims, const int *nsub,
         (rc = dev_upload(&ctx->d_ntab, ntab)))
         return rc;
     HIP_TRY(hipMalloc((void **)&ctx->d_partial, (size_t)parts * 6 * sizeof(double)));
-    HIP_TRY(hipMalloc((void **)&ctx->d_tile_flags, (size_t)tiles * sizeof(int)));
-    HIP_TRY(hipMemsetAsync(ctx->d_tile_flags, 0, (size_t)tiles * sizeof(int), ctx->stream));
-    ctx->tile_flags_n = tiles;
+    rc = alloc_tile_flags(ctx, tiles);
+    if (rc) return rc;
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     ctx->topo = true;
     return 0;
@@ -847,11 +966,8 @@ int acfd_step(acfd_ctx *ctx, int nsteps)
         if (rc) return rc;
     }
     HIP_TRY(hipStreamSynchronize(ctx->stream));
-    if (ctx->cfg.lusgs_kernel == 1 && ctx->d_tile_flags) {
-        int err = 0;
-        HIP_TRY(hipMemcpy(&err, ctx->d_tile_flags + (ctx->tile_flags_n - 1), sizeof(int), hipMemcpyDeviceToHost));
-        if (err) return fail(ACFD_EDEVICE, "acfd_step: pipelined LU-SGS sweep timed out waiting for a tile");
-    }
+    int rc = tile_error(ctx);
+    if (rc) return rc;
     return check_nneg(ctx);
 }
 
@@ -972,9 +1088,8 @@ int make_single(const acfd_config *cfg, const int *r, acfd_ctx **out)
     dev_upload(&ctx->d_nuttab, nt);
     dev_upload(&ctx->d_ntab, nn);
     HIP_TRY(hipMalloc((void **)&ctx->d_partial, (size_t)residual_partials(e) * 6 * sizeof(double)));
-    ctx->tile_flags_n = lusgs_tile_flags_needed(e);
-    HIP_TRY(hipMalloc((void **)&ctx->d_tile_flags, (size_t)ctx->tile_flags_n * sizeof(int)));
-    HIP_TRY(hipMemset(ctx->d_tile_flags, 0, (size_t)ctx->tile_flags_n * sizeof(int)));
+    rc = alloc_tile_flags(ctx, lusgs_tile_flags_needed(e));
+    if (rc) return rc;
     ctx->totalnodes = (long)(e.ie0 - e.is0 + 1) * (e.je0 - e.js0 + 1) * (e.ke0 - e.ks0 + 1);
     ctx->topo = true;
     return 0;
@@ -1091,10 +1206,9 @@ int acfd_lusgs(const acfd_config *cfg, const int *r, double *q, const double *dt
     ST_TRY(put(s.ctx, ACFD_INV_J, inv_j));
     ST_TRY(set_cfl(s.ctx, cfl));
     ST_TRY(acfd_run_stage(s.ctx, ACFD_STAGE_IMPLICIT1ST));
-    if (cfg->lusgs_kernel == 1) {
-        int err = 0;
-        HIP_TRY(hipMemcpy(&err, s.ctx->d_tile_flags + (s.ctx->tile_flags_n - 1), sizeof(int), hipMemcpyDeviceToHost));
-        if (err) return fail(ACFD_EDEVICE, "acfd_lusgs: pipelined sweep timed out waiting for a tile");
+    {
+        int rc = tile_error(s.ctx);
+        if (rc) return rc;
     }
     ST_TRY(acfd_download(s.ctx, 1, ACFD_Q, q));
     if (rhs) ST_TRY(acfd_download(s.ctx, 1, ACFD_RHS, rhs));

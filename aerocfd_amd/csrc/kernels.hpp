@@ -14,14 +14,18 @@ int launch_find_dt(hipStream_t s, const Ext &e, const Fields &f, const Consts &c
                    unsigned long long *stats);
 int launch_reset_stats(hipStream_t s, unsigned long long *stats);
 
-// k_implicit.hip
+// k_lusgs.hip
 int launch_lusgs(hipStream_t s, const Ext &e, const Fields &f, const Consts &c, const double *cfl,
                  int variant, int *tile_flags);
 int launch_lusgs_phase(hipStream_t s, const Ext &e, const Fields &f, const Consts &c, const double *cfl,
-                       int variant, int *tile_flags, int phase);
+                       int variant, int *tile_flags, int phase, int shape);
+long lusgs_tile_flags_needed(const Ext &e);
+int lusgs_tile_shapes();
+long lusgs_tiles(const Ext &e, int shape);
+
+// k_bdf2.hip
 int launch_crank(hipStream_t s, const Ext &e, const Fields &f, const Consts &c);
 int launch_update_q0(hipStream_t s, const Ext &e, const Fields &f);
-long lusgs_tile_flags_needed(const Ext &e);
 
 // k_sa.hip
 int launch_sa(hipStream_t s, const Ext &e, const Fields &f, const Consts &c, const double *cfl);

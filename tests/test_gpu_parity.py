@@ -373,4 +373,6 @@ def test_lusgs_variants_identical():
             outs.append(sol.download(1, "q"))
         finally:
             sol.close()
+    d = np.abs(outs[0] - outs[1])
+    say("  max |q_hyperplane - q_tiles| = %.3e" % d.max())
     assert np.array_equal(outs[0], outs[1])
