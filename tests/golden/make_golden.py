@@ -1,0 +1,247 @@
+"""Write the golden fixtures of the residual-and-update path from the
+reference's own Fortran (compiled from its sources into oracle/_ref by
+``make -C oracle ref``).
+
+    python tests/golden/make_golden.py
+
+Every fixture is an ``.npz`` (float64 / int32 arrays, no pickles) holding the
+inputs of one case and the reference's outputs on them; ``manifest.json``
+lists the cases.  Inputs are stored, not regenerated, so a fixture does not
+depend on the random generator of the numpy version that reads it.
+
+Cases (all small, a few hundred nodes per block):
+  kernels      one 7^3 farfield block: get_primitivevariables,
+               thirdorder_weno (3 directions x 2 sides), inviscid_flux_1,
+               viscous_flux, find_dt (steady/unsteady), implicit1st,
+               crank2nd, calc_jm
+  sa_des{0,1,2} one 6^3 wall block: sa_model with DES off / DES97 / DDES
+  halo_ori{1..4} two 5^3 blocks joined on three face pairs with orientation
+               code ori: UpdateBuffer then average1/average2
+  steps_s{solver}_t{turb}_a{tadv}  two 5^3 blocks with every physical
+               condition and an interface: three whole time steps
+               (main.f90:343-613)
+"""
+import json
+import os
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+from aerocfd_amd import cases  # noqa: E402
+from oracle import oracle as O  # noqa: E402
+from oracle import refwrap as R  # noqa: E402
+
+
+def rand_state(ext, rng, ma=0.5, amp=0.1):
+    """Smooth random flow state and a distorted metric on an extended block."""
+    from tests.helpers import smooth_noise
+    nk, nj, ni = ext.shape
+    g = 1.4
+    rho = 1 + smooth_noise((nk, nj, ni), rng, amp)
+    u = 0.3 + smooth_noise((nk, nj, ni), rng, amp)
+    v = smooth_noise((nk, nj, ni), rng, amp)
+    w = smooth_noise((nk, nj, ni), rng, amp)
+    p = 1 / g / ma / ma * (1 + smooth_noise((nk, nj, ni), rng, amp))
+    q = cases.conservative(rho, u, v, w, p)
+    dx = np.zeros((nk, nj, ni, 9))
+    for d in range(3):
+        dx[..., 4 * d] = 8.0
+    dx += 0.5 * rng.standard_normal(dx.shape)
+    ij = (1 / 8.0) ** 3 * (1 + 0.1 * rng.standard_normal((nk, nj, ni)))
+    return q, dx, ij
+
+
+def box6(n, codes):
+    """Six whole-face subfaces of an n^3 block; codes[f-1] = (blk_t, subface_t, ori)."""
+    return [[f, f, 1 if f != 2 else n, n if f != 1 else 1, 1 if f != 4 else n, n if f != 3 else 1,
+             1 if f != 6 else n, n if f != 5 else 1, codes[f - 1][0], codes[f - 1][1], codes[f - 1][2]]
+            for f in range(1, 7)]
+
+
+def subs_array(subs):
+    return np.array([r for s in subs for r in s], dtype=np.int32).reshape(-1, 11)
+
+
+def case_kernels():
+    R.setup(solver=0, ma=0.5, re=1e5)
+    n = 7
+    ext = R.begin([(n, n, n)], [box6(n, [(-10, 0, 0)] * 6)])[0]
+    sc = R.scalars()
+    rng = np.random.default_rng(0)
+    q, dx, ij = rand_state(ext, rng)
+    g, cv = sc["gamma"], sc["cv"]
+    shp = q.shape[:-1]
+    amu = np.zeros(shp + (3,))
+    amu[..., 0] = 1
+    amu[..., 1] = 0.01 * rng.random(shp)
+    amu[..., 2] = amu[..., 0] + amu[..., 1]
+    dt = 0.01 * (1 + rng.random(shp))
+    rhsv_in = 0.01 * rng.standard_normal(q.shape)
+    q0 = np.ascontiguousarray(np.stack([q * 0.999, q * 1.0005], axis=-1).reshape(shp + (10,)))
+    out = dict(ext=np.array(ext.as_list(), dtype=np.int32), q=q, dxidx=dx, inv_j=ij, amu=amu, dt=dt,
+               rhsv_in=rhsv_in, q0=q0,
+               scalars=np.array([sc[k] for k in ("gamma", "cv", "re", "tinf", "prl", "prt")]))
+    pr = R.getpri(ext, q, g, cv)
+    out["pri_v"] = pr
+    for c in (1, 2, 3):
+        for d in (1, -1):
+            out["weno3_c%d_%s" % (c, "l" if d == 1 else "r")] = R.weno3(ext, q, c, d)
+    ri = R.inviscid(ext, q, dx, ij, g)
+    out["rhsi"] = ri
+    rv, am, vor, dudx, tao = R.viscous(ext, pr, amu, dx, ij, sc["tinf"], sc["prl"], sc["prt"], g, cv, sc["re"])
+    out.update(viscous_rhsv=rv, viscous_amu=am, viscous_vor=vor, viscous_dudx=dudx, viscous_tao=tao)
+    for it in (0, 1):
+        d2, cfl2, st = R.find_dt(ext, dt, pr, amu, dx, sc["re"], 0.5, it)
+        out["find_dt%d_dt" % it] = d2
+        out["find_dt%d_cfl" % it] = np.array([cfl2])
+        out["find_dt%d_stats" % it] = np.asarray(st, dtype=np.float64)
+    a = R.implicit1st(ext, q, dt, pr, ri, rhsv_in, dx, ij, g, 0.5)
+    out["implicit1st_q"], out["implicit1st_rhs"] = a[0], a[1]
+    a = R.crank2nd(ext, q, q0, dt, pr, ri, rhsv_in, dx, ij, g)
+    out["crank2nd_q"], out["crank2nd_rhs"] = a[0], a[1]
+    jq = np.array([[1.1, 0.3, -0.2, 0.1, 3.0], [0.9, -0.5, 0.4, 0.2, 2.5]])
+    jm = np.array([0.3, -0.7, 1.2])
+    out["calc_jm_q"], out["calc_jm_m"] = jq, jm
+    out["calc_jm"] = np.stack([R.calc_jm(v, jm, g) for v in jq])
+    return out
+
+
+def case_sa(des):
+    R.setup(solver=0, turb=2, des=des, itime=0, ma=0.5, re=1e5)
+    n = 6
+    ext = R.begin([(n, n, n)], [box6(n, [(-20, 0, 0)] * 6)])[0]
+    rng = np.random.default_rng(10 + des)
+    q, dx, ij = rand_state(ext, rng)
+    sc = R.scalars()
+    pr = O.getpri(ext, q, sc["gamma"], sc["cv"])[0]
+    shp = q.shape[:-1]
+    amu = np.zeros(shp + (3,))
+    amu[..., 0] = 1
+    nut = 1 + rng.random(shp)
+    dt = 0.01 * (1 + rng.random(shp))
+    dudx = rng.standard_normal(shp + (9,))
+    vor = np.abs(rng.standard_normal(shp))
+    al, be, ga = [10 + rng.random(shp + (3,)) for _ in range(3)]
+    dist = 0.01 + rng.random(shp)
+    spac = 0.05 + 0.1 * rng.random(shp)
+    R.set_flag("iflag_des", des)
+    nut_o, sa_rhs, length = R.sa_model(ext, nut, dt, pr, amu, dudx, vor, dx, ij, al, be, ga, dist, spac,
+                                       sc["re"], 0.5)
+    return dict(ext=np.array(ext.as_list(), dtype=np.int32), des=np.array([des], dtype=np.int32),
+                re=np.array([sc["re"]]), cfl=np.array([0.5]), nut_in=nut, dt=dt, pri_v=pr, amu=amu,
+                dudx=dudx, vor=vor, dxidx=dx, inv_j=ij, aalpha=al, bbeta=be, ggamma=ga, dist=dist,
+                spacing=spac, nut=nut_o, sa_rhs=sa_rhs, length=length)
+
+
+def case_halo(ori):
+    n = 5
+    R.setup(solver=0, turb=2, itime=0)
+    b1 = box6(n, [(-10, 0, 0), (2, 3, ori), (-20, 0, 0), (2, 6, ori), (-30, 0, 0), (2, 1, ori)])
+    b2 = box6(n, [(1, 6, ori), (-40, 0, 0), (1, 2, ori), (-50, 0, 0), (-10, 0, 0), (1, 4, ori)])
+    exts = R.begin([(n, n, n)] * 2, [b1, b2])
+    rng = np.random.default_rng(20 + ori)
+    out = dict(dims=np.array([(n, n, n)] * 2, dtype=np.int32), subfaces=subs_array([b1, b2]),
+               nsub=np.array([6, 6], dtype=np.int32))
+    for m, e in enumerate(exts):
+        q = 1 + rng.random(e.shape + (5,))
+        nut = rng.random(e.shape)
+        R.put(m + 1, "q", q)
+        R.put(m + 1, "nut", nut)
+        out["ext%d" % m] = np.array(e.as_list(), dtype=np.int32)
+        out["q_in%d" % m], out["nut_in%d" % m] = q, nut
+    R.update_buffer()
+    R.average()
+    for m, e in enumerate(exts):
+        out["q%d" % m] = R.get(m + 1, "q", e)
+        out["nut%d" % m] = R.get(m + 1, "nut", e)
+    return out
+
+
+STEP_INPUTS = ("q", "q0", "dxidx", "inv_j", "amu", "dt", "nut", "dist", "spacing", "aalpha", "bbeta",
+               "ggamma", "pri_v")
+STEP_OUTPUTS = ("q", "nut", "rhs", "dt", "sa_rhs", "pri_v", "amu")
+
+
+def case_steps(solver, turb, tadv, nsteps=3):
+    n = 5
+    itime = 0 if turb else 1
+    R.setup(solver=solver, turb=turb, itime=itime, tadv=tadv, ma=0.5, re=1e5, cfl=0.5, dt=0.01)
+    b1 = box6(n, [(-10, 0, 0), (2, 1, 2), (-20, 0, 0), (-30, 0, 0), (-50, 0, 0), (-40, 0, 0)])
+    b2 = box6(n, [(1, 2, 2), (-40, 0, 0), (-20, 0, 0), (-10, 0, 0), (-30, 0, 0), (-10, 0, 0)])
+    exts = R.begin([(n, n, n)] * 2, [b1, b2])
+    sc = R.scalars()
+    rng = np.random.default_rng(30 + solver + 2 * turb + 4 * tadv)
+    out = dict(dims=np.array([(n, n, n)] * 2, dtype=np.int32), subfaces=subs_array([b1, b2]),
+               nsub=np.array([6, 6], dtype=np.int32),
+               flags=np.array([solver, turb, itime, tadv, nsteps], dtype=np.int32),
+               params=np.array([0.5, 1e5, 288.15, 0.5, 0.01, 0.0]))  # ma, re, tinf, cfl, dt, aoa
+    for m, e in enumerate(exts):
+        q, dx, ij = rand_state(e, rng, amp=0.03)
+        shp = e.shape
+        f = dict(q=q, dxidx=dx, inv_j=ij)
+        f["amu"] = np.zeros(shp + (3,))
+        f["amu"][..., 0] = 1
+        f["amu"][..., 2] = 1
+        f["dt"] = np.full(shp, 0.01)
+        f["nut"] = 1 + rng.random(shp)
+        f["dist"] = 0.01 + rng.random(shp)
+        f["spacing"] = 0.1 + 0.1 * rng.random(shp)
+        f["q0"] = np.ascontiguousarray(np.stack([q, q], axis=-1).reshape(shp + (5 * 2,)))
+        for a in ("aalpha", "bbeta", "ggamma"):
+            f[a] = 10 + rng.random(shp + (3,))
+        f["pri_v"] = O.getpri(e, q, sc["gamma"], sc["cv"])[0]
+        out["ext%d" % m] = np.array(e.as_list(), dtype=np.int32)
+        for k in STEP_INPUTS:
+            R.put(m + 1, k, f[k])
+            out["in_%s%d" % (k, m)] = f[k]
+    for _ in range(nsteps):
+        R.step()
+    for m, e in enumerate(exts):
+        for k in STEP_OUTPUTS:
+            out["out_%s%d" % (k, m)] = R.get(m + 1, k, e)
+    return out
+
+
+STEP_CASES = [(1, 0, 2), (0, 0, 2), (0, 2, 2), (0, 0, 3), (1, 0, 3)]
+
+
+def main():
+    if not R.available():
+        sys.exit("make_golden.py: the compiled reference (oracle/_ref) is not built; run make -C oracle ref")
+    cwd = os.getcwd()
+    work = tempfile.mkdtemp()
+    os.chdir(work)  # the reference's step writes result/resi_history.dat
+    os.makedirs("result", exist_ok=True)
+    open("result/resi_history.dat", "w").close()
+    made = {}
+    try:
+        made["kernels"] = case_kernels()
+        for des in (0, 1, 2):
+            made["sa_des%d" % des] = case_sa(des)
+        for ori in (1, 2, 3, 4):
+            made["halo_ori%d" % ori] = case_halo(ori)
+        for s, t, a in STEP_CASES:
+            made["steps_s%d_t%d_a%d" % (s, t, a)] = case_steps(s, t, a)
+    finally:
+        os.chdir(cwd)
+    manifest = {}
+    for name, arrs in made.items():
+        path = os.path.join(HERE, name + ".npz")
+        np.savez_compressed(path, **arrs)
+        manifest[name] = {"file": name + ".npz", "arrays": sorted(arrs),
+                          "bytes": os.path.getsize(path)}
+    manifest["_source"] = ("jiamin0329/aerocfd src/*.f90 compiled with amdflang -O2 "
+                           "(oracle/Makefile target ref), driven by oracle/ref_harness.f90")
+    with open(os.path.join(HERE, "manifest.json"), "w") as fh:
+        json.dump(manifest, fh, indent=1, sort_keys=True)
+    print("wrote %d fixtures, %d bytes" % (len(made), sum(v["bytes"] for k, v in manifest.items()
+                                                          if not k.startswith("_"))))
+
+
+if __name__ == "__main__":
+    main()
