@@ -49,7 +49,6 @@ struct Block {
     Ext e;
     Fields f;
     double *arena = nullptr;
-    double *uc = nullptr;   // dq*/dq in uncached memory (tile sweeps)
     int tile_shape = -1;    // chosen by the autotuner on first use
     std::vector<acfd_subface> sub;
 };
@@ -98,9 +97,8 @@ struct acfd_ctx {
     unsigned int *d_nneg = nullptr;
     double *d_res = nullptr;       // 6 sums
     double *d_partial = nullptr;   // residual partials
-    int *d_tile_flags = nullptr;   // [0] error word, [1..] tile counters
-    long tile_flags_n = 0;
-    bool flags_uc = false;
+    LuWs lu{nullptr, nullptr, nullptr};  // tile-sweep workspace (lusgs_kernel 1)
+    long lu_ex_n = 0;
     double *d_stage = nullptr;     // upload/download staging
     long stage_n = 0;
     // pointer tables for the multi-block copy kernels
@@ -135,8 +133,6 @@ void free_block(Block &b)
 {
     if (b.arena) (void)hipFree(b.arena);
     b.arena = nullptr;
-    if (b.uc) uc_release(b.uc);
-    b.uc = nullptr;
 }
 
 template <typename T>
@@ -267,7 +263,6 @@ int alloc_block(acfd_ctx *ctx, Block &b)
     const bool sa = ctx->c.turb == 2 && ctx->c.solver == 0;
     const bool bdf = ctx->cfg.iflag_timeadvance == 3;
     // the tile sweeps exchange dq* and dq through uncached memory
-    const bool uc = ctx->cfg.lusgs_kernel == 1;
     // field table: pointer slot and component count
     struct Slot {
         double **p;
@@ -277,7 +272,7 @@ int alloc_block(acfd_ctx *ctx, Block &b)
         {&b.f.inv_j, 1}, {&b.f.amu, 3}, {&b.f.dt, 1}, {&b.f.dudx, 9}, {&b.f.tao, 6}, {&b.f.vor, 1},
         {&b.f.aal, sa ? 3 : 0}, {&b.f.bbe, sa ? 3 : 0}, {&b.f.gga, sa ? 3 : 0}, {&b.f.dist, sa ? 1 : 0},
         {&b.f.spacing, sa ? 1 : 0}, {&b.f.length, sa ? 1 : 0}, {&b.f.nut, 1}, {&b.f.sa_rhs, 1},
-        {&b.f.q0, bdf ? 10 : 0}, {&b.f.w_flux, 12}, {&b.f.w_dqs, uc ? 0 : 5}, {&b.f.w_dq, uc ? 0 : 5},
+        {&b.f.q0, bdf ? 10 : 0}, {&b.f.w_flux, 12}, {&b.f.w_dqs, 5}, {&b.f.w_dq, 5},
         {&b.f.w_diag, 1}, {&b.f.w_r, 3}, {&b.f.w_sa, sa ? 9 : 0},
     };
     long tot = 0;
@@ -288,13 +283,6 @@ int alloc_block(acfd_ctx *ctx, Block &b)
     for (auto &s : slots) {
         *s.p = s.nc ? b.arena + off : nullptr;
         off += (long)s.nc * e.n;
-    }
-    if (uc) {
-        b.uc = (double *)uc_acquire((size_t)10 * e.n * sizeof(double));
-        if (!b.uc) return fail(ACFD_EDEVICE, "uncached device memory for the tile sweep is unavailable");
-        HIP_TRY(hipMemsetAsync(b.uc, 0, (size_t)10 * e.n * sizeof(double), ctx->stream));
-        b.f.w_dqs = b.uc;
-        b.f.w_dq = b.uc + 5 * e.n;
     }
     return 0;
 }
@@ -337,31 +325,45 @@ int ensure_stage(acfd_ctx *ctx, long n)
     return 0;
 }
 
-// Progress counters of the tile sweeps (uncached memory) or of nothing
-// (hyperplane sweeps).  The counters must start at or below the current
-// sweep epoch: fresh uncached memory is not zeroed and a recycled chunk may
-// have held other data, so the whole array is cleared here.
-int alloc_tile_flags(acfd_ctx *ctx, long n)
+// Workspace of the tile sweeps, sized for the largest block and every tile
+// shape: the sweep-layout operand array (ordinary device memory), the
+// exchange slots (uncached memory, filled with the sentinel; every sweep
+// leaves them that way) and an error word.  Nothing for hyperplane sweeps.
+int alloc_lu_ws(acfd_ctx *ctx)
 {
-    ctx->tile_flags_n = n;
-    if (ctx->cfg.lusgs_kernel == 1) {
-        ctx->d_tile_flags = (int *)uc_acquire((size_t)n * sizeof(int));
-        if (!ctx->d_tile_flags) return fail(ACFD_EDEVICE, "uncached device memory for the tile counters is unavailable");
-        ctx->flags_uc = true;
-        HIP_TRY(hipMemset(ctx->d_tile_flags, 0, (size_t)n * sizeof(int)));
-    } else {
-        HIP_TRY(hipMalloc((void **)&ctx->d_tile_flags, (size_t)n * sizeof(int)));
-        HIP_TRY(hipMemset(ctx->d_tile_flags, 0, (size_t)n * sizeof(int)));
+    if (ctx->cfg.lusgs_kernel != 1) return 0;
+    long sl_n = 0, ex_n = 0;
+    for (auto &b : ctx->blk) {
+        long a = 0, x = 0;
+        lusgs_ws_sizes(b.e, &a, &x);
+        sl_n = std::max(sl_n, a);
+        ex_n = std::max(ex_n, x);
     }
+    if (sl_n <= 0) return 0;
+    HIP_TRY(hipMalloc((void **)&ctx->lu.sl, (size_t)sl_n * sizeof(double)));
+    HIP_TRY(hipMalloc((void **)&ctx->lu.err, sizeof(int)));
+    HIP_TRY(hipMemset(ctx->lu.err, 0, sizeof(int)));
+    ctx->lu.ex = (double *)uc_acquire((size_t)ex_n * sizeof(double));
+    if (!ctx->lu.ex) return fail(ACFD_EDEVICE, "uncached device memory for the tile exchange is unavailable");
+    ctx->lu_ex_n = ex_n;
+    KRN_TRY(launch_fill_sentinel(ctx->stream, ctx->lu.ex, ex_n));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
     return 0;
 }
 
+// A tile sweep that gave up waiting leaves its results invalid and some
+// exchange slots written: report it and restore the slots.
 int tile_error(acfd_ctx *ctx)
 {
-    if (ctx->cfg.lusgs_kernel != 1 || !ctx->d_tile_flags) return 0;
+    if (!ctx->lu.err) return 0;
     int err = 0;
-    HIP_TRY(hipMemcpy(&err, ctx->d_tile_flags, sizeof(int), hipMemcpyDeviceToHost));
-    if (err) return fail(ACFD_EDEVICE, "pipelined LU-SGS sweep timed out waiting for a neighbouring tile");
+    HIP_TRY(hipMemcpy(&err, ctx->lu.err, sizeof(int), hipMemcpyDeviceToHost));
+    if (err) {
+        (void)launch_fill_sentinel(ctx->stream, ctx->lu.ex, ctx->lu_ex_n);
+        (void)hipMemsetAsync(ctx->lu.err, 0, sizeof(int), ctx->stream);
+        (void)hipStreamSynchronize(ctx->stream);
+        return fail(ACFD_EDEVICE, "pipelined LU-SGS sweep timed out waiting for a neighbouring tile");
+    }
     return 0;
 }
 
@@ -447,12 +449,13 @@ int st_find_dt(acfd_ctx *ctx, int m)
     return 0;
 }
 
-// sub-stage timing ids of the two LU-SGS sweeps
-constexpr int kTimeLuForward = 12, kTimeLuBackward = 13;
+// sub-stage timing ids of the LU-SGS phases
+constexpr int kTimeLuForward = 12, kTimeLuBackward = 13, kTimeLuLayout = 14, kTimeLuOther = 15;
 
-// Autotuner of the tile sweep: on a block's first sweep, time the forward
-// sweep once per tile shape (the sweep recomputes dq* from unchanged inputs,
-// so repeating it changes nothing) and keep the fastest shape.
+// Autotuner of the tile sweep: on a block's first update, time the gather
+// and forward sweep once per tile shape (they recompute dq* in the sweep
+// layout from unchanged inputs, so repeating them changes nothing) and keep
+// the fastest shape.
 int tune_tiles(acfd_ctx *ctx, Block &b)
 {
     if (const char *fix = std::getenv("ACFD_TILE_SHAPE")) {
@@ -470,7 +473,8 @@ int tune_tiles(acfd_ctx *ctx, Block &b)
     for (int s = 0; s < lusgs_tile_shapes(); s++) {
         if (lusgs_tiles(b.e, s) <= 0) continue;
         (void)hipEventRecord(e0, ctx->stream);
-        int rc = launch_lusgs_phase(ctx->stream, b.e, b.f, ctx->c, ctx->d_cfl, 1, ctx->d_tile_flags, 1, s);
+        int rc = launch_lusgs_phase(ctx->stream, b.e, b.f, ctx->c, ctx->d_cfl, 1, &ctx->lu, 1, s);
+        if (!rc) rc = launch_lusgs_phase(ctx->stream, b.e, b.f, ctx->c, ctx->d_cfl, 1, &ctx->lu, 2, s);
         (void)hipEventRecord(e1, ctx->stream);
         if (rc) return fail(ACFD_EDEVICE, "tile sweep launch failed during tuning");
         HIP_TRY(hipEventSynchronize(e1));
@@ -494,13 +498,14 @@ int st_implicit(acfd_ctx *ctx, int m)
     StageTimer t(ctx, ACFD_STAGE_IMPLICIT1ST);
     Block &b = ctx->blk[m];
     const int variant = ctx->cfg.lusgs_kernel;
-    for (int ph = 0; ph < 4; ph++) {
+    for (int ph = 0; ph < kLusgsPhases; ph++) {
         if (ph == 1 && variant == 1 && b.tile_shape < 0) {
             int rc = tune_tiles(ctx, b);
             if (rc) return rc;
         }
-        StageTimer ts(ctx, ph == 1 ? kTimeLuForward : ph == 2 ? kTimeLuBackward : 15);
-        KRN_TRY(launch_lusgs_phase(ctx->stream, b.e, b.f, ctx->c, ctx->d_cfl, variant, ctx->d_tile_flags, ph,
+        StageTimer ts(ctx, ph == 2 ? kTimeLuForward : ph == 3 ? kTimeLuBackward
+                           : (ph == 1 || ph == 4) ? kTimeLuLayout : kTimeLuOther);
+        KRN_TRY(launch_lusgs_phase(ctx->stream, b.e, b.f, ctx->c, ctx->d_cfl, variant, &ctx->lu, ph,
                                    b.tile_shape < 0 ? 0 : b.tile_shape));
     }
     return 0;
@@ -680,10 +685,9 @@ int acfd_destroy(acfd_ctx *ctx)
     if (!ctx) return 0;
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     for (auto &b : ctx->blk) free_block(b);
-    if (ctx->d_tile_flags) {
-        if (ctx->flags_uc) uc_release(ctx->d_tile_flags);
-        else (void)hipFree(ctx->d_tile_flags);
-    }
+    if (ctx->lu.ex) uc_release(ctx->lu.ex);
+    if (ctx->lu.sl) (void)hipFree(ctx->lu.sl);
+    if (ctx->lu.err) (void)hipFree(ctx->lu.err);
     void *ptrs[] = {ctx->d_cfl, ctx->d_stats, ctx->d_nneg, ctx->d_res, ctx->d_partial,
                     ctx->d_stage, ctx->d_qtab, ctx->d_nuttab, ctx->d_ntab, ctx->d_h_sb, ctx->d_h_db,
                     ctx->d_h_sn, ctx->d_h_dn, ctx->d_a_sb, ctx->d_a_gb, ctx->d_a_gs, ctx->d_a_sn,
@@ -840,20 +844,18 @@ int acfd_set_topology(acfd_ctx *ctx, int nblk, const int *dims, const int *nsub,
     std::vector<long> ntab(nblk);
     long maxn = 0;
     int parts = 0;
-    long tiles = 0;
     for (int m = 0; m < nblk; m++) {
         qt[m] = ctx->blk[m].f.q;
         nt[m] = ctx->blk[m].f.nut;
         ntab[m] = ctx->blk[m].e.n;
         maxn = std::max(maxn, ctx->blk[m].e.n);
         parts += residual_partials(ctx->blk[m].e);
-        tiles = std::max(tiles, lusgs_tile_flags_needed(ctx->blk[m].e));
     }
     if ((rc = dev_upload(&ctx->d_qtab, qt)) || (rc = dev_upload(&ctx->d_nuttab, nt)) ||
         (rc = dev_upload(&ctx->d_ntab, ntab)))
         return rc;
     HIP_TRY(hipMalloc((void **)&ctx->d_partial, (size_t)parts * 6 * sizeof(double)));
-    rc = alloc_tile_flags(ctx, tiles);
+    rc = alloc_lu_ws(ctx);
     if (rc) return rc;
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     ctx->topo = true;
@@ -1088,7 +1090,7 @@ int make_single(const acfd_config *cfg, const int *r, acfd_ctx **out)
     dev_upload(&ctx->d_nuttab, nt);
     dev_upload(&ctx->d_ntab, nn);
     HIP_TRY(hipMalloc((void **)&ctx->d_partial, (size_t)residual_partials(e) * 6 * sizeof(double)));
-    rc = alloc_tile_flags(ctx, lusgs_tile_flags_needed(e));
+    rc = alloc_lu_ws(ctx);
     if (rc) return rc;
     ctx->totalnodes = (long)(e.ie0 - e.is0 + 1) * (e.je0 - e.js0 + 1) * (e.ke0 - e.ks0 + 1);
     ctx->topo = true;

@@ -8,18 +8,37 @@
 // Nodes on one hyperplane i+j+k = l are independent, and any schedule that
 // respects these three dependencies computes every node from the same
 // operands as the sequential loop.  Two schedules:
-//   hyperplanes: one launch per plane l (2 (Li+Lj+Lk-2) launches per step);
-//   tiles:       one persistent launch per sweep.  Each workgroup owns a
+//
+//   hyperplanes  one launch per plane l (2 (Li+Lj+Lk-2) launches per step),
+//                operands read in place.
+//
+//   tiles        one persistent launch per sweep.  Each workgroup owns a
 //                TJ x TK tile of (j,k) lines and marches along i on a skewed
-//                clock (thread (tj,tk) handles i = t - tj - tk at step t):
-//                the i neighbour stays in registers, the j and k neighbours
-//                inside the tile come through LDS, and the tile's first
-//                row/column read the neighbouring tile's output from global
-//                memory once that tile has published its step counter.
-//                dq*, dq and the counters live in uncached device memory, so
-//                a producer only has to wait for its stores to complete
-//                before publishing, and consumers read memory directly; no
-//                cache writeback or invalidation is needed across XCDs.
+//                clock: thread (tj,tk) handles the node at sweep position
+//                s - tj - tk at step s.
+//                * Sweep layout.  Before the forward sweep the operands of
+//                  every node (q, metric row, 1/J, spectral radii, diag, rhs:
+//                  24 values) are gathered into an array ordered by (tile,
+//                  step, field, lane), so at every step a tile reads 24
+//                  contiguous rows of TJ*TK values -- fully coalesced, where
+//                  the natural layout would have every lane on its own cache
+//                  line.  The sweeps write dq* and dq over the rhs slots and a
+//                  final pass adds dq to q.
+//                * Push form.  A node, once solved, forms the three
+//                  off-diagonal products 1/2 inv_j (A(q; grad xi_d) +- r_d I) dq
+//                  that its successors need, with its own operands: the i
+//                  product stays in registers, the j and k products go to the
+//                  next lanes through LDS, and the tile's last row/column
+//                  hand them to the neighbouring tile through exchange slots.
+//                  Each product is the same expression of the same operands
+//                  as in the pull form, so both schedules give the same bits.
+//                * Exchange.  Slots live in uncached memory and hold a
+//                  sentinel bit pattern (a signalling NaN no arithmetic
+//                  produces) until the producer stores the value; the
+//                  consumer spins on the value itself and writes the sentinel
+//                  back.  No counters, no fences, no waits for store
+//                  acknowledgements on the per-step path; values are loaded
+//                  one step ahead so that in steady state the wait is hidden.
 //
 // The flux Jacobian products use the closed form of A(q) v for the Euler
 // flux in generalised coordinates (one reciprocal of rho instead of the ~25
@@ -36,7 +55,7 @@
 
 namespace acfd {
 
-// 0.5 * (A(q_n; m) v +- r_n v) * inv_j_n, with m the metric row c0 of node n
+// 0.5 * (A(q; m) v +- r v) * inv_j for the operands of one node
 // (sub_timeadvancement_1stimplicit.f90:283-301 forward, :459-477 backward).
 // A v = [ m.v_mom,
 //         U v_mom + u theta + m dp,
@@ -44,14 +63,10 @@ namespace acfd {
 // with u = mom/rho, U = m.u, theta = m.v_mom - U v1,
 // dp = (gamma-1)(|u|^2/2 v1 - u.v_mom + v5), a1 = gamma E/rho - (gamma-1)|u|^2/2.
 template <int SGN>
-__device__ inline void jac_mv(const double *__restrict__ q, const double *__restrict__ dxidx,
-                              const double *__restrict__ inv_j, const double *__restrict__ rr, long N, long n,
-                              int c0, double gamma, const double *v, double *out)
+__device__ inline void jac_mv_r(const double *qv, double mx, double my, double mz, double ij, double rr,
+                                double gamma, const double *v, double *out)
 {
-    // every fused multiply-add is written out (the file is compiled without
-    // contraction), so both schedules round identically
-    const double rho = q[n], mu = q[N + n], mv = q[2 * N + n], mw = q[3 * N + n], E = q[4 * N + n];
-    const double mx = dxidx[c0 * N + n], my = dxidx[(c0 + 1) * N + n], mz = dxidx[(c0 + 2) * N + n];
+    const double rho = qv[0], mu = qv[1], mv = qv[2], mw = qv[3], E = qv[4];
     const double ri = 1.0 / rho;
     const double u = mu * ri, w2 = mv * ri, w3 = mw * ri;
     const double U = fma(mz, w3, fma(my, w2, mx * u));
@@ -62,13 +77,40 @@ __device__ inline void jac_mv(const double *__restrict__ q, const double *__rest
     const double theta = fma(-U, v[0], mvm);
     const double uv = fma(w3, v[3], fma(w2, v[2], u * v[1]));
     const double dp = g1 * (fma(0.5 * q2, v[0], -uv) + v[4]);
-    const double r = SGN > 0 ? rr[n] : -rr[n];
-    const double s = 0.5 * inv_j[n];
+    const double r = SGN > 0 ? rr : -rr;
+    const double s = 0.5 * ij;
     out[0] = s * fma(r, v[0], mvm);
     out[1] = s * fma(r, v[1], fma(mx, dp, fma(u, theta, U * v[1])));
     out[2] = s * fma(r, v[2], fma(my, dp, fma(w2, theta, U * v[2])));
     out[3] = s * fma(r, v[3], fma(mz, dp, fma(w3, theta, U * v[3])));
     out[4] = s * fma(r, v[4], fma(U, dp + v[4], a1 * theta));
+}
+
+// the same product with the operands of node n read from the SoA fields
+template <int SGN>
+__device__ inline void jac_mv(const double *__restrict__ q, const double *__restrict__ dxidx,
+                              const double *__restrict__ inv_j, const double *__restrict__ rr, long N, long n,
+                              int c0, double gamma, const double *v, double *out)
+{
+    const double qv[5] = {q[n], q[N + n], q[2 * N + n], q[3 * N + n], q[4 * N + n]};
+    jac_mv_r<SGN>(qv, dxidx[c0 * N + n], dxidx[(c0 + 1) * N + n], dxidx[(c0 + 2) * N + n], inv_j[n], rr[n],
+                  gamma, v, out);
+}
+
+// forward node (:345-365): dq* = (rhs + A+ dq*_i-1 + B+ dq*_j-1 + C+ dq*_k-1)/diag/inv_j
+__device__ inline void lu_fwd_combine(const double *rhs, const double *ta, const double *tb, const double *tc,
+                                      double dg, double ij, double *res)
+{
+#pragma unroll
+    for (int x = 0; x < 5; x++) res[x] = (rhs[x] + (ta[x] + tb[x] + tc[x])) / dg / ij;
+}
+
+// backward node (:521-542): dq = dq* - (A- dq_i+1 + B- dq_j+1 + C- dq_k+1)/inv_j/diag
+__device__ inline void lu_bwd_combine(const double *dqs, const double *ta, const double *tb, const double *tc,
+                                      double dg, double ij, double *res)
+{
+#pragma unroll
+    for (int x = 0; x < 5; x++) res[x] = dqs[x] - (ta[x] + tb[x] + tc[x]) / ij / dg;
 }
 
 struct LuArgs {
@@ -78,37 +120,35 @@ struct LuArgs {
     double gamma;
 };
 
-// forward node (:345-365): dq* = (rhs + A+ dq*_i-1 + B+ dq*_j-1 + C+ dq*_k-1)/diag/inv_j
 __device__ inline void lu_forward(const LuArgs &a, long n, const double *vi, const double *vj,
                                   const double *vk, double *res)
 {
     const long N = a.e.n, sj = a.e.ni, sk = (long)a.e.ni * a.e.nj;
-    double ta[5], tb[5], tc[5];
+    double ta[5], tb[5], tc[5], rhs[5];
     jac_mv<1>(a.q, a.dxidx, a.inv_j, a.rr, N, n - 1, 0, a.gamma, vi, ta);
     jac_mv<1>(a.q, a.dxidx, a.inv_j, a.rr + N, N, n - sj, 3, a.gamma, vj, tb);
     jac_mv<1>(a.q, a.dxidx, a.inv_j, a.rr + 2 * N, N, n - sk, 6, a.gamma, vk, tc);
-    const double dg = a.diag[n], ij = a.inv_j[n];
 #pragma unroll
-    for (int x = 0; x < 5; x++) res[x] = (a.rhs[x * N + n] + (ta[x] + tb[x] + tc[x])) / dg / ij;
+    for (int x = 0; x < 5; x++) rhs[x] = a.rhs[x * N + n];
+    lu_fwd_combine(rhs, ta, tb, tc, a.diag[n], a.inv_j[n], res);
 }
 
-// backward node (:521-542): dq = dq* - (A- dq_i+1 + B- dq_j+1 + C- dq_k+1)/inv_j/diag
 __device__ inline void lu_backward(const LuArgs &a, long n, const double *vi, const double *vj,
                                    const double *vk, double *res)
 {
     const long N = a.e.n, sj = a.e.ni, sk = (long)a.e.ni * a.e.nj;
-    double ta[5], tb[5], tc[5];
+    double ta[5], tb[5], tc[5], dqs[5];
     jac_mv<-1>(a.q, a.dxidx, a.inv_j, a.rr, N, n + 1, 0, a.gamma, vi, ta);
     jac_mv<-1>(a.q, a.dxidx, a.inv_j, a.rr + N, N, n + sj, 3, a.gamma, vj, tb);
     jac_mv<-1>(a.q, a.dxidx, a.inv_j, a.rr + 2 * N, N, n + sk, 6, a.gamma, vk, tc);
-    const double dg = a.diag[n], ij = a.inv_j[n];
 #pragma unroll
-    for (int x = 0; x < 5; x++) res[x] = a.dqs[x * N + n] - (ta[x] + tb[x] + tc[x]) / ij / dg;
+    for (int x = 0; x < 5; x++) dqs[x] = a.dqs[x * N + n];
+    lu_bwd_combine(dqs, ta, tb, tc, a.diag[n], a.inv_j[n], res);
 }
 
 // rhs = rhsi + rhsv on the computed range; spectral radii ra, rb, rc and
-// diag on the ring is1-1..ie1+1 (:72-156); dq* and dq cleared on the ring so
-// that out-of-range neighbours read 0 (:105-106).
+// diag on the ring is1-1..ie1+1 (:72-156); with clear_dq, dq* and dq cleared
+// on the ring so that out-of-range neighbours read 0 (:105-106).
 __global__ void __launch_bounds__(256) k_lu_prep(Ext e, Box ring, const double *__restrict__ pri,
                                                  const double *__restrict__ dxidx,
                                                  const double *__restrict__ dt,
@@ -116,7 +156,7 @@ __global__ void __launch_bounds__(256) k_lu_prep(Ext e, Box ring, const double *
                                                  const double *__restrict__ rhsv,
                                                  double *__restrict__ rhs, double *__restrict__ rr,
                                                  double *__restrict__ diag, double *__restrict__ dqs,
-                                                 double *__restrict__ dq, const double *cfl)
+                                                 double *__restrict__ dq, const double *cfl, int clear_dq)
 {
     int i, j, k;
     if (!box_node(ring, i, j, k)) return;
@@ -138,8 +178,10 @@ __global__ void __launch_bounds__(256) k_lu_prep(Ext e, Box ring, const double *
     bool inner = i >= e.is1 && i <= e.ie1 && j >= e.js1 && j <= e.je1 && k >= e.ks1 && k <= e.ke1;
 #pragma unroll
     for (int x = 0; x < 5; x++) {
-        dqs[x * N + n] = 0.0;
-        dq[x * N + n] = 0.0;
+        if (clear_dq) {
+            dqs[x * N + n] = 0.0;
+            dq[x * N + n] = 0.0;
+        }
         if (inner) rhs[x * N + n] = rhsi[x * N + n] + rhsv[x * N + n];
     }
 }
@@ -172,102 +214,6 @@ __global__ void __launch_bounds__(256) k_lu_plane(LuArgs a, int l, int ja, int k
     for (int x = 0; x < 5; x++) out[x * N + n] = res[x];
 }
 
-// Wait (thread 0 only) until the counter at f reaches need.  The wait is
-// bounded: running out sets *err and stops further waiting, so the sweep
-// always ends (its results are then flagged as invalid by the host).
-__device__ inline void wait_progress(const int *f, int need, int &known, bool &failed, int *err)
-{
-    if (!f || failed || known >= need) return;
-    int v = 0;
-    for (long spins = 0;; spins++) {
-        v = __hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (v >= need) break;
-        if (spins > (1L << 20)) {
-            atomicOr(err, 1);
-            failed = true;
-            break;
-        }
-        __builtin_amdgcn_s_sleep(1);
-    }
-    known = v;
-}
-
-// Exchange values between tiles are read and written at agent scope: the
-// accesses bypass the CU's vector L1, and the array lives in uncached memory
-// so no XCD's L2 holds a stale copy.  No acquire/release fence is needed (a
-// fence would write back and invalidate the whole L2 at every step).
-__device__ inline double xload(const double *p)
-{
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ inline void xstore(double *p, double v)
-{
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Persistent tile sweep; flags[tile] = epoch + completed steps of the tile.
-// Tiles are numbered bj + nbj*bk and only wait on lower numbers, which are
-// dispatched first, so every wait is on a tile that is running or done.
-template <int DIR, int TJ, int TK>
-__global__ void __launch_bounds__(TJ *TK) k_lu_tiles(LuArgs a, int *flags, int epoch, int *err)
-{
-    __shared__ double sv[2][5][TK][TJ + 1];
-    const Ext &e = a.e;
-    const int Li = e.ie1 - e.is1 + 1, Lj = e.je1 - e.js1 + 1, Lk = e.ke1 - e.ks1 + 1;
-    const int nbj = (Lj + TJ - 1) / TJ;
-    const int tile = blockIdx.x;
-    const int bj = tile % nbj, bk = tile / nbj;
-    const int tj = threadIdx.x % TJ, tk = threadIdx.x / TJ;
-    const int lj = bj * TJ + tj, lk = bk * TK + tk;
-    const bool active = lj < Lj && lk < Lk;
-    const long N = e.n, sj = e.ni, sk = (long)e.ni * e.nj;
-    const int nsteps = Li + TJ + TK - 2;
-    double *out = DIR > 0 ? a.dqs : a.dq;
-    int *myflag = flags + tile;
-    const int *fj = bj > 0 ? flags + (tile - 1) : nullptr;
-    const int *fk = bk > 0 ? flags + (tile - nbj) : nullptr;
-    int known_j = 0, known_k = 0;
-    bool failed = false;
-    double vi[5] = {0, 0, 0, 0, 0};
-    for (int t = 0; t < nsteps; t++) {
-        if (threadIdx.x == 0) {
-            // the neighbour tile's thread on the shared row/column handles
-            // this plane TJ-1 (TK-1) steps later on its own clock
-            wait_progress(fj, epoch + min(t + TJ, nsteps), known_j, failed, err);
-            wait_progress(fk, epoch + min(t + TK, nsteps), known_k, failed, err);
-        }
-        __syncthreads();
-        const int li = t - tj - tk;
-        if (active && li >= 0 && li < Li) {
-            int i = DIR > 0 ? e.is1 + li : e.ie1 - li;
-            int j = DIR > 0 ? e.js1 + lj : e.je1 - lj;
-            int k = DIR > 0 ? e.ks1 + lk : e.ke1 - lk;
-            long n = e.ix(i, j, k);
-            const int p = t & 1;
-            double vj[5], vk[5], res[5];
-#pragma unroll
-            for (int x = 0; x < 5; x++) {
-                vj[x] = tj > 0 ? sv[p ^ 1][x][tk][tj - 1] : xload(&out[x * N + n - DIR * sj]);
-                vk[x] = tk > 0 ? sv[p ^ 1][x][tk - 1][tj] : xload(&out[x * N + n - DIR * sk]);
-                if (li == 0) vi[x] = xload(&out[x * N + n - DIR]);
-            }
-            if (DIR > 0) lu_forward(a, n, vi, vj, vk, res);
-            else lu_backward(a, n, vi, vj, vk, res);
-#pragma unroll
-            for (int x = 0; x < 5; x++) {
-                xstore(&out[x * N + n], res[x]);
-                vi[x] = res[x];
-                sv[p][x][tk][tj] = res[x];
-            }
-        }
-        // every store of this step is acknowledged before the barrier, so
-        // the counter cannot overtake the values it announces
-        __asm__ volatile("s_waitcnt vmcnt(0)" ::: "memory");
-        __syncthreads();
-        if (threadIdx.x == 0) __hip_atomic_store(myflag, epoch + t + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-}
-
 // q += dq on the computed range (:568-574)
 __global__ void __launch_bounds__(256) k_add5(Ext e, Box b, double *__restrict__ q,
                                               const double *__restrict__ dq)
@@ -280,90 +226,375 @@ __global__ void __launch_bounds__(256) k_add5(Ext e, Box b, double *__restrict__
     for (int x = 0; x < 5; x++) q[x * N + n] = q[x * N + n] + dq[x * N + n];
 }
 
-// tile shapes the autotuner chooses from
-const int kTileShapes[][2] = {{16, 16}, {8, 8}, {16, 8}, {8, 4}};
+// ------------------------------------------------------------ tile sweeps
+// Sweep-layout fields: q 0-4, dxidx 5-13, inv_j 14, rr 15-17, diag 18,
+// rhs 19-23 (overwritten by dq* in the forward and dq in the backward sweep).
+constexpr int kSlF = 24;
+constexpr int kSlOut = 19;
+// exchange sentinel: a negative signalling NaN with a payload; arithmetic
+// only ever produces quiet NaNs
+constexpr unsigned long long kSentinel = 0xFFF5A5A5A5A5A5A5ull;
+// consecutive steps per workgroup of the layout passes
+constexpr int kChunk = 16;
+
+// geometry of one tiling of a block's computed range
+struct SlGeom {
+    int TJ, TK, TT, nbj, nbk, ntiles, nsteps, Li, Lj, Lk;
+    long exj_tile, exk_tile;  // doubles per tile of the j / k exchange regions
+    __host__ __device__ long slot(int tile, int t, int f, int l) const {
+        return (((long)tile * nsteps + t) * kSlF + f) * TT + l;
+    }
+};
+
+// tile shapes the autotuner chooses from (TJ, TK)
+const int kTileShapes[][2] = {{16, 16}, {16, 8}, {8, 8}, {16, 4}};
 const int kNumTileShapes = 4;
+
+static SlGeom sl_geom(const Ext &e, int shape)
+{
+    SlGeom g;
+    g.TJ = kTileShapes[shape][0];
+    g.TK = kTileShapes[shape][1];
+    g.TT = g.TJ * g.TK;
+    g.Li = e.ie1 - e.is1 + 1;
+    g.Lj = e.je1 - e.js1 + 1;
+    g.Lk = e.ke1 - e.ks1 + 1;
+    g.nbj = (g.Lj + g.TJ - 1) / g.TJ;
+    g.nbk = (g.Lk + g.TK - 1) / g.TK;
+    g.ntiles = g.nbj * g.nbk;
+    g.nsteps = g.Li + g.TJ + g.TK - 2;
+    g.exj_tile = (long)(g.Li + g.TK - 1) * g.TK * 5;
+    g.exk_tile = (long)(g.Li + g.TJ - 1) * g.TJ * 5;
+    return g;
+}
+
+__device__ inline unsigned long long xload_bits(const double *p)
+{
+    return __hip_atomic_load(reinterpret_cast<const unsigned long long *>(p), __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline void xstore_bits(double *p, unsigned long long v)
+{
+    __hip_atomic_store(reinterpret_cast<unsigned long long *>(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline void xstore(double *p, double v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Take the five values of one exchange slot: `pre` holds what was loaded a
+// step earlier; components still at the sentinel are polled until the
+// producer's store arrives.  The wait is bounded: running out sets *err and
+// every later wait of the launch gives up at once, so the sweep always ends
+// (the host then reports the failure).  The slot is reset for the next sweep.
+__device__ inline void ex_take(double *slot, const unsigned long long *pre, double *v, int *err, bool &failed)
+{
+#pragma unroll
+    for (int x = 0; x < 5; x++) {
+        unsigned long long b = pre[x];
+        if (b == kSentinel) {
+            for (long spins = 0; !failed; spins++) {
+                b = xload_bits(slot + x);
+                if (b != kSentinel) break;
+                if ((spins & 1023) == 1023 &&
+                    (spins > (1L << 21) || __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
+                    atomicOr(err, 1);
+                    failed = true;
+                }
+                __builtin_amdgcn_s_sleep(1);
+            }
+            if (failed) b = 0;
+        }
+        v[x] = __longlong_as_double((long long)b);
+        xstore_bits(slot + x, kSentinel);
+    }
+}
+
+// LDS barrier that does not wait for outstanding global loads (the
+// prefetched operands stay in flight across it)
+__device__ inline void lds_barrier()
+{
+    __asm__ volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+
+// Gather the 24 operands of every node into the sweep layout.  Workgroup =
+// (tile, chunk of kChunk steps); lane = (tj, tk).  One field at a time so a
+// lane walks its own line along i and reuses each cache line.
+__global__ void __launch_bounds__(256) k_sl_gather(Ext e, SlGeom g, const double *__restrict__ q,
+                                                   const double *__restrict__ dxidx,
+                                                   const double *__restrict__ inv_j,
+                                                   const double *__restrict__ rr,
+                                                   const double *__restrict__ diag,
+                                                   const double *__restrict__ rhs, double *__restrict__ sl)
+{
+    const int tile = blockIdx.x, t0 = blockIdx.y * kChunk;
+    const int l = threadIdx.x;
+    if (l >= g.TT) return;
+    const int tj = l % g.TJ, tk = l / g.TJ;
+    const int lj = (tile % g.nbj) * g.TJ + tj, lk = (tile / g.nbj) * g.TK + tk;
+    if (lj >= g.Lj || lk >= g.Lk) return;
+    const long N = e.n;
+    const long n0 = e.ix(e.is1, e.js1 + lj, e.ks1 + lk);
+    for (int f = 0; f < kSlF; f++) {
+        const double *src = f < 5 ? q + f * N : f < 14 ? dxidx + (f - 5) * N : f == 14 ? inv_j
+                          : f < 18 ? rr + (f - 15) * N : f == 18 ? diag : rhs + (f - 19) * N;
+        for (int t = t0; t < t0 + kChunk && t < g.nsteps; t++) {
+            const int li = t - tj - tk;
+            if (li >= 0 && li < g.Li) sl[g.slot(tile, t, f, l)] = src[n0 + li];
+        }
+    }
+}
+
+// q += dq from the sweep layout (:568-574), same work split as the gather
+__global__ void __launch_bounds__(256) k_sl_add(Ext e, SlGeom g, const double *__restrict__ sl,
+                                                double *__restrict__ q)
+{
+    const int tile = blockIdx.x, t0 = blockIdx.y * kChunk;
+    const int l = threadIdx.x;
+    if (l >= g.TT) return;
+    const int tj = l % g.TJ, tk = l / g.TJ;
+    const int lj = (tile % g.nbj) * g.TJ + tj, lk = (tile / g.nbj) * g.TK + tk;
+    if (lj >= g.Lj || lk >= g.Lk) return;
+    const long N = e.n;
+    const long n0 = e.ix(e.is1, e.js1 + lj, e.ks1 + lk);
+    for (int x = 0; x < 5; x++)
+        for (int t = t0; t < t0 + kChunk && t < g.nsteps; t++) {
+            const int li = t - tj - tk;
+            if (li >= 0 && li < g.Li) {
+                const long n = x * N + n0 + li;
+                q[n] = q[n] + sl[g.slot(tile, t, kSlOut + x, l)];
+            }
+        }
+}
+
+// Fill exchange slots with the sentinel.
+__global__ void __launch_bounds__(256) k_fill_sentinel(double *p, long n)
+{
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) xstore_bits(p + i, kSentinel);
+}
+
+// One step of a lane: combine, solve, push.  op = the node's 24 operands.
+template <int DIR>
+__device__ inline void lu_node(const double *op, const double *ta, const double *tb, const double *tc,
+                               double gamma, double *res, double *pa, double *pb, double *pc)
+{
+    if (DIR > 0) lu_fwd_combine(op + kSlOut, ta, tb, tc, op[18], op[14], res);
+    else lu_bwd_combine(op + kSlOut, ta, tb, tc, op[18], op[14], res);
+    jac_mv_r<DIR>(op, op[5], op[6], op[7], op[14], op[15], gamma, res, pa);
+    jac_mv_r<DIR>(op, op[8], op[9], op[10], op[14], op[16], gamma, res, pb);
+    jac_mv_r<DIR>(op, op[11], op[12], op[13], op[14], op[17], gamma, res, pc);
+}
+
+// Persistent tile sweep over the sweep layout.  The forward sweep numbers
+// tiles bj + nbj*bk and waits only on lower numbers; the backward sweep runs
+// the same tiles mirrored (lane (tj,tk) schedules as (TJ-1-tj, TK-1-tk)) and
+// is dispatched in reverse order.  Either way every wait is on a tile that
+// was dispatched earlier, so it is running or done.
+template <int DIR, int TJ, int TK>
+__global__ void __launch_bounds__(TJ *TK) k_lu_tiles(SlGeom g, double *__restrict__ sl, double gamma,
+                                                      double *ex, int *err)
+{
+    constexpr int TT = TJ * TK;
+    __shared__ double pj[2][5][TT], pk[2][5][TT];
+    const int tile = DIR > 0 ? (int)blockIdx.x : g.ntiles - 1 - (int)blockIdx.x;
+    const int bj = tile % g.nbj, bk = tile / g.nbj;
+    const int l = threadIdx.x, tj = l % TJ, tk = l / TJ;
+    const int lj = bj * TJ + tj, lk = bk * TK + tk;
+    const bool active = lj < g.Lj && lk < g.Lk;
+    // schedule coordinates (mirrored for the backward sweep)
+    const int sj = DIR > 0 ? tj : TJ - 1 - tj, sk = DIR > 0 ? tk : TK - 1 - tk;
+    // the j / k predecessor: outside the computed range (0), in the
+    // neighbouring tile (exchange slot) or the previous lane (LDS)
+    const bool j_ring = DIR > 0 ? lj == 0 : lj + 1 >= g.Lj;
+    const bool k_ring = DIR > 0 ? lk == 0 : lk + 1 >= g.Lk;
+    const bool j_in = !j_ring && sj == 0, k_in = !k_ring && sk == 0;
+    const int lpj = l - DIR, lpk = l - DIR * TJ;
+    // this lane's products leave the tile when it is last in schedule and a
+    // neighbouring tile exists on that side
+    const bool j_out = sj == TJ - 1 && (DIR > 0 ? bj + 1 < g.nbj : bj > 0);
+    const bool k_out = sk == TK - 1 && (DIR > 0 ? bk + 1 < g.nbk : bk > 0);
+    double *exj = ex, *exk = ex + (long)g.ntiles * g.exj_tile;
+    // slots: j region [tile][d = pos + sk][tk][5], k region [tile][d = pos + sj][tj][5]
+    double *jin = exj + (long)(DIR > 0 ? tile - 1 : tile + 1) * g.exj_tile + (long)tk * 5;
+    double *kin = exk + (long)(DIR > 0 ? tile - g.nbj : tile + g.nbj) * g.exk_tile + (long)tj * 5;
+    double *jout = exj + (long)tile * g.exj_tile + (long)tk * 5;
+    double *kout = exk + (long)tile * g.exk_tile + (long)tj * 5;
+    bool failed = false;
+    double ta[5] = {0, 0, 0, 0, 0};
+    double opA[kSlF], opB[kSlF];
+    unsigned long long xjA[5], xkA[5], xjB[5], xkB[5];
+
+    auto row = [&](int s) { return DIR > 0 ? s : g.nsteps - 1 - s; };
+    auto on_at = [&](int s) {
+        const int pos = s - sj - sk;
+        return active && pos >= 0 && pos < g.Li;
+    };
+    // Issue the loads of step s (operands and exchange values).  The loads
+    // are unconditional -- lanes with nothing to do at step s read a valid
+    // slot and ignore it -- so the compiler can count outstanding loads
+    // exactly and the prefetch stays in flight across the previous step.
+    const int xj_last = g.Li + TK - 2, xk_last = g.Li + TJ - 2;
+    const double *jsrc = j_in ? jin : ex;
+    const double *ksrc = k_in ? kin : ex;
+    auto fetch = [&](int s, double *op, unsigned long long *xj, unsigned long long *xk) {
+        const int sc = min(s, g.nsteps - 1);
+        const long base = g.slot(tile, row(sc), 0, l);
+#pragma unroll
+        for (int f = 0; f < kSlF; f++) op[f] = sl[base + (long)f * TT];
+        const int pos = s - sj - sk;
+        const int dj = j_in ? min(max(pos + sk, 0), xj_last) : 0;
+        const int dk = k_in ? min(max(pos + sj, 0), xk_last) : 0;
+#pragma unroll
+        for (int x = 0; x < 5; x++) xj[x] = xload_bits(jsrc + (long)dj * TK * 5 + x);
+#pragma unroll
+        for (int x = 0; x < 5; x++) xk[x] = xload_bits(ksrc + (long)dk * TJ * 5 + x);
+    };
+    auto step = [&](int s, const double *op, const unsigned long long *xj, const unsigned long long *xk) {
+        const int p = s & 1;
+        if (on_at(s)) {
+            const int pos = s - sj - sk;
+            double tb[5], tc[5], res[5], pa[5], pb[5], pc[5];
+            if (j_ring) {
+#pragma unroll
+                for (int x = 0; x < 5; x++) tb[x] = 0.0;
+            } else if (j_in) {
+                ex_take(jin + (long)(pos + sk) * TK * 5, xj, tb, err, failed);
+            } else {
+#pragma unroll
+                for (int x = 0; x < 5; x++) tb[x] = pj[p ^ 1][x][lpj];
+            }
+            if (k_ring) {
+#pragma unroll
+                for (int x = 0; x < 5; x++) tc[x] = 0.0;
+            } else if (k_in) {
+                ex_take(kin + (long)(pos + sj) * TJ * 5, xk, tc, err, failed);
+            } else {
+#pragma unroll
+                for (int x = 0; x < 5; x++) tc[x] = pk[p ^ 1][x][lpk];
+            }
+            lu_node<DIR>(op, ta, tb, tc, gamma, res, pa, pb, pc);
+            const long base = g.slot(tile, row(s), kSlOut, l);
+#pragma unroll
+            for (int x = 0; x < 5; x++) {
+                sl[base + (long)x * TT] = res[x];
+                ta[x] = pa[x];
+                pj[p][x][l] = pb[x];
+                pk[p][x][l] = pc[x];
+            }
+            if (j_out) {
+#pragma unroll
+                for (int x = 0; x < 5; x++) xstore(jout + (long)(pos + sk) * TK * 5 + x, pb[x]);
+            }
+            if (k_out) {
+#pragma unroll
+                for (int x = 0; x < 5; x++) xstore(kout + (long)(pos + sj) * TJ * 5 + x, pc[x]);
+            }
+        }
+        lds_barrier();
+    };
+
+    fetch(0, opA, xjA, xkA);
+    for (int s = 0; s < g.nsteps; s += 2) {
+        fetch(s + 1, opB, xjB, xkB);
+        step(s, opA, xjA, xkA);
+        if (s + 1 < g.nsteps) {
+            fetch(s + 2, opA, xjA, xkA);
+            step(s + 1, opB, xjB, xkB);
+        }
+    }
+}
 
 int lusgs_tile_shapes() { return kNumTileShapes; }
 
 long lusgs_tiles(const Ext &e, int shape)
 {
-    int Lj = e.je1 - e.js1 + 1, Lk = e.ke1 - e.ks1 + 1;
-    int TJ = kTileShapes[shape][0], TK = kTileShapes[shape][1];
-    return (long)((Lj + TJ - 1) / TJ) * ((Lk + TK - 1) / TK);
+    SlGeom g = sl_geom(e, shape);
+    return g.Li > 0 && g.Lj > 0 && g.Lk > 0 ? g.ntiles : 0;
 }
 
-// one error word (flags[0]) plus per-tile progress counters (flags[1..]),
-// for any tile shape
-long lusgs_tile_flags_needed(const Ext &e)
+void lusgs_ws_sizes(const Ext &e, long *sl_n, long *ex_n)
 {
-    long mx = 0;
-    for (int s = 0; s < kNumTileShapes; s++) mx = std::max(mx, lusgs_tiles(e, s));
-    return mx + 1;
+    long a = 0, b = 0;
+    for (int s = 0; s < kNumTileShapes; s++) {
+        SlGeom g = sl_geom(e, s);
+        if (g.Li <= 0 || g.Lj <= 0 || g.Lk <= 0) continue;
+        a = std::max(a, (long)g.ntiles * g.nsteps * kSlF * g.TT);
+        b = std::max(b, (long)g.ntiles * (g.exj_tile + g.exk_tile));
+    }
+    *sl_n = a;
+    *ex_n = b;
 }
 
-static int lu_epoch = 0;
+int launch_fill_sentinel(hipStream_t s, double *p, long n)
+{
+    if (n <= 0) return 0;
+    hipLaunchKernelGGL(k_fill_sentinel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n);
+    ACFD_CHECK_LAUNCH();
+    return 0;
+}
 
 template <int DIR, int TJ, int TK>
-static void launch_tiles(hipStream_t s, const LuArgs &a, int *flags, long ntiles, long nflags)
+static void launch_tiles(hipStream_t s, const SlGeom &g, const LuWs &ws, double gamma)
 {
-    const int Li = a.e.ie1 - a.e.is1 + 1;
-    const int nsteps = Li + TJ + TK - 2;
-    // counters only grow: each sweep starts above every value written before
-    (void)nflags;
-    hipLaunchKernelGGL((k_lu_tiles<DIR, TJ, TK>), dim3((unsigned)ntiles), dim3(TJ * TK), 0, s, a, flags + 1, lu_epoch,
-                       flags);
-    lu_epoch += nsteps;
+    hipLaunchKernelGGL((k_lu_tiles<DIR, TJ, TK>), dim3((unsigned)g.ntiles), dim3(TJ * TK), 0, s, g, ws.sl, gamma,
+                       ws.ex, ws.err);
 }
 
-// phase 0: rhs, spectral radii, diag, cleared dq*/dq; 1: forward sweep;
-// 2: backward sweep; 3: q += dq.  variant 0 = hyperplanes, 1 = tiles of the
-// given shape index.
+template <int DIR>
+static void launch_tiles_shape(hipStream_t s, int shape, const SlGeom &g, const LuWs &ws, double gamma)
+{
+    switch (shape) {
+    case 0: launch_tiles<DIR, 16, 16>(s, g, ws, gamma); break;
+    case 1: launch_tiles<DIR, 16, 8>(s, g, ws, gamma); break;
+    case 2: launch_tiles<DIR, 8, 8>(s, g, ws, gamma); break;
+    default: launch_tiles<DIR, 16, 4>(s, g, ws, gamma); break;
+    }
+}
+
+// Phases of one LU-SGS update:
+//   0 rhs, spectral radii, diag (hyperplanes also clear dq* and dq),
+//   1 tiles: gather the operands into the sweep layout,
+//   2 forward sweep, 3 backward sweep,
+//   4 q += dq (tiles: from the sweep layout).
+// variant 0 = hyperplanes, 1 = tiles of the given shape index.
 int launch_lusgs_phase(hipStream_t s, const Ext &e, const Fields &f, const Consts &c, const double *cfl,
-                       int variant, int *tile_flags, int phase, int shape)
+                       int variant, const LuWs *ws, int phase, int shape)
 {
     int Li = e.ie1 - e.is1 + 1, Lj = e.je1 - e.js1 + 1, Lk = e.ke1 - e.ks1 + 1;
+    const bool tiles = variant == 1 && ws && ws->sl && ws->ex;
+    const bool empty = Li <= 0 || Lj <= 0 || Lk <= 0;
     if (phase == 0) {
         Box ring{e.is1 - 1, e.ie1 + 1, e.js1 - 1, e.je1 + 1, e.ks1 - 1, e.ke1 + 1};
         hipLaunchKernelGGL(k_lu_prep, box_grid(ring), dim3(64, 4, 1), 0, s, e, ring, f.pri, f.dxidx, f.dt,
-                           f.rhsi, f.rhsv, f.rhs, f.w_r, f.w_diag, f.w_dqs, f.w_dq, cfl);
-    } else if (phase == 3) {
+                           f.rhsi, f.rhsv, f.rhs, f.w_r, f.w_diag, f.w_dqs, f.w_dq, cfl, tiles ? 0 : 1);
+    } else if (empty) {
+        return 0;
+    } else if (tiles) {
+        const SlGeom g = sl_geom(e, shape);
+        const dim3 lg((unsigned)g.ntiles, (unsigned)((g.nsteps + kChunk - 1) / kChunk));
+        if (phase == 1)
+            hipLaunchKernelGGL(k_sl_gather, lg, dim3(g.TT), 0, s, e, g, f.q, f.dxidx, f.inv_j, f.w_r, f.w_diag,
+                               f.rhs, ws->sl);
+        else if (phase == 2)
+            launch_tiles_shape<1>(s, shape, g, *ws, c.gamma);
+        else if (phase == 3)
+            launch_tiles_shape<-1>(s, shape, g, *ws, c.gamma);
+        else
+            hipLaunchKernelGGL(k_sl_add, lg, dim3(g.TT), 0, s, e, g, ws->sl, f.q);
+    } else if (phase == 4) {
         Box b{e.is1, e.ie1, e.js1, e.je1, e.ks1, e.ke1};
-        if (Li > 0 && Lj > 0 && Lk > 0)
-            hipLaunchKernelGGL(k_add5, box_grid(b), dim3(64, 4, 1), 0, s, e, b, f.q, f.w_dq);
-    } else {
-        if (Li <= 0 || Lj <= 0 || Lk <= 0) return 0;
+        hipLaunchKernelGGL(k_add5, box_grid(b), dim3(64, 4, 1), 0, s, e, b, f.q, f.w_dq);
+    } else if (phase == 2 || phase == 3) {
         LuArgs a{e, f.q, f.dxidx, f.inv_j, f.w_r, f.w_diag, f.rhs, f.w_dqs, f.w_dq, c.gamma};
-        const bool fwd = phase == 1;
-        if (variant == 1 && tile_flags) {
-            long nt = lusgs_tiles(e, shape), nf = lusgs_tile_flags_needed(e);
-            switch (shape) {
-            case 0:
-                if (fwd) launch_tiles<1, 16, 16>(s, a, tile_flags, nt, nf);
-                else launch_tiles<-1, 16, 16>(s, a, tile_flags, nt, nf);
-                break;
-            case 1:
-                if (fwd) launch_tiles<1, 8, 8>(s, a, tile_flags, nt, nf);
-                else launch_tiles<-1, 8, 8>(s, a, tile_flags, nt, nf);
-                break;
-            case 2:
-                if (fwd) launch_tiles<1, 16, 8>(s, a, tile_flags, nt, nf);
-                else launch_tiles<-1, 16, 8>(s, a, tile_flags, nt, nf);
-                break;
-            default:
-                if (fwd) launch_tiles<1, 8, 4>(s, a, tile_flags, nt, nf);
-                else launch_tiles<-1, 8, 4>(s, a, tile_flags, nt, nf);
-                break;
-            }
-        } else {
-            int L = Li + Lj + Lk - 2;
-            for (int l = 0; l < L; l++) {
-                int ja = max(0, l - (Li - 1) - (Lk - 1)), jb = min(Lj - 1, l);
-                int ka = max(0, l - (Li - 1) - (Lj - 1)), kb = min(Lk - 1, l);
-                dim3 g((jb - ja + 64) / 64, (kb - ka + 4) / 4, 1);
-                if (fwd) hipLaunchKernelGGL((k_lu_plane<1>), g, dim3(64, 4, 1), 0, s, a, l, ja, ka);
-                else hipLaunchKernelGGL((k_lu_plane<-1>), g, dim3(64, 4, 1), 0, s, a, l, ja, ka);
-            }
+        const bool fwd = phase == 2;
+        int L = Li + Lj + Lk - 2;
+        for (int l = 0; l < L; l++) {
+            int ja = max(0, l - (Li - 1) - (Lk - 1)), jb = min(Lj - 1, l);
+            int ka = max(0, l - (Li - 1) - (Lj - 1)), kb = min(Lk - 1, l);
+            dim3 gr((jb - ja + 64) / 64, (kb - ka + 4) / 4, 1);
+            if (fwd) hipLaunchKernelGGL((k_lu_plane<1>), gr, dim3(64, 4, 1), 0, s, a, l, ja, ka);
+            else hipLaunchKernelGGL((k_lu_plane<-1>), gr, dim3(64, 4, 1), 0, s, a, l, ja, ka);
         }
     }
     ACFD_CHECK_LAUNCH();
@@ -371,10 +602,10 @@ int launch_lusgs_phase(hipStream_t s, const Ext &e, const Fields &f, const Const
 }
 
 int launch_lusgs(hipStream_t s, const Ext &e, const Fields &f, const Consts &c, const double *cfl,
-                 int variant, int *tile_flags)
+                 int variant, const LuWs *ws, int shape)
 {
-    for (int ph = 0; ph < 4; ph++) {
-        int rc = launch_lusgs_phase(s, e, f, c, cfl, variant, tile_flags, ph, 0);
+    for (int ph = 0; ph < kLusgsPhases; ph++) {
+        int rc = launch_lusgs_phase(s, e, f, c, cfl, variant, ws, ph, shape);
         if (rc) return rc;
     }
     return 0;

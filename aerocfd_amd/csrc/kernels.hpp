@@ -15,13 +15,23 @@ int launch_find_dt(hipStream_t s, const Ext &e, const Fields &f, const Consts &c
 int launch_reset_stats(hipStream_t s, unsigned long long *stats);
 
 // k_lusgs.hip
+// workspace of the tile sweeps: sweep-layout operands (normal memory),
+// exchange slots (uncached memory, sentinel-filled) and an error word
+struct LuWs {
+    double *sl;
+    double *ex;
+    int *err;
+};
+constexpr int kLusgsPhases = 5;
 int launch_lusgs(hipStream_t s, const Ext &e, const Fields &f, const Consts &c, const double *cfl,
-                 int variant, int *tile_flags);
+                 int variant, const LuWs *ws, int shape);
 int launch_lusgs_phase(hipStream_t s, const Ext &e, const Fields &f, const Consts &c, const double *cfl,
-                       int variant, int *tile_flags, int phase, int shape);
-long lusgs_tile_flags_needed(const Ext &e);
+                       int variant, const LuWs *ws, int phase, int shape);
 int lusgs_tile_shapes();
 long lusgs_tiles(const Ext &e, int shape);
+// doubles of the sweep layout and of the exchange slots (max over shapes)
+void lusgs_ws_sizes(const Ext &e, long *sl_n, long *ex_n);
+int launch_fill_sentinel(hipStream_t s, double *p, long n);
 
 // k_bdf2.hip
 int launch_crank(hipStream_t s, const Ext &e, const Fields &f, const Consts &c);

@@ -37,7 +37,9 @@ STAGES = {
 
 # sub-stage timers inside implicit1st: the two Gauss-Seidel sweeps and the
 # rest (spectral radii, diagonal, update)
-TIMED_KERNELS = {"lusgs_forward": 12, "lusgs_backward": 13, "lusgs_other": 15}
+# sub-stage timers of the LU-SGS update: the two sweeps, the sweep-layout
+# gather and q += dq (tile sweeps), and rhs / spectral radii / diag
+TIMED_KERNELS = {"lusgs_forward": 12, "lusgs_backward": 13, "lusgs_layout": 14, "lusgs_other": 15}
 
 GAMMA = 1.4
 

@@ -33,9 +33,16 @@ BYTES_PER_NODE = {
     "getpri": 40 + 56,                              # q -> pri_v
     "inviscid": 40 + 72 + 8 + 40,                   # q, dxidx, inv_j -> rhsi
     "viscous": 32 + 8 + 72 + 8 + 16 + 40,           # u,v,w,T, mu_t, dxidx, inv_j -> mu, rhsv
-    "lusgs_forward": 40 + 72 + 8 + 24 + 8 + 40 + 40,  # q, dxidx, inv_j, r_abc, diag, rhs -> dq*
-    "lusgs_backward": 40 + 72 + 8 + 24 + 8 + 40 + 40,  # q, dxidx, inv_j, r_abc, diag, dq* -> dq
-    "lusgs_other": 32 + 72 + 8 + 80 + 40 + 32 + 80 + 80,  # spectral radii/diag/rhs + q += dq
+    # tile sweeps over the sweep layout: 24 operands in, 5 results out
+    "lusgs_forward": 24 * 8 + 40,                   # q, dxidx, inv_j, r_abc, diag, rhs -> dq*
+    "lusgs_backward": 24 * 8 + 40,                  # q, dxidx, inv_j, r_abc, diag, dq* -> dq
+    "lusgs_layout": 2 * 24 * 8 + 40 + 40 + 40,      # gather 24 in + 24 out; q += dq
+    "lusgs_other": 32 + 72 + 8 + 80 + 40 + 32,      # u,v,w,c, dxidx, dt, rhsi+rhsv -> rhs, r_abc, diag
+}
+HYPERPLANE_BYTES = {  # the hyperplane sweeps read the operands in place
+    "lusgs_forward": 40 + 72 + 8 + 24 + 8 + 40 + 40,
+    "lusgs_backward": 40 + 72 + 8 + 24 + 8 + 40 + 40,
+    "lusgs_other": 32 + 72 + 8 + 80 + 40 + 32 + 80 + 80,
 }
 
 
@@ -150,10 +157,13 @@ def main():
     value = nodes * args.steps / secs / 1e6
     # dominant kernel among the timed ones (per launch averages)
     comp = (args.n) ** 3  # computed nodes (all faces are interfaces)
-    cand = {k: v for k, v in tm.items() if k in BYTES_PER_NODE and v[1] > 0}
+    table = dict(BYTES_PER_NODE)
+    if args.lusgs == 0:
+        table.update(HYPERPLANE_BYTES)
+    cand = {k: v for k, v in tm.items() if k in table and v[1] > 0}
     dom = max(cand, key=lambda k: cand[k][0])
     avg_ms = cand[dom][0] / cand[dom][1]
-    bytes_launch = BYTES_PER_NODE[dom] * comp
+    bytes_launch = table[dom] * comp
     achieved = bytes_launch / (avg_ms * 1e-3) / 1e9
     out = {
         "metric": "Mcell-updates/s",
@@ -173,7 +183,7 @@ def main():
                    "lusgs_kernel": "pipelined-tiles" if args.lusgs == 1 else "hyperplanes"},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": PEAK_HBM_GBS,
                      "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None,
-                     "bytes_per_node": BYTES_PER_NODE[dom], "avg_launch_ms": round(avg_ms, 4)},
+                     "bytes_per_node": table[dom], "avg_launch_ms": round(avg_ms, 4)},
         "stage_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in tm.items() if v[1] > 0},
         "residual": [float("%.6e" % x) for x in res[:5]],
     }
