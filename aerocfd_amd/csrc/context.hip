@@ -45,6 +45,8 @@ int fail(int code, const std::string &msg)
 
 const int kNcomp[22] = {0, 5, 7, 5, 5, 5, 9, 6, 1, 3, 1, 9, 1, 3, 3, 3, 1, 1, 1, 1, 10, 1};
 
+constexpr int kTimeSlots = 20;  // acfd_timing entries
+
 struct Block {
     Ext e;
     Fields f;
@@ -120,8 +122,8 @@ struct acfd_ctx {
     // timing: event pairs recorded on the stream without waiting; read
     // back (and the pool recycled) when full or when the totals are asked for
     bool timing = false;
-    double stage_ms[16] = {0};
-    long stage_calls[16] = {0};
+    double stage_ms[kTimeSlots] = {0};
+    long stage_calls[kTimeSlots] = {0};
     std::vector<hipEvent_t> ev_pool;
     std::vector<int> ev_stage;   // stage of each recorded pair
     size_t ev_used = 0;          // pairs in use
@@ -450,7 +452,8 @@ int st_find_dt(acfd_ctx *ctx, int m)
 }
 
 // sub-stage timing ids of the LU-SGS phases
-constexpr int kTimeLuForward = 12, kTimeLuBackward = 13, kTimeLuLayout = 14, kTimeLuOther = 15;
+constexpr int kTimeLuForward = 12, kTimeLuBackward = 13, kTimeLuGather = 14, kTimeLuOther = 15,
+              kTimeLuAdd = 16;
 
 // Autotuner of the tile sweep: on a block's first update, time the gather
 // and forward sweep once per tile shape (they recompute dq* in the sweep
@@ -504,7 +507,7 @@ int st_implicit(acfd_ctx *ctx, int m)
             if (rc) return rc;
         }
         StageTimer ts(ctx, ph == 2 ? kTimeLuForward : ph == 3 ? kTimeLuBackward
-                           : (ph == 1 || ph == 4) ? kTimeLuLayout : kTimeLuOther);
+                           : ph == 1 ? kTimeLuGather : ph == 4 ? kTimeLuAdd : kTimeLuOther);
         KRN_TRY(launch_lusgs_phase(ctx->stream, b.e, b.f, ctx->c, ctx->d_cfl, variant, &ctx->lu, ph,
                                    b.tile_shape < 0 ? 0 : b.tile_shape));
     }
@@ -1018,12 +1021,12 @@ int acfd_timing(acfd_ctx *ctx, int enable, double *ms, long *calls)
     if (!ctx) return fail(ACFD_EINVAL, "acfd_timing: null context");
     timing_flush(ctx);
     if (ms)
-        for (int s = 0; s < 16; s++) ms[s] = ctx->stage_ms[s];
+        for (int s = 0; s < kTimeSlots; s++) ms[s] = ctx->stage_ms[s];
     if (calls)
-        for (int s = 0; s < 16; s++) calls[s] = ctx->stage_calls[s];
+        for (int s = 0; s < kTimeSlots; s++) calls[s] = ctx->stage_calls[s];
     if (enable >= 0) {
         ctx->timing = enable != 0;
-        for (int s = 0; s < 16; s++) {
+        for (int s = 0; s < kTimeSlots; s++) {
             ctx->stage_ms[s] = 0;
             ctx->stage_calls[s] = 0;
         }

@@ -19,8 +19,8 @@
 //                * Sweep layout.  Before the forward sweep the operands of
 //                  every node (q, metric row, 1/J, spectral radii, diag, rhs:
 //                  24 values) are gathered into an array ordered by (tile,
-//                  step, field, lane), so at every step a tile reads 24
-//                  contiguous rows of TJ*TK values -- fully coalesced, where
+//                  step, field pair, lane), so at every step a tile reads 12
+//                  contiguous rows of TJ*TK 16-byte pairs -- fully coalesced, where
 //                  the natural layout would have every lane on its own cache
 //                  line.  The sweeps write dq* and dq over the rhs slots and a
 //                  final pass adds dq to q.
@@ -37,8 +37,9 @@
 //                  produces) until the producer stores the value; the
 //                  consumer spins on the value itself and writes the sentinel
 //                  back.  No counters, no fences, no waits for store
-//                  acknowledgements on the per-step path; values are loaded
-//                  one step ahead so that in steady state the wait is hidden.
+//                  acknowledgements on the per-step path; operands and
+//                  exchange values are loaded two steps ahead so that in
+//                  steady state memory latency is hidden.
 //
 // The flux Jacobian products use the closed form of A(q) v for the Euler
 // flux in generalised coordinates (one reciprocal of rho instead of the ~25
@@ -234,15 +235,15 @@ constexpr int kSlOut = 19;
 // exchange sentinel: a negative signalling NaN with a payload; arithmetic
 // only ever produces quiet NaNs
 constexpr unsigned long long kSentinel = 0xFFF5A5A5A5A5A5A5ull;
-// consecutive steps per workgroup of the layout passes
-constexpr int kChunk = 16;
 
 // geometry of one tiling of a block's computed range
 struct SlGeom {
     int TJ, TK, TT, nbj, nbk, ntiles, nsteps, Li, Lj, Lk;
     long exj_tile, exk_tile;  // doubles per tile of the j / k exchange regions
+    // fields are stored in pairs, so a lane's operands of one step are 12
+    // 16-byte loads; double index of field f of lane l at row t
     __host__ __device__ long slot(int tile, int t, int f, int l) const {
-        return (((long)tile * nsteps + t) * kSlF + f) * TT + l;
+        return ((((long)tile * nsteps + t) * (kSlF / 2) + (f >> 1)) * TT + l) * 2 + (f & 1);
     }
 };
 
@@ -317,9 +318,40 @@ __device__ inline void lds_barrier()
     __asm__ volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
 }
 
-// Gather the 24 operands of every node into the sweep layout.  Workgroup =
-// (tile, chunk of kChunk steps); lane = (tj, tk).  One field at a time so a
-// lane walks its own line along i and reuses each cache line.
+// The layout passes move data between the natural layout (i fastest) and
+// the sweep layout through LDS.  A workgroup handles one tile and a run of
+// kChunksPerWg chunks of kCT steps, in sequence (the cache lines a chunk
+// straddles are reused by the next one).  In a chunk starting at step t0,
+// lane l's line contributes the kCT nodes li = t0 + ii - tj - tk,
+// ii = 0..kCT-1, which are exactly rows t0 + ii of the sweep layout.
+// Natural side: thread x handles ii = x % kCT of lines x / kCT + m TT/kCT,
+// so 32 consecutive lanes touch 32 consecutive doubles of one line.
+// Sweep side: thread x is lane x and touches row slices of TT doubles.
+// LDS holds [kCT][TT] doubles (64 KiB at TT = 256) with the line index
+// XOR-swizzled by ii, so both access patterns are free of bank conflicts.
+constexpr int kCT = 32;
+constexpr int kChunksPerWg = 3;
+
+struct LaneLine {
+    int tj, tk;
+    bool ok;
+};
+__device__ inline LaneLine lane_line(const SlGeom &g, int tile, int line)
+{
+    LaneLine r;
+    r.tj = line % g.TJ;
+    r.tk = line / g.TJ;
+    r.ok = (tile % g.nbj) * g.TJ + r.tj < g.Lj && (tile / g.nbj) * g.TK + r.tk < g.Lk;
+    return r;
+}
+
+// natural index of (tile, line, node li) -- li already range-checked
+__device__ inline long nat_index(const Ext &e, const SlGeom &g, int tile, const LaneLine &ln, int li)
+{
+    return e.ix(e.is1 + li, e.js1 + (tile % g.nbj) * g.TJ + ln.tj, e.ks1 + (tile / g.nbj) * g.TK + ln.tk);
+}
+
+// Gather the 24 operands of every node into the sweep layout.
 __global__ void __launch_bounds__(256) k_sl_gather(Ext e, SlGeom g, const double *__restrict__ q,
                                                    const double *__restrict__ dxidx,
                                                    const double *__restrict__ inv_j,
@@ -327,20 +359,33 @@ __global__ void __launch_bounds__(256) k_sl_gather(Ext e, SlGeom g, const double
                                                    const double *__restrict__ diag,
                                                    const double *__restrict__ rhs, double *__restrict__ sl)
 {
-    const int tile = blockIdx.x, t0 = blockIdx.y * kChunk;
-    const int l = threadIdx.x;
-    if (l >= g.TT) return;
-    const int tj = l % g.TJ, tk = l / g.TJ;
-    const int lj = (tile % g.nbj) * g.TJ + tj, lk = (tile / g.nbj) * g.TK + tk;
-    if (lj >= g.Lj || lk >= g.Lk) return;
+    extern __shared__ double buf[];  // [kCT][TT], swizzled
+    const int tile = blockIdx.x, TT = g.TT, x = threadIdx.x;
+    const LaneLine me = lane_line(g, tile, x);
+    const int ii_n = x % kCT, lstep = TT / kCT;
     const long N = e.n;
-    const long n0 = e.ix(e.is1, e.js1 + lj, e.ks1 + lk);
-    for (int f = 0; f < kSlF; f++) {
-        const double *src = f < 5 ? q + f * N : f < 14 ? dxidx + (f - 5) * N : f == 14 ? inv_j
-                          : f < 18 ? rr + (f - 15) * N : f == 18 ? diag : rhs + (f - 19) * N;
-        for (int t = t0; t < t0 + kChunk && t < g.nsteps; t++) {
-            const int li = t - tj - tk;
-            if (li >= 0 && li < g.Li) sl[g.slot(tile, t, f, l)] = src[n0 + li];
+    const int nch = (g.nsteps + kCT - 1) / kCT;
+    const int c1 = min(nch, ((int)blockIdx.y + 1) * kChunksPerWg);
+    for (int c = blockIdx.y * kChunksPerWg; c < c1; c++) {
+        const int t0 = c * kCT;
+        for (int f = 0; f < kSlF; f++) {
+            const double *src = f < 5 ? q + f * N : f < 14 ? dxidx + (f - 5) * N : f == 14 ? inv_j
+                              : f < 18 ? rr + (f - 15) * N : f == 18 ? diag : rhs + (f - 19) * N;
+            for (int m = 0; m < kCT; m++) {
+                const int line = m * lstep + x / kCT;
+                const LaneLine ln = lane_line(g, tile, line);
+                const int li = t0 + ii_n - ln.tj - ln.tk;
+                double v = 0.0;
+                if (ln.ok && li >= 0 && li < g.Li) v = src[nat_index(e, g, tile, ln, li)];
+                buf[ii_n * TT + (line ^ ii_n)] = v;
+            }
+            __syncthreads();
+            if (me.ok)
+                for (int ii = 0; ii < kCT; ii++) {
+                    const int t = t0 + ii, li = t - me.tj - me.tk;
+                    if (t < g.nsteps && li >= 0 && li < g.Li) sl[g.slot(tile, t, f, x)] = buf[ii * TT + (x ^ ii)];
+                }
+            __syncthreads();
         }
     }
 }
@@ -349,22 +394,34 @@ __global__ void __launch_bounds__(256) k_sl_gather(Ext e, SlGeom g, const double
 __global__ void __launch_bounds__(256) k_sl_add(Ext e, SlGeom g, const double *__restrict__ sl,
                                                 double *__restrict__ q)
 {
-    const int tile = blockIdx.x, t0 = blockIdx.y * kChunk;
-    const int l = threadIdx.x;
-    if (l >= g.TT) return;
-    const int tj = l % g.TJ, tk = l / g.TJ;
-    const int lj = (tile % g.nbj) * g.TJ + tj, lk = (tile / g.nbj) * g.TK + tk;
-    if (lj >= g.Lj || lk >= g.Lk) return;
+    extern __shared__ double buf[];  // [kCT][TT], swizzled
+    const int tile = blockIdx.x, TT = g.TT, x = threadIdx.x;
+    const LaneLine me = lane_line(g, tile, x);
+    const int ii_n = x % kCT, lstep = TT / kCT;
     const long N = e.n;
-    const long n0 = e.ix(e.is1, e.js1 + lj, e.ks1 + lk);
-    for (int x = 0; x < 5; x++)
-        for (int t = t0; t < t0 + kChunk && t < g.nsteps; t++) {
-            const int li = t - tj - tk;
-            if (li >= 0 && li < g.Li) {
-                const long n = x * N + n0 + li;
-                q[n] = q[n] + sl[g.slot(tile, t, kSlOut + x, l)];
+    const int nch = (g.nsteps + kCT - 1) / kCT;
+    const int c1 = min(nch, ((int)blockIdx.y + 1) * kChunksPerWg);
+    for (int c = blockIdx.y * kChunksPerWg; c < c1; c++) {
+        const int t0 = c * kCT;
+        for (int v = 0; v < 5; v++) {
+            if (me.ok)
+                for (int ii = 0; ii < kCT; ii++) {
+                    const int t = t0 + ii, li = t - me.tj - me.tk;
+                    if (t < g.nsteps && li >= 0 && li < g.Li) buf[ii * TT + (x ^ ii)] = sl[g.slot(tile, t, kSlOut + v, x)];
+                }
+            __syncthreads();
+            for (int m = 0; m < kCT; m++) {
+                const int line = m * lstep + x / kCT;
+                const LaneLine ln = lane_line(g, tile, line);
+                const int li = t0 + ii_n - ln.tj - ln.tk;
+                if (ln.ok && li >= 0 && li < g.Li && t0 + ii_n < g.nsteps) {
+                    const long n = v * N + nat_index(e, g, tile, ln, li);
+                    q[n] = q[n] + buf[ii_n * TT + (line ^ ii_n)];
+                }
             }
+            __syncthreads();
         }
+    }
 }
 
 // Fill exchange slots with the sentinel.
@@ -422,8 +479,8 @@ __global__ void __launch_bounds__(TJ *TK) k_lu_tiles(SlGeom g, double *__restric
     double *kout = exk + (long)tile * g.exk_tile + (long)tj * 5;
     bool failed = false;
     double ta[5] = {0, 0, 0, 0, 0};
-    double opA[kSlF], opB[kSlF];
-    unsigned long long xjA[5], xkA[5], xjB[5], xkB[5];
+    double opA[kSlF], opB[kSlF], opC[kSlF];
+    unsigned long long xjA[5], xkA[5], xjB[5], xkB[5], xjC[5], xkC[5];
 
     auto row = [&](int s) { return DIR > 0 ? s : g.nsteps - 1 - s; };
     auto on_at = [&](int s) {
@@ -439,9 +496,13 @@ __global__ void __launch_bounds__(TJ *TK) k_lu_tiles(SlGeom g, double *__restric
     const double *ksrc = k_in ? kin : ex;
     auto fetch = [&](int s, double *op, unsigned long long *xj, unsigned long long *xk) {
         const int sc = min(s, g.nsteps - 1);
-        const long base = g.slot(tile, row(sc), 0, l);
+        const double2 *src = reinterpret_cast<const double2 *>(sl + g.slot(tile, row(sc), 0, l));
 #pragma unroll
-        for (int f = 0; f < kSlF; f++) op[f] = sl[base + (long)f * TT];
+        for (int f = 0; f < kSlF / 2; f++) {
+            const double2 v = src[(long)f * TT];
+            op[2 * f] = v.x;
+            op[2 * f + 1] = v.y;
+        }
         const int pos = s - sj - sk;
         const int dj = j_in ? min(max(pos + sk, 0), xj_last) : 0;
         const int dk = k_in ? min(max(pos + sj, 0), xk_last) : 0;
@@ -474,10 +535,9 @@ __global__ void __launch_bounds__(TJ *TK) k_lu_tiles(SlGeom g, double *__restric
                 for (int x = 0; x < 5; x++) tc[x] = pk[p ^ 1][x][lpk];
             }
             lu_node<DIR>(op, ta, tb, tc, gamma, res, pa, pb, pc);
-            const long base = g.slot(tile, row(s), kSlOut, l);
 #pragma unroll
             for (int x = 0; x < 5; x++) {
-                sl[base + (long)x * TT] = res[x];
+                sl[g.slot(tile, row(s), kSlOut + x, l)] = res[x];
                 ta[x] = pa[x];
                 pj[p][x][l] = pb[x];
                 pk[p][x][l] = pc[x];
@@ -494,14 +554,17 @@ __global__ void __launch_bounds__(TJ *TK) k_lu_tiles(SlGeom g, double *__restric
         lds_barrier();
     };
 
+    // operands are loaded two steps ahead (three register sets in rotation);
+    // steps past the end are no-ops that keep the barriers uniform
     fetch(0, opA, xjA, xkA);
-    for (int s = 0; s < g.nsteps; s += 2) {
-        fetch(s + 1, opB, xjB, xkB);
+    fetch(1, opB, xjB, xkB);
+    for (int s = 0; s < g.nsteps; s += 3) {
+        fetch(s + 2, opC, xjC, xkC);
         step(s, opA, xjA, xkA);
-        if (s + 1 < g.nsteps) {
-            fetch(s + 2, opA, xjA, xkA);
-            step(s + 1, opB, xjB, xkB);
-        }
+        fetch(s + 3, opA, xjA, xkA);
+        step(s + 1, opB, xjB, xkB);
+        fetch(s + 4, opB, xjB, xkB);
+        step(s + 2, opC, xjC, xkC);
     }
 }
 
@@ -572,16 +635,18 @@ int launch_lusgs_phase(hipStream_t s, const Ext &e, const Fields &f, const Const
         return 0;
     } else if (tiles) {
         const SlGeom g = sl_geom(e, shape);
-        const dim3 lg((unsigned)g.ntiles, (unsigned)((g.nsteps + kChunk - 1) / kChunk));
+        const int nch = (g.nsteps + kCT - 1) / kCT;
+        const dim3 lg((unsigned)g.ntiles, (unsigned)((nch + kChunksPerWg - 1) / kChunksPerWg));
+        const size_t lds = (size_t)kCT * g.TT * sizeof(double);
         if (phase == 1)
-            hipLaunchKernelGGL(k_sl_gather, lg, dim3(g.TT), 0, s, e, g, f.q, f.dxidx, f.inv_j, f.w_r, f.w_diag,
+            hipLaunchKernelGGL(k_sl_gather, lg, dim3(g.TT), lds, s, e, g, f.q, f.dxidx, f.inv_j, f.w_r, f.w_diag,
                                f.rhs, ws->sl);
         else if (phase == 2)
             launch_tiles_shape<1>(s, shape, g, *ws, c.gamma);
         else if (phase == 3)
             launch_tiles_shape<-1>(s, shape, g, *ws, c.gamma);
         else
-            hipLaunchKernelGGL(k_sl_add, lg, dim3(g.TT), 0, s, e, g, ws->sl, f.q);
+            hipLaunchKernelGGL(k_sl_add, lg, dim3(g.TT), lds, s, e, g, ws->sl, f.q);
     } else if (phase == 4) {
         Box b{e.is1, e.ie1, e.js1, e.je1, e.ks1, e.ke1};
         hipLaunchKernelGGL(k_add5, box_grid(b), dim3(64, 4, 1), 0, s, e, b, f.q, f.w_dq);

@@ -37,9 +37,11 @@ STAGES = {
 
 # sub-stage timers inside implicit1st: the two Gauss-Seidel sweeps and the
 # rest (spectral radii, diagonal, update)
-# sub-stage timers of the LU-SGS update: the two sweeps, the sweep-layout
-# gather and q += dq (tile sweeps), and rhs / spectral radii / diag
-TIMED_KERNELS = {"lusgs_forward": 12, "lusgs_backward": 13, "lusgs_layout": 14, "lusgs_other": 15}
+# sub-stage timers of the LU-SGS update: the two sweeps, the gather into the
+# sweep layout (tile sweeps), rhs / spectral radii / diag, and q += dq
+TIMED_KERNELS = {"lusgs_forward": 12, "lusgs_backward": 13, "lusgs_gather": 14, "lusgs_other": 15,
+                 "lusgs_add": 16}
+TIME_SLOTS = 20
 
 GAMMA = 1.4
 
@@ -185,8 +187,8 @@ class Solver:
     def timing(self, enable=None):
         """Per-stage HIP-event totals {stage: (ms, calls)}; enable=True/False
         restarts recording, None only reads."""
-        ms = np.zeros(16)
-        calls = np.zeros(16, dtype=np.int64)
+        ms = np.zeros(TIME_SLOTS)
+        calls = np.zeros(TIME_SLOTS, dtype=np.int64)
         flag = -1 if enable is None else int(bool(enable))
         _lib.check(self.lib.acfd_timing(self.h, C.c_int(flag), _dp(ms),
                                         calls.ctypes.data_as(C.POINTER(C.c_long))))

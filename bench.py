@@ -36,13 +36,15 @@ BYTES_PER_NODE = {
     # tile sweeps over the sweep layout: 24 operands in, 5 results out
     "lusgs_forward": 24 * 8 + 40,                   # q, dxidx, inv_j, r_abc, diag, rhs -> dq*
     "lusgs_backward": 24 * 8 + 40,                  # q, dxidx, inv_j, r_abc, diag, dq* -> dq
-    "lusgs_layout": 2 * 24 * 8 + 40 + 40 + 40,      # gather 24 in + 24 out; q += dq
+    "lusgs_gather": 2 * 24 * 8,                     # 24 operands in (natural) and out (sweep layout)
+    "lusgs_add": 40 + 40 + 40,                      # q, dq -> q
     "lusgs_other": 32 + 72 + 8 + 80 + 40 + 32,      # u,v,w,c, dxidx, dt, rhsi+rhsv -> rhs, r_abc, diag
 }
 HYPERPLANE_BYTES = {  # the hyperplane sweeps read the operands in place
     "lusgs_forward": 40 + 72 + 8 + 24 + 8 + 40 + 40,
     "lusgs_backward": 40 + 72 + 8 + 24 + 8 + 40 + 40,
     "lusgs_other": 32 + 72 + 8 + 80 + 40 + 32 + 80 + 80,
+    "lusgs_add": 40 + 40 + 40,
 }
 
 

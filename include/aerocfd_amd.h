@@ -122,9 +122,15 @@ int acfd_residual(acfd_ctx *ctx, double *out);
 int acfd_stats(acfd_ctx *ctx, double *out);
 /* Wait for all work of the context. */
 int acfd_sync(acfd_ctx *ctx);
-/* Per-stage GPU time accumulated with HIP events on the context's stream
- * (enable != 0 turns recording on and clears the totals).  ms[stage] and
- * calls[stage] for stage 1..11; index 0 unused. */
+/* Per-stage GPU time accumulated with HIP events on the context's stream.
+ * Returns the totals recorded so far (ms/calls may be NULL); then, for
+ * enable >= 0, clears them and turns recording on (enable > 0) or off
+ * (enable == 0); enable < 0 only reads.  ms and calls have 20 entries: [stage] for
+ * stages 1..11, and sub-stages of ACFD_STAGE_IMPLICIT1ST: [12] forward
+ * sweep, [13] backward sweep, [14] gather into the sweep layout (tile
+ * sweeps), [15] rhs / spectral radii / diagonal, [16] q += dq.  Indices 0
+ * and 17..19 are unused.  The
+ * reference's only instrument is a per-step wall time (main.f90:575-586). */
 int acfd_timing(acfd_ctx *ctx, int enable, double *ms, long *calls);
 
 /* ---------------------------------------------------- stateless routines */
