@@ -65,34 +65,28 @@ int launch_fill(hipStream_t s, double *p, long n, double v)
 }
 
 // ------------------------------------------------------------ WENO3 + Roe
-// Candidate pairs of sub_inviscid_upwind.f90:32-41 (right state from
-// u1,u2,u3 = f(a),f(a+1),f(a+2)) and :57-66 (left state from f(a-1),f(a),f(a+1)).
+// The formulas are the reference's, rearranged to need fewer divisions
+// (FP64 division is a ~10-instruction sequence on CDNA): the WENO3 weights
+// a1/(a1+a2), a2/(a1+a2) with a1 = 1/(3 t1^2), a2 = 2/(3 t2^2) are
+// t2^2/(t2^2 + 2 t1^2) and 2 t1^2/(t2^2 + 2 t1^2) (one division instead of
+// four), and the Roe flux divides once per distinct denominator.  Results
+// agree with the reference's operation order to rounding.
+//
+// WENO3 state at a face from three values; sub_inviscid_upwind.f90:32-41
+// (right state: u1,u2,u3 = f(a),f(a+1),f(a+2)) and :57-66 (left state:
+// f(a-1),f(a),f(a+1)).  `near` is the value next to the face on the side of
+// the state, `far` the one beyond it, `other` the value across the face.
 template <typename R>
-__device__ inline R weno3_right(R u1, R u2, R u3)
+__device__ inline R weno3(R far, R near, R other)
 {
     const R eps = (R)1.0e-6;
-    R s1 = (u2 - u3) * (u2 - u3);
-    R s2 = (u1 - u2) * (u1 - u2);
-    R t1 = eps + s1, t2 = eps + s2;
-    R a1 = (R)1.0 / ((R)3.0 * (t1 * t1));
-    R a2 = (R)2.0 / ((R)3.0 * (t2 * t2));
-    R w1 = a1 / (a1 + a2);
-    R w2 = a2 / (a1 + a2);
-    return w1 * (-u3 / (R)2.0 + (R)3.0 * u2 / (R)2.0) + w2 * (u2 / (R)2.0 + u1 / (R)2.0);
-}
-
-template <typename R>
-__device__ inline R weno3_left(R u1, R u2, R u3)
-{
-    const R eps = (R)1.0e-6;
-    R s1 = (u2 - u1) * (u2 - u1);
-    R s2 = (u3 - u2) * (u3 - u2);
-    R t1 = eps + s1, t2 = eps + s2;
-    R a1 = (R)1.0 / ((R)3.0 * (t1 * t1));
-    R a2 = (R)2.0 / ((R)3.0 * (t2 * t2));
-    R w1 = a1 / (a1 + a2);
-    R w2 = a2 / (a1 + a2);
-    return w1 * (-u1 / (R)2.0 + (R)3.0 * u2 / (R)2.0) + w2 * (u2 / (R)2.0 + u3 / (R)2.0);
+    const R s1 = (near - far) * (near - far);
+    const R s2 = (other - near) * (other - near);
+    const R t1 = eps + s1, t2 = eps + s2;
+    const R T1 = t1 * t1, T2 = t2 * t2;
+    const R inv = (R)1.0 / (T2 + (R)2.0 * T1);
+    const R w1 = T2 * inv, w2 = (R)2.0 * T1 * inv;
+    return w1 * ((R)1.5 * near - (R)0.5 * far) + w2 * ((R)0.5 * (near + other));
 }
 
 // Roe flux with Harten's entropy fix (delta = 0.06) on a face with metric
@@ -101,47 +95,52 @@ __device__ inline R weno3_left(R u1, R u2, R u3)
 template <typename R>
 __device__ inline void roe_flux(const R *ql, const R *qr, R mx, R my, R mz, R gamma, R *out)
 {
-    const R delt = (R)0.06;
-    R temp = sqrt(mx * mx + my * my + mz * mz);
-    R nx = mx / temp, ny = my / temp, nz = mz / temp;
-    R dl = ql[0], ul = ql[1], vl = ql[2], wl = ql[3], pl = ql[4];
-    R dr = qr[0], ur = qr[1], vr = qr[2], wr = qr[3], pr = qr[4];
-    R hl = gamma * pl / ((gamma - (R)1.0) * dl) + (R)0.5 * (ul * ul + vl * vl + wl * wl);
-    R hr = gamma * pr / ((gamma - (R)1.0) * dr) + (R)0.5 * (ur * ur + vr * vr + wr * wr);
-    R uul = ul * mx + vl * my + wl * mz;
-    R uur = ur * mx + vr * my + wr * mz;
-    R ubarl = uul / temp, ubarr = uur / temp;
-    R tmpp = sqrt(dr / dl);
-    R avd = sqrt(dl * dr);
-    R avu = (ul + ur * tmpp) / ((R)1.0 + tmpp);
-    R avv = (vl + vr * tmpp) / ((R)1.0 + tmpp);
-    R avw = (wl + wr * tmpp) / ((R)1.0 + tmpp);
-    R avh = (hl + hr * tmpp) / ((R)1.0 + tmpp);
-    R ava = sqrt((gamma - (R)1.0) * (avh - (R)0.5 * (avu * avu + avv * avv + avw * avw)));
-    R avuu = avu * nx + avv * ny + avw * nz;
+    const R delt = (R)0.06, rdelt2 = (R)1.0 / ((R)2.0 * (R)0.06);
+    const R g1 = gamma - (R)1.0, rg1 = (R)1.0 / g1, ggg = gamma * rg1;
+    const R temp = sqrt(mx * mx + my * my + mz * mz);
+    const R rt = (R)1.0 / temp;
+    const R nx = mx * rt, ny = my * rt, nz = mz * rt;
+    const R dl = ql[0], ul = ql[1], vl = ql[2], wl = ql[3], pl = ql[4];
+    const R dr = qr[0], ur = qr[1], vr = qr[2], wr = qr[3], pr = qr[4];
+    const R rdl = (R)1.0 / dl;
+    const R hl = ggg * pl * rdl + (R)0.5 * (ul * ul + vl * vl + wl * wl);
+    const R hr = ggg * pr / dr + (R)0.5 * (ur * ur + vr * vr + wr * wr);
+    const R uul = ul * mx + vl * my + wl * mz;
+    const R uur = ur * mx + vr * my + wr * mz;
+    const R ubarl = uul * rt, ubarr = uur * rt;
+    const R tmpp = sqrt(dr * rdl);
+    const R avd = sqrt(dl * dr);
+    const R r1t = (R)1.0 / ((R)1.0 + tmpp);
+    const R avu = (ul + ur * tmpp) * r1t;
+    const R avv = (vl + vr * tmpp) * r1t;
+    const R avw = (wl + wr * tmpp) * r1t;
+    const R avh = (hl + hr * tmpp) * r1t;
+    const R ava = sqrt(g1 * (avh - (R)0.5 * (avu * avu + avv * avv + avw * avw)));
+    const R avuu = avu * nx + avv * ny + avw * nz;
     R l1 = fabs(avuu - ava), l2 = fabs(avuu), l5 = fabs(avuu + ava);
     // Harten: sign(1, delt - lamda) >= 0 selects (l^2 + d^2)/(2d)
-    if (delt - l1 >= (R)0) l1 = (l1 * l1 + delt * delt) / ((R)2.0 * delt);
-    if (delt - l2 >= (R)0) l2 = (l2 * l2 + delt * delt) / ((R)2.0 * delt);
-    if (delt - l5 >= (R)0) l5 = (l5 * l5 + delt * delt) / ((R)2.0 * delt);
-    R deltad = dr - dl, deltap = pr - pl;
-    R deltau = ur - ul, deltav = vr - vl, deltaw = wr - wl;
-    R deltauu = ubarr - ubarl;
-    R ava2 = ava * ava;
-    R alpha1 = l2 * (deltad - deltap / ava2);
-    R alpha2 = l5 * (deltap + avd * ava * deltauu) / ((R)2.0 * ava2);
-    R alpha3 = l1 * (deltap - avd * ava * deltauu) / ((R)2.0 * ava2);
-    R alpha4 = alpha1 + alpha2 + alpha3;
-    R alpha5 = ava * (alpha2 - alpha3);
-    R alpha6 = l2 * (avd * deltau - avd * deltauu * nx);
-    R alpha7 = l2 * (avd * deltav - avd * deltauu * ny);
-    R alpha8 = l2 * (avd * deltaw - avd * deltauu * nz);
-    R f0 = alpha4;
-    R f1 = avu * alpha4 + alpha5 * nx + alpha6;
-    R f2 = avv * alpha4 + alpha5 * ny + alpha7;
-    R f3 = avw * alpha4 + alpha5 * nz + alpha8;
-    R f4 = avh * alpha4 + avuu * alpha5 + avu * alpha6 + avv * alpha7 + avw * alpha8
-         - ava2 * alpha1 / (gamma - (R)1.0);
+    if (delt - l1 >= (R)0) l1 = (l1 * l1 + delt * delt) * rdelt2;
+    if (delt - l2 >= (R)0) l2 = (l2 * l2 + delt * delt) * rdelt2;
+    if (delt - l5 >= (R)0) l5 = (l5 * l5 + delt * delt) * rdelt2;
+    const R deltad = dr - dl, deltap = pr - pl;
+    const R deltau = ur - ul, deltav = vr - vl, deltaw = wr - wl;
+    const R deltauu = ubarr - ubarl;
+    const R ava2 = ava * ava;
+    const R rh = (R)1.0 / ((R)2.0 * ava2);
+    const R alpha1 = l2 * (deltad - deltap * ((R)2.0 * rh));
+    const R alpha2 = l5 * (deltap + avd * ava * deltauu) * rh;
+    const R alpha3 = l1 * (deltap - avd * ava * deltauu) * rh;
+    const R alpha4 = alpha1 + alpha2 + alpha3;
+    const R alpha5 = ava * (alpha2 - alpha3);
+    const R alpha6 = l2 * (avd * deltau - avd * deltauu * nx);
+    const R alpha7 = l2 * (avd * deltav - avd * deltauu * ny);
+    const R alpha8 = l2 * (avd * deltaw - avd * deltauu * nz);
+    const R f0 = alpha4;
+    const R f1 = avu * alpha4 + alpha5 * nx + alpha6;
+    const R f2 = avv * alpha4 + alpha5 * ny + alpha7;
+    const R f3 = avw * alpha4 + alpha5 * nz + alpha8;
+    const R f4 = avh * alpha4 + avuu * alpha5 + avu * alpha6 + avv * alpha7 + avw * alpha8
+               - ava2 * alpha1 * rg1;
     out[0] = (R)0.5 * (dl * uul + dr * uur - temp * f0);
     out[1] = (R)0.5 * (dl * ul * uul + mx * pl + (dr * ur * uur + mx * pr) - temp * f1);
     out[2] = (R)0.5 * (dl * vl * uul + my * pl + (dr * vr * uur + my * pr) - temp * f2);
@@ -153,65 +152,158 @@ __device__ inline void roe_flux(const R *ql, const R *qr, R mx, R my, R mz, R ga
 template <typename R>
 __device__ inline void cons2prim(const R *u, R *o, R gamma)
 {
+    const R r0 = (R)1.0 / u[0];
     o[0] = u[0];
-    o[1] = u[1] / u[0];
-    o[2] = u[2] / u[0];
-    o[3] = u[3] / u[0];
-    o[4] = (u[4] - (R)0.5 * (u[1] * u[1] + u[2] * u[2] + u[3] * u[3]) / u[0]) * (gamma - (R)1.0);
+    o[1] = u[1] * r0;
+    o[2] = u[2] * r0;
+    o[3] = u[3] * r0;
+    o[4] = (u[4] - (R)0.5 * (u[1] * u[1] + u[2] * u[2] + u[3] * u[3]) * r0) * (gamma - (R)1.0);
 }
 
-// Flux through the face between node n (axis position a) and n + st, with
-// WENO3 reconstruction (first order at the extended edges alo / ahi).
+// Flux through the face between nodes a and a+1 of one line from the four
+// values w[0..3] = q at a-1, a, a+1, a+2 of every component (alo / ahi are
+// the extended range ends: first order there) and the face metric.
 template <typename R>
-__device__ inline void face_flux(const double *__restrict__ q, const double *__restrict__ dxidx,
-                                 const double *__restrict__ inv_j, long N, long n, long st,
-                                 int a, int alo, int ahi, int axis, R gamma, R *F)
+__device__ inline void face_flux_w(const R (*w)[4], int a, int alo, int ahi, R mx, R my, R mz, R gamma,
+                                   R *F)
 {
     R ul[5], ur[5];
 #pragma unroll
     for (int v = 0; v < 5; v++) {
-        const double *qv = q + v * N;
-        R f0 = (R)qv[n], f1 = (R)qv[n + st];
-        if (a == alo) ul[v] = f0;
-        else ul[v] = weno3_left<R>((R)qv[n - st], f0, f1);
-        if (a == ahi - 1) ur[v] = f1;
-        else ur[v] = weno3_right<R>(f0, f1, (R)qv[n + 2 * st]);
+        ul[v] = a == alo ? w[v][1] : weno3<R>(w[v][0], w[v][1], w[v][2]);
+        ur[v] = a == ahi - 1 ? w[v][2] : weno3<R>(w[v][3], w[v][2], w[v][1]);
     }
     R pl[5], pr[5];
     cons2prim<R>(ul, pl, gamma);
     cons2prim<R>(ur, pr, gamma);
-    const int c0 = 3 * axis;
-    double j0 = inv_j[n], j1 = inv_j[n + st];
-    R mx = (R)(0.5 * (dxidx[c0 * N + n] * j0 + dxidx[c0 * N + n + st] * j1));
-    R my = (R)(0.5 * (dxidx[(c0 + 1) * N + n] * j0 + dxidx[(c0 + 1) * N + n + st] * j1));
-    R mz = (R)(0.5 * (dxidx[(c0 + 2) * N + n] * j0 + dxidx[(c0 + 2) * N + n + st] * j1));
     roe_flux<R>(pl, pr, mx, my, mz, gamma, F);
 }
 
-// One direction of inviscid_flux_1 (sub_inviscid1.f90:113-243) for the
-// computed nodes: rhsi = -(F(a+1/2) - F(a-1/2)) for axis 0, rhsi -= ... else.
-template <typename R, int AXIS>
-__global__ void __launch_bounds__(256) k_inviscid_dir(Ext e, Box b, const double *__restrict__ q,
-                                                      const double *__restrict__ dxidx,
-                                                      const double *__restrict__ inv_j,
-                                                      double *__restrict__ rhsi, double gamma)
+// face metric 0.5 (grad xi_d / J at a + at a+1), sub_inviscid_splitting.f90:65-67
+__device__ inline void face_metric(const double *__restrict__ dxidx, const double *__restrict__ inv_j, long N,
+                                   long n, long st, int c0, double *m)
 {
-    int i, j, k;
-    if (!box_node(b, i, j, k)) return;
+    const double j0 = inv_j[n], j1 = inv_j[n + st];
+#pragma unroll
+    for (int c = 0; c < 3; c++) m[c] = 0.5 * (dxidx[(c0 + c) * N + n] * j0 + dxidx[(c0 + c) * N + n + st] * j1);
+}
+
+// The i direction (sub_inviscid1.f90:113-145): a wavefront of 64 lanes along
+// i evaluates the faces a+1/2, a = i0-1 .. i0+62, once each; lane L > 0 takes
+// F(a-1/2) from lane L-1, so a wave produces 63 nodes.  rhsi = -(F+ - F-).
+constexpr int kInvI = 63;
+template <typename R>
+__global__ void __launch_bounds__(256) k_inviscid_i(Ext e, const double *__restrict__ q,
+                                                   const double *__restrict__ dxidx,
+                                                   const double *__restrict__ inv_j,
+                                                   double *__restrict__ rhsi, double gamma)
+{
+    const int lane = threadIdx.x;
+    const int j = e.js1 + (int)(blockIdx.y * blockDim.y + threadIdx.y);
+    const int k = e.ks1 + (int)blockIdx.z;
+    const int a_raw = e.is1 - 1 + (int)blockIdx.x * kInvI + lane;
+    const bool row = j <= e.je1;
+    // lanes past the range evaluate a valid face and discard it, so that
+    // every lane takes part in the exchange
+    const int a = min(a_raw, e.ie1);
     const long N = e.n;
-    long n = e.ix(i, j, k);
-    long st = e.stride(AXIS);
-    int a = AXIS == 0 ? i : AXIS == 1 ? j : k;
-    int alo = e.lo(AXIS), ahi = e.hi(AXIS);
-    R fm[5], fp[5];
-    face_flux<R>(q, dxidx, inv_j, N, n - st, st, a - 1, alo, ahi, AXIS, (R)gamma, fm);
-    face_flux<R>(q, dxidx, inv_j, N, n, st, a, alo, ahi, AXIS, (R)gamma, fp);
+    const long n = e.ix(a, row ? j : e.je1, k);
+    // the outer values of the window are not used at the extended range
+    // ends (first-order states there) and would lie outside the block
+    const long nm = a > e.is ? n - 1 : n, np = a + 2 <= e.ie ? n + 2 : n + 1;
+    R w[5][4];
 #pragma unroll
     for (int v = 0; v < 5; v++) {
-        double dF = (double)fp[v] - (double)fm[v];
-        if (AXIS == 0) rhsi[v * N + n] = -dF;
-        else rhsi[v * N + n] = rhsi[v * N + n] - dF;
+        const double *qv = q + v * N;
+        w[v][0] = (R)qv[nm];
+        w[v][1] = (R)qv[n];
+        w[v][2] = (R)qv[n + 1];
+        w[v][3] = (R)qv[np];
     }
+    double m[3];
+    face_metric(dxidx, inv_j, N, n, 1, 0, m);
+    R F[5];
+    face_flux_w<R>(w, a, e.is, e.ie, (R)m[0], (R)m[1], (R)m[2], (R)gamma, F);
+    double Fm[5];
+#pragma unroll
+    for (int v = 0; v < 5; v++) Fm[v] = __shfl_up((double)F[v], 1, 64);
+    if (row && lane > 0 && a_raw <= e.ie1) {
+#pragma unroll
+        for (int v = 0; v < 5; v++) rhsi[v * N + n] = -((double)F[v] - Fm[v]);
+    }
+}
+
+// The j and k directions (:160-243): each thread marches along the axis
+// over kInvRun nodes, keeping the four-value window and the previous face
+// flux in registers, so each face is evaluated once.  rhsi -= (F+ - F-).
+constexpr int kInvRun = 16;
+template <typename R, int AXIS>
+__global__ void __launch_bounds__(256) k_inviscid_jk(Ext e, const double *__restrict__ q,
+                                                    const double *__restrict__ dxidx,
+                                                    const double *__restrict__ inv_j,
+                                                    double *__restrict__ rhsi, double gamma)
+{
+    const int i = e.is1 + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    const int o = (int)(blockIdx.y * blockDim.y + threadIdx.y);  // the other transverse index
+    const int a0 = e.lo1(AXIS) + (int)blockIdx.z * kInvRun;
+    const int a1 = min(a0 + kInvRun - 1, e.hi1(AXIS));
+    const int j = AXIS == 1 ? a0 : e.js1 + o;
+    const int k = AXIS == 2 ? a0 : e.ks1 + o;
+    if (i > e.ie1 || (AXIS == 1 ? k > e.ke1 : j > e.je1)) return;
+    const long N = e.n, st = e.stride(AXIS);
+    const int alo = e.lo(AXIS), ahi = e.hi(AXIS);
+    const int c0 = 3 * AXIS;
+    long n = e.ix(i, j, k) - st;  // node a0-1: first face is (a0-1)+1/2
+    // window values outside the extended range are never used (first-order
+    // states at its ends); their loads are redirected inside the block
+    const long back = a0 - 2 >= alo ? st : 0;
+    R w[5][4];
+#pragma unroll
+    for (int v = 0; v < 5; v++) {
+        const double *qv = q + v * N + n;
+        w[v][0] = (R)qv[-back];
+        w[v][1] = (R)qv[0];
+        w[v][2] = (R)qv[st];
+        w[v][3] = (R)qv[2 * st];
+    }
+    R Fm[5], Fp[5];
+    {
+        double m[3];
+        face_metric(dxidx, inv_j, N, n, st, c0, m);
+        face_flux_w<R>(w, a0 - 1, alo, ahi, (R)m[0], (R)m[1], (R)m[2], (R)gamma, Fm);
+    }
+    for (int a = a0; a <= a1; a++) {
+        n += st;
+        const long ahead = a + 2 <= ahi ? 2 * st : st;
+#pragma unroll
+        for (int v = 0; v < 5; v++) {
+            w[v][0] = w[v][1];
+            w[v][1] = w[v][2];
+            w[v][2] = w[v][3];
+            w[v][3] = (R)q[v * N + n + ahead];
+        }
+        double m[3];
+        face_metric(dxidx, inv_j, N, n, st, c0, m);
+        face_flux_w<R>(w, a, alo, ahi, (R)m[0], (R)m[1], (R)m[2], (R)gamma, Fp);
+#pragma unroll
+        for (int v = 0; v < 5; v++) {
+            rhsi[v * N + n] = rhsi[v * N + n] - ((double)Fp[v] - (double)Fm[v]);
+            Fm[v] = Fp[v];
+        }
+    }
+}
+
+template <typename R>
+static void launch_inviscid_t(hipStream_t s, const Ext &e, const Fields &f, double gamma)
+{
+    const int Li = e.ie1 - e.is1 + 1, Lj = e.je1 - e.js1 + 1, Lk = e.ke1 - e.ks1 + 1;
+    dim3 t(64, 4, 1);
+    dim3 gi((Li + kInvI - 1) / kInvI, (Lj + 3) / 4, Lk);
+    hipLaunchKernelGGL((k_inviscid_i<R>), gi, t, 0, s, e, f.q, f.dxidx, f.inv_j, f.rhsi, gamma);
+    dim3 gj((Li + 63) / 64, (Lk + 3) / 4, (Lj + kInvRun - 1) / kInvRun);
+    hipLaunchKernelGGL((k_inviscid_jk<R, 1>), gj, t, 0, s, e, f.q, f.dxidx, f.inv_j, f.rhsi, gamma);
+    dim3 gk((Li + 63) / 64, (Lj + 3) / 4, (Lk + kInvRun - 1) / kInvRun);
+    hipLaunchKernelGGL((k_inviscid_jk<R, 2>), gk, t, 0, s, e, f.q, f.dxidx, f.inv_j, f.rhsi, gamma);
 }
 
 int launch_inviscid(hipStream_t s, const Ext &e, const Fields &f, const Consts &c, int precision)
@@ -219,18 +311,9 @@ int launch_inviscid(hipStream_t s, const Ext &e, const Fields &f, const Consts &
     // rhsi is zero outside the computed range (sub_inviscid1.f90:102)
     int rc = launch_fill(s, f.rhsi, 5 * e.n, 0.0);
     if (rc) return rc;
-    Box b{e.is1, e.ie1, e.js1, e.je1, e.ks1, e.ke1};
-    dim3 g = box_grid(b), t(64, 4, 1);
-    if (g.x == 0) return 0;
-    if (precision == 1) {
-        hipLaunchKernelGGL((k_inviscid_dir<float, 0>), g, t, 0, s, e, b, f.q, f.dxidx, f.inv_j, f.rhsi, c.gamma);
-        hipLaunchKernelGGL((k_inviscid_dir<float, 1>), g, t, 0, s, e, b, f.q, f.dxidx, f.inv_j, f.rhsi, c.gamma);
-        hipLaunchKernelGGL((k_inviscid_dir<float, 2>), g, t, 0, s, e, b, f.q, f.dxidx, f.inv_j, f.rhsi, c.gamma);
-    } else {
-        hipLaunchKernelGGL((k_inviscid_dir<double, 0>), g, t, 0, s, e, b, f.q, f.dxidx, f.inv_j, f.rhsi, c.gamma);
-        hipLaunchKernelGGL((k_inviscid_dir<double, 1>), g, t, 0, s, e, b, f.q, f.dxidx, f.inv_j, f.rhsi, c.gamma);
-        hipLaunchKernelGGL((k_inviscid_dir<double, 2>), g, t, 0, s, e, b, f.q, f.dxidx, f.inv_j, f.rhsi, c.gamma);
-    }
+    if (e.ie1 < e.is1 || e.je1 < e.js1 || e.ke1 < e.ks1) return 0;
+    if (precision == 1) launch_inviscid_t<float>(s, e, f, c.gamma);
+    else launch_inviscid_t<double>(s, e, f, c.gamma);
     ACFD_CHECK_LAUNCH();
     return 0;
 }

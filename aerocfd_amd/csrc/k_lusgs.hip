@@ -239,6 +239,7 @@ constexpr unsigned long long kSentinel = 0xFFF5A5A5A5A5A5A5ull;
 // geometry of one tiling of a block's computed range
 struct SlGeom {
     int TJ, TK, TT, nbj, nbk, ntiles, nsteps, Li, Lj, Lk;
+    int lgTJ;  // TJ is a power of two
     long exj_tile, exk_tile;  // doubles per tile of the j / k exchange regions
     // fields are stored in pairs, so a lane's operands of one step are 12
     // 16-byte loads; double index of field f of lane l at row t
@@ -257,6 +258,8 @@ static SlGeom sl_geom(const Ext &e, int shape)
     g.TJ = kTileShapes[shape][0];
     g.TK = kTileShapes[shape][1];
     g.TT = g.TJ * g.TK;
+    g.lgTJ = 0;
+    while ((1 << g.lgTJ) < g.TJ) g.lgTJ++;
     g.Li = e.ie1 - e.is1 + 1;
     g.Lj = e.je1 - e.js1 + 1;
     g.Lk = e.ke1 - e.ks1 + 1;
@@ -319,109 +322,161 @@ __device__ inline void lds_barrier()
 }
 
 // The layout passes move data between the natural layout (i fastest) and
-// the sweep layout through LDS.  A workgroup handles one tile and a run of
-// kChunksPerWg chunks of kCT steps, in sequence (the cache lines a chunk
-// straddles are reused by the next one).  In a chunk starting at step t0,
-// lane l's line contributes the kCT nodes li = t0 + ii - tj - tk,
-// ii = 0..kCT-1, which are exactly rows t0 + ii of the sweep layout.
-// Natural side: thread x handles ii = x % kCT of lines x / kCT + m TT/kCT,
-// so 32 consecutive lanes touch 32 consecutive doubles of one line.
-// Sweep side: thread x is lane x and touches row slices of TT doubles.
-// LDS holds [kCT][TT] doubles (64 KiB at TT = 256) with the line index
-// XOR-swizzled by ii, so both access patterns are free of bank conflicts.
-constexpr int kCT = 32;
-constexpr int kChunksPerWg = 3;
+// the sweep layout through LDS.  One workgroup per tile walks the tile's
+// steps in chunks of CT steps, so consecutive chunks of a line
+// are adjacent in memory and the cache lines a chunk boundary splits are
+// completed by the same workgroup moments later.  In a chunk starting at
+// step t0, lane l's line contributes the CT nodes li = t0 + ii - tj - tk,
+// ii = 0..CT-1, which are exactly rows t0 + ii of the sweep layout.
+// Natural side: thread x handles ii = x % CT of lines x / CT + m TT/CT,
+// so CT consecutive lanes touch CT consecutive doubles of one line
+// (CT = min(32, 4096/TT)).
+// Sweep side: thread x is lane x.  LDS holds [CT][TT] doubles per field
+// with the line index XOR-swizzled by ii (no bank conflicts either way).
+constexpr int kSlElems = 4096;  // CT * TT at most
+constexpr int sl_ct_of(int tt) { return kSlElems / tt < 32 ? kSlElems / tt : 32; }
 
-struct LaneLine {
-    int tj, tk;
-    bool ok;
-};
-__device__ inline LaneLine lane_line(const SlGeom &g, int tile, int line)
+// node index of (tile, line, li), li in range
+__device__ inline long sl_nat(const Ext &e, const SlGeom &g, int tile, int line, int li)
 {
-    LaneLine r;
-    r.tj = line % g.TJ;
-    r.tk = line / g.TJ;
-    r.ok = (tile % g.nbj) * g.TJ + r.tj < g.Lj && (tile / g.nbj) * g.TK + r.tk < g.Lk;
-    return r;
+    const int tj = line & (g.TJ - 1), tk = line >> g.lgTJ;
+    return e.ix(e.is1 + li, e.js1 + (tile % g.nbj) * g.TJ + tj, e.ks1 + (tile / g.nbj) * g.TK + tk);
+}
+__device__ inline bool sl_line_ok(const SlGeom &g, int tile, int line)
+{
+    return (tile % g.nbj) * g.TJ + (line & (g.TJ - 1)) < g.Lj && (tile / g.nbj) * g.TK + (line >> g.lgTJ) < g.Lk;
 }
 
-// natural index of (tile, line, node li) -- li already range-checked
-__device__ inline long nat_index(const Ext &e, const SlGeom &g, int tile, const LaneLine &ln, int li)
+__device__ inline const double *gather_src(int f, long N, const double *q, const double *dxidx,
+                                           const double *inv_j, const double *rr, const double *diag,
+                                           const double *rhs)
 {
-    return e.ix(e.is1 + li, e.js1 + (tile % g.nbj) * g.TJ + ln.tj, e.ks1 + (tile / g.nbj) * g.TK + ln.tk);
+    return f < 5 ? q + f * N : f < 14 ? dxidx + (f - 5) * N : f == 14 ? inv_j
+         : f < 18 ? rr + (f - 15) * N : f == 18 ? diag : rhs + (f - 19) * N;
 }
 
-// Gather the 24 operands of every node into the sweep layout.
-__global__ void __launch_bounds__(256) k_sl_gather(Ext e, SlGeom g, const double *__restrict__ q,
-                                                   const double *__restrict__ dxidx,
-                                                   const double *__restrict__ inv_j,
-                                                   const double *__restrict__ rr,
-                                                   const double *__restrict__ diag,
-                                                   const double *__restrict__ rhs, double *__restrict__ sl)
+// Gather the 24 operands of every node into the sweep layout, a field pair
+// at a time (each lane then stores whole 16-byte pairs).  TT is a template
+// parameter so that a thread's CT loads are issued back to back.
+template <int TT>
+__global__ void __launch_bounds__(TT) k_sl_gather(Ext e, SlGeom g, const double *__restrict__ q,
+                                                  const double *__restrict__ dxidx,
+                                                  const double *__restrict__ inv_j,
+                                                  const double *__restrict__ rr,
+                                                  const double *__restrict__ diag,
+                                                  const double *__restrict__ rhs, double *__restrict__ sl)
 {
-    extern __shared__ double buf[];  // [kCT][TT], swizzled
-    const int tile = blockIdx.x, TT = g.TT, x = threadIdx.x;
-    const LaneLine me = lane_line(g, tile, x);
-    const int ii_n = x % kCT, lstep = TT / kCT;
+    constexpr int CT = sl_ct_of(TT), lstep = TT / CT;
+    extern __shared__ double buf[];  // [2][CT][TT], swizzled
+    const int tile = blockIdx.x, x = threadIdx.x;
+    const int ii_n = x % CT;
+    const int tj = x & (g.TJ - 1), tk = x >> g.lgTJ;
+    const bool me_ok = sl_line_ok(g, tile, x);
     const long N = e.n;
-    const int nch = (g.nsteps + kCT - 1) / kCT;
-    const int c1 = min(nch, ((int)blockIdx.y + 1) * kChunksPerWg);
-    for (int c = blockIdx.y * kChunksPerWg; c < c1; c++) {
-        const int t0 = c * kCT;
-        for (int f = 0; f < kSlF; f++) {
-            const double *src = f < 5 ? q + f * N : f < 14 ? dxidx + (f - 5) * N : f == 14 ? inv_j
-                              : f < 18 ? rr + (f - 15) * N : f == 18 ? diag : rhs + (f - 19) * N;
-            for (int m = 0; m < kCT; m++) {
-                const int line = m * lstep + x / kCT;
-                const LaneLine ln = lane_line(g, tile, line);
-                const int li = t0 + ii_n - ln.tj - ln.tk;
-                double v = 0.0;
-                if (ln.ok && li >= 0 && li < g.Li) v = src[nat_index(e, g, tile, ln, li)];
-                buf[ii_n * TT + (line ^ ii_n)] = v;
+    double2 *sl2 = reinterpret_cast<double2 *>(sl);
+    for (int t0 = 0; t0 < g.nsteps; t0 += CT) {
+        // natural-side node offsets of this thread's CT elements (-1: none)
+        int nat[CT];  // node offsets fit 32 bits (blocks below 2^31 nodes)
+#pragma unroll
+        for (int m = 0; m < CT; m++) {
+            const int line = m * lstep + x / CT;
+            const int li = t0 + ii_n - (line & (g.TJ - 1)) - (line >> g.lgTJ);
+            nat[m] = (t0 + ii_n < g.nsteps && li >= 0 && li < g.Li && sl_line_ok(g, tile, line))
+                         ? (int)sl_nat(e, g, tile, line, li) : -1;
+        }
+        for (int fp = 0; fp < kSlF / 2; fp++) {
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                const double *src = gather_src(2 * fp + h, N, q, dxidx, inv_j, rr, diag, rhs);
+                double v[CT];
+#pragma unroll
+                for (int m = 0; m < CT; m++) v[m] = nat[m] >= 0 ? src[nat[m]] : 0.0;
+#pragma unroll
+                for (int m = 0; m < CT; m++) {
+                    const int line = m * lstep + x / CT;
+                    buf[(h * CT + ii_n) * TT + (line ^ ii_n)] = v[m];
+                }
             }
             __syncthreads();
-            if (me.ok)
-                for (int ii = 0; ii < kCT; ii++) {
-                    const int t = t0 + ii, li = t - me.tj - me.tk;
-                    if (t < g.nsteps && li >= 0 && li < g.Li) sl[g.slot(tile, t, f, x)] = buf[ii * TT + (x ^ ii)];
+            if (me_ok) {
+#pragma unroll
+                for (int ii = 0; ii < CT; ii++) {
+                    const int t = t0 + ii, li = t - tj - tk;
+                    if (t < g.nsteps && li >= 0 && li < g.Li)
+                        sl2[g.slot(tile, t, 2 * fp, x) >> 1] =
+                            make_double2(buf[ii * TT + (x ^ ii)], buf[(CT + ii) * TT + (x ^ ii)]);
                 }
+            }
             __syncthreads();
         }
     }
 }
 
-// q += dq from the sweep layout (:568-574), same work split as the gather
-__global__ void __launch_bounds__(256) k_sl_add(Ext e, SlGeom g, const double *__restrict__ sl,
-                                                double *__restrict__ q)
+// q += dq from the sweep layout (:568-574), same walk as the gather, one
+// component at a time.
+template <int TT>
+__global__ void __launch_bounds__(TT) k_sl_add(Ext e, SlGeom g, const double *__restrict__ sl,
+                                               double *__restrict__ q)
 {
-    extern __shared__ double buf[];  // [kCT][TT], swizzled
-    const int tile = blockIdx.x, TT = g.TT, x = threadIdx.x;
-    const LaneLine me = lane_line(g, tile, x);
-    const int ii_n = x % kCT, lstep = TT / kCT;
+    constexpr int CT = sl_ct_of(TT), lstep = TT / CT;
+    extern __shared__ double buf[];  // [CT][TT], swizzled
+    const int tile = blockIdx.x, x = threadIdx.x;
+    const int ii_n = x % CT;
+    const int tj = x & (g.TJ - 1), tk = x >> g.lgTJ;
+    const bool me_ok = sl_line_ok(g, tile, x);
     const long N = e.n;
-    const int nch = (g.nsteps + kCT - 1) / kCT;
-    const int c1 = min(nch, ((int)blockIdx.y + 1) * kChunksPerWg);
-    for (int c = blockIdx.y * kChunksPerWg; c < c1; c++) {
-        const int t0 = c * kCT;
+    for (int t0 = 0; t0 < g.nsteps; t0 += CT) {
+        int nat[CT];  // node offsets fit 32 bits (blocks below 2^31 nodes)
+#pragma unroll
+        for (int m = 0; m < CT; m++) {
+            const int line = m * lstep + x / CT;
+            const int li = t0 + ii_n - (line & (g.TJ - 1)) - (line >> g.lgTJ);
+            nat[m] = (t0 + ii_n < g.nsteps && li >= 0 && li < g.Li && sl_line_ok(g, tile, line))
+                         ? (int)sl_nat(e, g, tile, line, li) : -1;
+        }
         for (int v = 0; v < 5; v++) {
-            if (me.ok)
-                for (int ii = 0; ii < kCT; ii++) {
-                    const int t = t0 + ii, li = t - me.tj - me.tk;
-                    if (t < g.nsteps && li >= 0 && li < g.Li) buf[ii * TT + (x ^ ii)] = sl[g.slot(tile, t, kSlOut + v, x)];
+            if (me_ok) {
+                double d[CT];
+#pragma unroll
+                for (int ii = 0; ii < CT; ii++) {
+                    const int t = t0 + ii, li = t - tj - tk;
+                    d[ii] = (t < g.nsteps && li >= 0 && li < g.Li) ? sl[g.slot(tile, t, kSlOut + v, x)] : 0.0;
                 }
+#pragma unroll
+                for (int ii = 0; ii < CT; ii++) buf[ii * TT + (x ^ ii)] = d[ii];
+            }
             __syncthreads();
-            for (int m = 0; m < kCT; m++) {
-                const int line = m * lstep + x / kCT;
-                const LaneLine ln = lane_line(g, tile, line);
-                const int li = t0 + ii_n - ln.tj - ln.tk;
-                if (ln.ok && li >= 0 && li < g.Li && t0 + ii_n < g.nsteps) {
-                    const long n = v * N + nat_index(e, g, tile, ln, li);
-                    q[n] = q[n] + buf[ii_n * TT + (line ^ ii_n)];
-                }
+            double a[CT];
+#pragma unroll
+            for (int m = 0; m < CT; m++) a[m] = nat[m] >= 0 ? q[v * N + nat[m]] : 0.0;
+#pragma unroll
+            for (int m = 0; m < CT; m++) {
+                const int line = m * lstep + x / CT;
+                if (nat[m] >= 0) q[v * N + nat[m]] = a[m] + buf[ii_n * TT + (line ^ ii_n)];
             }
             __syncthreads();
         }
     }
+}
+
+template <int TT>
+static void launch_layout(hipStream_t s, int which, const Ext &e, const SlGeom &g, const Fields &f,
+                          const LuWs &ws)
+{
+    const size_t lds = 2 * (size_t)kSlElems * sizeof(double);
+    if (which == 0)
+        hipLaunchKernelGGL(k_sl_gather<TT>, dim3((unsigned)g.ntiles), dim3(TT), lds, s, e, g, f.q, f.dxidx,
+                           f.inv_j, f.w_r, f.w_diag, f.rhs, ws.sl);
+    else
+        hipLaunchKernelGGL(k_sl_add<TT>, dim3((unsigned)g.ntiles), dim3(TT), lds, s, e, g, ws.sl, f.q);
+}
+
+static void launch_layout_tt(hipStream_t s, int which, const Ext &e, const SlGeom &g, const Fields &f,
+                             const LuWs &ws)
+{
+    if (g.TT == 256) launch_layout<256>(s, which, e, g, f, ws);
+    else if (g.TT == 128) launch_layout<128>(s, which, e, g, f, ws);
+    else launch_layout<64>(s, which, e, g, f, ws);
 }
 
 // Fill exchange slots with the sentinel.
@@ -635,18 +690,14 @@ int launch_lusgs_phase(hipStream_t s, const Ext &e, const Fields &f, const Const
         return 0;
     } else if (tiles) {
         const SlGeom g = sl_geom(e, shape);
-        const int nch = (g.nsteps + kCT - 1) / kCT;
-        const dim3 lg((unsigned)g.ntiles, (unsigned)((nch + kChunksPerWg - 1) / kChunksPerWg));
-        const size_t lds = (size_t)kCT * g.TT * sizeof(double);
         if (phase == 1)
-            hipLaunchKernelGGL(k_sl_gather, lg, dim3(g.TT), lds, s, e, g, f.q, f.dxidx, f.inv_j, f.w_r, f.w_diag,
-                               f.rhs, ws->sl);
+            launch_layout_tt(s, 0, e, g, f, *ws);
         else if (phase == 2)
             launch_tiles_shape<1>(s, shape, g, *ws, c.gamma);
         else if (phase == 3)
             launch_tiles_shape<-1>(s, shape, g, *ws, c.gamma);
         else
-            hipLaunchKernelGGL(k_sl_add, lg, dim3(g.TT), lds, s, e, g, ws->sl, f.q);
+            launch_layout_tt(s, 1, e, g, f, *ws);
     } else if (phase == 4) {
         Box b{e.is1, e.ie1, e.js1, e.je1, e.ks1, e.ke1};
         hipLaunchKernelGGL(k_add5, box_grid(b), dim3(64, 4, 1), 0, s, e, b, f.q, f.w_dq);
