@@ -48,6 +48,18 @@ HYPERPLANE_BYTES = {  # the hyperplane sweeps read the operands in place
 }
 
 
+# HBM bytes per launch of the timed kernels on the default workload (TGV
+# 256^3, FP64, tile sweeps), from two rocprofv3 --pmc runs (FETCH_SIZE,
+# WRITE_SIZE) scaled by a 1 GiB copy; profiles/tgv256_fp64_hbm_traffic.md.
+MEASURED_TRAFFIC_256 = {
+    "lusgs_forward": 5.409e9,
+    "lusgs_backward": 5.407e9,
+    "lusgs_gather": 8.165e9,
+    "lusgs_add": 3.132e9,
+    "lusgs_other": 5.578e9,
+}
+
+
 def cpu_baseline(n_sample=96, steps=8, timeout=300):
     """Time the reference's own Fortran (or the oracle) on a TGV sample of
     n_sample^3 nodes for `steps` steps in a child process (one core)."""
@@ -116,6 +128,9 @@ def main():
     ap.add_argument("--precision", type=int, default=0, help="0 FP64, 1 FP32 flux / FP64 update")
     ap.add_argument("--lusgs", type=int, default=1, help="0 hyperplane launches, 1 pipelined tile sweep")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe-mb", type=int, default=0,
+                    help="first run one device-to-device copy of this many MiB (a known-size kernel that "
+                         "calibrates HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; every block of the workload is resident on it. "
@@ -125,6 +140,14 @@ def main():
 
     import numpy as np
     from aerocfd_amd import Solver, cases
+
+    if args.copy_probe_mb > 0:
+        import torch
+        src = torch.ones(args.copy_probe_mb * (1 << 20) // 8, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     case = cases.tgv_box(args.n)
     case.cfg.precision = args.precision
@@ -167,6 +190,8 @@ def main():
     avg_ms = cand[dom][0] / cand[dom][1]
     bytes_launch = table[dom] * comp
     achieved = bytes_launch / (avg_ms * 1e-3) / 1e9
+    default_cfg = args.n == 256 and args.lusgs == 1 and args.precision == 0
+    traffic = MEASURED_TRAFFIC_256.get(dom) if default_cfg else None
     out = {
         "metric": "Mcell-updates/s",
         "value": round(value, 3),
@@ -184,7 +209,8 @@ def main():
                    "blocks": 1, "mach": 0.1, "re": 1600, "scheme": "WENO3+Roe, central viscous, LU-SGS",
                    "lusgs_kernel": "pipelined-tiles" if args.lusgs == 1 else "hyperplanes"},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": PEAK_HBM_GBS,
-                     "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4),
+                     "traffic": traffic,
                      "bytes_per_node": table[dom], "avg_launch_ms": round(avg_ms, 4)},
         "stage_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in tm.items() if v[1] > 0},
         "residual": [float("%.6e" % x) for x in res[:5]],
