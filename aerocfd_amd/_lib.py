@@ -25,8 +25,8 @@ class Config(C.Structure):
     """acfd_config"""
     _fields_ = [(n, C.c_double) for n in ("ma", "re", "tinf", "aoa_deg", "cfl", "dt", "prl", "prt")] + [
         (n, C.c_int) for n in ("iflag_solver", "iflag_time", "iflag_turbulence", "iflag_des",
-                               "iflag_timeadvance", "precision", "lusgs_kernel")] + [
-        ("reserved", C.c_int * 5)]
+                               "iflag_timeadvance", "precision", "lusgs_kernel", "inviscid_scheme")] + [
+        ("reserved", C.c_int * 4)]
 
 
 class Subface(C.Structure):

@@ -432,7 +432,11 @@ int st_getpri(acfd_ctx *ctx, int m)
 int st_inviscid(acfd_ctx *ctx, int m)
 {
     StageTimer t(ctx, ACFD_STAGE_INVISCID);
-    KRN_TRY(launch_inviscid(ctx->stream, ctx->blk[m].e, ctx->blk[m].f, ctx->c, ctx->cfg.precision));
+    if (ctx->cfg.inviscid_scheme == 0)
+        KRN_TRY(launch_inviscid(ctx->stream, ctx->blk[m].e, ctx->blk[m].f, ctx->c, ctx->cfg.precision));
+    else
+        KRN_TRY(launch_inviscid_ext(ctx->stream, ctx->blk[m].e, ctx->blk[m].f, ctx->c, ctx->cfg.precision,
+                                    ctx->cfg.inviscid_scheme));
     return 0;
 }
 
@@ -619,8 +623,30 @@ int one_step(acfd_ctx *ctx)
         for (int m = 0; m < nb && !rc && sa; m++) rc = st_sa(ctx, m);
         if (!rc) rc = refresh(ctx);
         for (int m = 0; m < nb && !rc; m++) rc = st_getpri(ctx, m);
+    } else if (ctx->cfg.iflag_timeadvance == 4) {
+        // explicit Shu-Osher RK3: q^n is kept in w_dqs (free in this mode)
+        for (int m = 0; m < nb; m++) {
+            const Block &b = ctx->blk[m];
+            KRN_TRY(launch_copy(ctx->stream, b.f.q, b.f.w_dqs, 5 * b.e.n));
+        }
+        for (int stage = 0; stage < 3 && !rc; stage++) {
+            if (stage > 0) rc = refresh(ctx);
+            for (int m = 0; m < nb && !rc; m++) {
+                rc = st_getpri(ctx, m);
+                if (!rc) rc = st_inviscid(ctx, m);
+                if (!rc && ctx->c.solver == 0) rc = st_viscous(ctx, m, 0);
+                if (!rc && stage == 0) rc = st_find_dt(ctx, m);
+                if (!rc) {
+                    const Block &b = ctx->blk[m];
+                    rc = launch_rk3_stage(ctx->stream, b.e, b.f, b.f.w_dqs, stage, ctx->c.solver == 0)
+                             ? fail(ACFD_EDEVICE, "rk3 stage launch failed") : 0;
+                }
+            }
+        }
+        if (!rc) rc = refresh(ctx);
+        for (int m = 0; m < nb && !rc; m++) rc = st_getpri(ctx, m);
     } else {
-        return fail(ACFD_EINVAL, "iflag_timeadvance must be 2 or 3");
+        return fail(ACFD_EINVAL, "iflag_timeadvance must be 2, 3 or 4");
     }
     if (!rc) rc = launch_residual_all(ctx) ? ACFD_EDEVICE : 0;
     return rc;
@@ -637,8 +663,14 @@ int acfd_create(const acfd_config *cfg, acfd_ctx **out)
 {
     if (!cfg || !out) return fail(ACFD_EINVAL, "acfd_create: null argument");
     if (cfg->ma <= 0.0) return fail(ACFD_EINVAL, "acfd_create: ma must be positive");
-    if (cfg->iflag_timeadvance != 2 && cfg->iflag_timeadvance != 3)
-        return fail(ACFD_EINVAL, "acfd_create: iflag_timeadvance must be 2 (1st implicit) or 3 (2nd crank)");
+    if (cfg->iflag_timeadvance < 2 || cfg->iflag_timeadvance > 4)
+        return fail(ACFD_EINVAL,
+                    "acfd_create: iflag_timeadvance must be 2 (1st implicit), 3 (2nd crank) or 4 (explicit RK3)");
+    if (cfg->inviscid_scheme < 0 || cfg->inviscid_scheme > 2)
+        return fail(ACFD_EINVAL, "acfd_create: inviscid_scheme must be 0 (WENO3 + Roe), 1 (Steger-Warming + WENO5) "
+                                 "or 2 (WCNS)");
+    if (cfg->iflag_timeadvance == 4 && cfg->iflag_turbulence == 2 && cfg->iflag_solver == 0)
+        return fail(ACFD_EINVAL, "acfd_create: the explicit RK3 advance is for laminar or Euler flow");
     if (cfg->iflag_turbulence != 0 && cfg->iflag_turbulence != 2)
         return fail(ACFD_EINVAL, "acfd_create: wrong turbulence model type");
     int ndev = 0;

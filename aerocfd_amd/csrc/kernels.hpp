@@ -14,6 +14,12 @@ int launch_find_dt(hipStream_t s, const Ext &e, const Fields &f, const Consts &c
                    unsigned long long *stats);
 int launch_reset_stats(hipStream_t s, unsigned long long *stats);
 
+// k_flux5.hip: schemes and time integration with no reference counterpart
+// (scheme 1 Steger-Warming + WENO5, 2 WCNS; Shu-Osher RK3)
+int launch_inviscid_ext(hipStream_t s, const Ext &e, const Fields &f, const Consts &c, int precision, int scheme);
+int launch_rk3_stage(hipStream_t s, const Ext &e, const Fields &f, const double *qn, int stage, bool viscous);
+int launch_copy(hipStream_t s, const double *a, double *b, long n);
+
 // k_lusgs.hip
 // workspace of the tile sweeps: sweep-layout operands (normal memory),
 // exchange slots (uncached memory, sentinel-filled) and an error word

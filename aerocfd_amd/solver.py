@@ -61,9 +61,10 @@ class FlowConfig:
     iflag_time: int = 1          # 0 steady, 1 unsteady
     iflag_turbulence: int = 0    # 0 laminar, 2 SA
     iflag_des: int = 0
-    iflag_timeadvance: int = 2   # 2 LU-SGS, 3 BDF2
+    iflag_timeadvance: int = 2   # 2 LU-SGS, 3 BDF2, 4 explicit RK3 (no reference counterpart)
     precision: int = 0           # 0 FP64, 1 FP32 flux / FP64 update
     lusgs_kernel: int = 0        # 0 hyperplane launches, 1 pipelined tiles
+    inviscid_scheme: int = 0     # 0 WENO3 + Roe (reference); 1 Steger-Warming + WENO5, 2 WCNS
 
     # non-dimensional freestream, sub_init_flowfield.f90:30-39
     @property

@@ -44,10 +44,15 @@ typedef struct acfd_config {
     int iflag_time;                 /* 0 = steady (local dt), 1 = unsteady */
     int iflag_turbulence;           /* 0 = laminar, 2 = Spalart-Allmaras */
     int iflag_des;                  /* 0 = RANS, 1 = DES97, 2 = DDES */
-    int iflag_timeadvance;          /* 2 = first-order implicit (LU-SGS), 3 = BDF2 sub-iterations */
+    int iflag_timeadvance;          /* 2 = first-order implicit (LU-SGS), 3 = BDF2 sub-iterations,
+                                       4 = explicit three-stage Shu-Osher RK3 (no counterpart in the
+                                       reference; laminar or Euler) */
     int precision;                  /* 0 = FP64 everywhere; 1 = FP32 reconstruction/splitting/viscous, FP64 update */
     int lusgs_kernel;               /* 0 = hyperplane launches, 1 = pipelined tile sweep (same results) */
-    int reserved[5];
+    int inviscid_scheme;            /* 0 = WENO3 + Roe (the reference's inviscid_flux_1); no counterpart
+                                       in the reference: 1 = Steger-Warming splitting + WENO5,
+                                       2 = WCNS (WENO5 midpoint interpolation + Roe, 6th-order difference) */
+    int reserved[4];
 } acfd_config;
 
 /* One subface record, as read from bcinfo (sub_init_readbcinfo.f90:96-98):

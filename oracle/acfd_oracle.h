@@ -144,6 +144,21 @@ int ora_step(int nblk, ora_block *blk, ora_config *c);
 /* Initial refresh before the time loop (main.f90:309-318) */
 void ora_prepare(int nblk, ora_block *blk, ora_config *c);
 
+/* Extensions with no counterpart in the reference (oracle_ext.c): published
+ * methods, pinned by closed forms. */
+void ora_sw_split(const double *q, const double *M, double gamma, double *fp, double *fm);
+double ora_weno5_rec(double a, double b, double c, double d, double e5);
+double ora_weno3_rec(double a, double b, double c);
+double ora_weno5_interp(double a, double b, double c, double d, double e5);
+int ora_wcns_diff_order(int a, int lo, int hi);
+/* scheme 1 = Steger-Warming + WENO5, 2 = WCNS (WENO5 midpoint interpolation + Roe) */
+void ora_inviscid_ext(const ora_ext *e, double *rhsi, const double *q, const double *dxidx,
+                      const double *inv_j, double gamma, int scheme);
+void ora_rk3_stage(const ora_ext *e, double *q, const double *qn, const double *rhs, const double *dt,
+                   const double *inv_j, int s);
+/* one Shu-Osher RK3 step; scheme 0 = the reference's WENO3 + Roe */
+int ora_step_rk3(int nblk, ora_block *blk, ora_config *c, int scheme);
+
 #ifdef __cplusplus
 }
 #endif

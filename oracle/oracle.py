@@ -254,3 +254,37 @@ def step(states, cfg):
     rc = lib().ora_step(C.c_int(len(states)), arr, C.byref(cfg))
     if rc != 0:
         raise RuntimeError("ora_step: unsupported time advance %d" % cfg.iflag_timeadvance)
+
+
+# ------------------------------------------- extensions (no reference form)
+def sw_split(qv, M, gamma):
+    q = np.ascontiguousarray(qv, dtype=np.float64)
+    m = np.ascontiguousarray(M, dtype=np.float64)
+    fp = np.zeros(5)
+    fm = np.zeros(5)
+    lib().ora_sw_split(ptr(q), ptr(m), C.c_double(gamma), ptr(fp), ptr(fm))
+    return fp, fm
+
+
+def weno5_rec(a, b, c, d, e):
+    f = lib().ora_weno5_rec
+    f.restype = C.c_double
+    return f(*[C.c_double(x) for x in (a, b, c, d, e)])
+
+
+def weno5_interp(a, b, c, d, e):
+    f = lib().ora_weno5_interp
+    f.restype = C.c_double
+    return f(*[C.c_double(x) for x in (a, b, c, d, e)])
+
+
+def inviscid_ext(ext, q, dxidx, inv_j, gamma, scheme):
+    rhsi = np.zeros_like(q)
+    lib().ora_inviscid_ext(C.byref(ext), ptr(rhsi), ptr(q), ptr(dxidx), ptr(inv_j), C.c_double(gamma),
+                           C.c_int(scheme))
+    return rhsi
+
+
+def step_rk3(states, cfg, scheme):
+    arr = blocks_array(states)
+    lib().ora_step_rk3(C.c_int(len(states)), arr, C.byref(cfg), C.c_int(scheme))

@@ -52,7 +52,7 @@ long ora_getpri(const ora_ext *e, const double *q, double *pri_v, double gamma, 
 /* WENO3 candidate pair, sub_inviscid_upwind.f90:32-41 (right) and :57-66
  * (left). For the right state u1,u2,u3 = f(i),f(i+1),f(i+2); for the left
  * state u1,u2,u3 = f(i-1),f(i),f(i+1). */
-static inline double weno3_right(double u1, double u2, double u3)
+double ora_weno3_right(double u1, double u2, double u3)
 {
     const double eps = 1.0e-6;
     double s1 = (u2 - u3) * (u2 - u3);
@@ -65,7 +65,7 @@ static inline double weno3_right(double u1, double u2, double u3)
     return w1 * (-u3 / 2.0 + 3.0 * u2 / 2.0) + w2 * (u2 / 2.0 + u1 / 2.0);
 }
 
-static inline double weno3_left(double u1, double u2, double u3)
+double ora_weno3_left(double u1, double u2, double u3)
 {
     const double eps = 1.0e-6;
     double s1 = (u2 - u1) * (u2 - u1);
@@ -101,7 +101,7 @@ void ora_weno3(const ora_ext *e, double *df, const double *f, int coord, int dir
                         const double *p1 = f + 5 * IX(i, j, k);
                         const double *p2 = f + 5 * IX(i + di, j + dj, k + dk);
                         const double *p3 = f + 5 * IX(i + 2 * di, j + 2 * dj, k + 2 * dk);
-                        for (int v = 0; v < 5; v++) o[v] = weno3_right(p1[v], p2[v], p3[v]);
+                        for (int v = 0; v < 5; v++) o[v] = ora_weno3_right(p1[v], p2[v], p3[v]);
                     }
                 } else {
                     if (a == ext_lo) {
@@ -111,14 +111,14 @@ void ora_weno3(const ora_ext *e, double *df, const double *f, int coord, int dir
                         const double *p1 = f + 5 * IX(i - di, j - dj, k - dk);
                         const double *p2 = f + 5 * IX(i, j, k);
                         const double *p3 = f + 5 * IX(i + di, j + dj, k + dk);
-                        for (int v = 0; v < 5; v++) o[v] = weno3_left(p1[v], p2[v], p3[v]);
+                        for (int v = 0; v < 5; v++) o[v] = ora_weno3_left(p1[v], p2[v], p3[v]);
                     }
                 }
             }
 }
 
 /* Roe flux at one face with face metric (mx,my,mz); sub_inviscid_splitting.f90:65-162 */
-static void roe_face(const double *qlf, const double *qrf, double mx, double my, double mz,
+void ora_roe_face(const double *qlf, const double *qrf, double mx, double my, double mz,
                      double gamma, double *out)
 {
     const double delt = 0.06;
@@ -195,7 +195,7 @@ void ora_roe(const ora_ext *e, const double *ql, const double *qr, double *flux,
                 double mx = 0.5 * (dxidx[9 * n0 + c0] * inv_j[n0] + dxidx[9 * n1 + c0] * inv_j[n1]);
                 double my = 0.5 * (dxidx[9 * n0 + c0 + 1] * inv_j[n0] + dxidx[9 * n1 + c0 + 1] * inv_j[n1]);
                 double mz = 0.5 * (dxidx[9 * n0 + c0 + 2] * inv_j[n0] + dxidx[9 * n1 + c0 + 2] * inv_j[n1]);
-                roe_face(ql + 5 * n0, qr + 5 * n0, mx, my, mz, gamma, flux + 5 * n0);
+                ora_roe_face(ql + 5 * n0, qr + 5 * n0, mx, my, mz, gamma, flux + 5 * n0);
             }
 }
 
