@@ -17,7 +17,9 @@ EXPORTS = (
     "acfd_upload", "acfd_download", "acfd_run_stage", "acfd_prepare", "acfd_step",
     "acfd_residual", "acfd_stats", "acfd_sync", "acfd_timing", "acfd_getpri",
     "acfd_inviscid_rhs", "acfd_viscous_rhs", "acfd_find_dt", "acfd_lusgs", "acfd_bdf2",
-    "acfd_sa_step", "acfd_exchange_coordinates",
+    "acfd_sa_step", "acfd_exchange_coordinates", "acfd_sensor_weights", "acfd_sensor_collect",
+    "acfd_sensor_train", "acfd_sensor_evaluate", "acfd_sensor_enable", "acfd_sensor_stats", "acfd_mlp_grad",
+    "acfd_sensor_infer",
 )
 
 
@@ -25,8 +27,9 @@ class Config(C.Structure):
     """acfd_config"""
     _fields_ = [(n, C.c_double) for n in ("ma", "re", "tinf", "aoa_deg", "cfl", "dt", "prl", "prt")] + [
         (n, C.c_int) for n in ("iflag_solver", "iflag_time", "iflag_turbulence", "iflag_des",
-                               "iflag_timeadvance", "precision", "lusgs_kernel", "inviscid_scheme")] + [
-        ("reserved", C.c_int * 4)]
+                               "iflag_timeadvance", "precision", "lusgs_kernel", "inviscid_scheme",
+                               "use_graph")] + [
+        ("reserved", C.c_int * 3)]
 
 
 class Subface(C.Structure):

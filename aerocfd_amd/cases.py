@@ -343,3 +343,51 @@ def tgv_box(n=256, cfg=None, blocks=((), (), ()), permute=None):
         q = tgv_state(xyz, cfg.ma)
         _common_fields(case, m, q, *uniform_metrics(case.ranges[m], (h, h, h), perm))
     return case
+
+
+def shock_vortex(ni=96, nj=48, nk=4, mach=1.5, seed=4, cfg=None):
+    """Planar normal shock at x = 1 (Rankine-Hugoniot jump from the Mach
+    `mach` freestream moving along +x) hit by an isentropic vortex, with
+    entropy waves rho' = 0.2 sin(5 x) upstream; x in [0, 2], uniform spacing,
+    extruded in k.  Faces: inflow (i-), outflow (i+), symmetry (j, k).  Euler,
+    unsteady, explicit RK3, WCNS.  The vortex position comes from `seed`."""
+    from .solver import FlowConfig
+    h = 2.0 / (ni - 1)
+    if cfg is None:
+        cfg = FlowConfig(ma=mach, re=1.0e6, iflag_solver=1, iflag_time=1, iflag_timeadvance=4,
+                         inviscid_scheme=2, dt=0.25 * h / (1.0 + 1.0 / mach))
+    subs = box_subfaces(ni, nj, nk, [-50, -40, -30, -30, -30, -30])
+    case = Case("shock_vortex_%dx%dx%d" % (ni, nj, nk), cfg, [(ni, nj, nk)], [subs])
+    rng = case.ranges[0]
+    xyz = node_coords(rng, (1, 1, 1), (h, h, h), (0.0, 0.0, 0.0))
+    x, y = xyz[0], xyz[1]
+    g = GAMMA
+    p_inf = 1.0 / (g * mach * mach)
+    t_inf = p_inf
+    r = np.random.default_rng(seed)
+    ly = (nj - 1) * h
+    xc, yc, rad, gam = 0.45 + 0.1 * r.random(), ly * (0.4 + 0.2 * r.random()), 0.12, 0.25
+    dx, dy = x - xc, y - yc
+    rr = (dx * dx + dy * dy) / (rad * rad)
+    ex = np.exp(0.5 * (1.0 - rr))
+    du = -gam * dy / rad * ex / (2 * math.pi)
+    dv = gam * dx / rad * ex / (2 * math.pi)
+    dT = -(g - 1.0) * gam * gam * ex * ex / (8.0 * g * math.pi * math.pi)
+    T = t_inf + dT
+    rho = (T / t_inf) ** (1.0 / (g - 1.0)) * (1.0 + 0.2 * np.sin(5.0 * x))
+    p = p_inf * (T / t_inf) ** (g / (g - 1.0))
+    u = 1.0 + du
+    v = dv
+    # post-shock state for x >= 1
+    m2 = mach * mach
+    r21 = (g + 1.0) * m2 / ((g - 1.0) * m2 + 2.0)
+    p21 = 1.0 + 2.0 * g / (g + 1.0) * (m2 - 1.0)
+    down = x >= 1.0
+    rho = np.where(down, r21, rho)
+    u = np.where(down, 1.0 / r21, u)
+    v = np.where(down, 0.0, v)
+    p = np.where(down, p_inf * p21, p)
+    q = conservative(rho, u, v, np.zeros_like(rho), p)
+    dxidx, inv_j, al, be, ga = uniform_metrics(rng, (h, h, h))
+    _common_fields(case, 0, q, dxidx, inv_j, al, be, ga)
+    return case

@@ -16,6 +16,7 @@
 #include "../../include/aerocfd_amd.h"
 #include "common.hpp"
 #include "kernels.hpp"
+#include "sensor.hpp"
 
 using namespace acfd;
 
@@ -46,6 +47,7 @@ int fail(int code, const std::string &msg)
 const int kNcomp[22] = {0, 5, 7, 5, 5, 5, 9, 6, 1, 3, 1, 9, 1, 3, 3, 3, 1, 1, 1, 1, 10, 1};
 
 constexpr int kTimeSlots = 20;  // acfd_timing entries
+constexpr int kSensorParams = 1505, kSensorFeatures = 12;  // the sensor network's size
 
 struct Block {
     Ext e;
@@ -127,6 +129,27 @@ struct acfd_ctx {
     std::vector<hipEvent_t> ev_pool;
     std::vector<int> ev_stage;   // stage of each recorded pair
     size_t ev_used = 0;          // pairs in use
+    // learned WCNS sensor: FP32 parameters and Adam moments, the BF16
+    // inference copy, training samples and statistics
+    float *s_w = nullptr, *s_m = nullptr, *s_v = nullptr, *s_grad = nullptr, *s_loss = nullptr;
+    float *s_part = nullptr, *s_lpart = nullptr;
+    long s_part_cap = 0;
+    int s_adam_t = 0;
+    SensorNet *s_net = nullptr;
+    float *s_X = nullptr, *s_Y = nullptr;
+    long s_n = 0, s_cap = 0;
+    unsigned long long *s_stats = nullptr;   // faces evaluated, faces linear
+    bool s_on = false;
+    float s_thr = 0.5f;
+    int s_audit = 0;             // audit every s_audit inviscid evaluations of a block (0: never)
+    double s_bound = 1e-6;       // largest relative deviation an audit accepts
+    std::vector<long> s_calls;   // inviscid evaluations per block
+    std::vector<char> s_off;     // blocks that fell back to plain WCNS
+    int s_audits = 0;
+    double s_audit_max = 0.0;
+    hipGraphExec_t graph_exec = nullptr;  // one captured step (use_graph)
+    double *s_dpart = nullptr;
+    long s_dpart_n = 0;
 };
 
 namespace {
@@ -429,9 +452,51 @@ int st_getpri(acfd_ctx *ctx, int m)
     return 0;
 }
 
+// WCNS with the learned sensor on block m; every s_audit evaluations the
+// plain scheme runs too and a relative deviation above s_bound makes the
+// block fall back to plain WCNS for good.
+int inviscid_with_sensor(acfd_ctx *ctx, int m)
+{
+    Block &b = ctx->blk[m];
+    SensorArgs sa{ctx->s_net, ctx->s_thr, ctx->s_stats};
+    const long call = ctx->s_calls[m]++;
+    if (ctx->s_audit > 0 && call % ctx->s_audit == 0) {
+        KRN_TRY(launch_inviscid_ext(ctx->stream, b.e, b.f, ctx->c, ctx->cfg.precision, 2));
+        KRN_TRY(launch_copy(ctx->stream, b.f.rhsi, b.f.w_flux, 5 * b.e.n));
+        KRN_TRY(launch_inviscid_wcns_sensor(ctx->stream, b.e, b.f, ctx->c, ctx->cfg.precision, sa));
+        const long np = diff_partials(b.e);
+        if (ctx->s_dpart_n < 2 * np) {
+            if (ctx->s_dpart) (void)hipFree(ctx->s_dpart);
+            ctx->s_dpart = nullptr;
+            HIP_TRY(hipMalloc((void **)&ctx->s_dpart, 2 * np * sizeof(double)));
+            ctx->s_dpart_n = 2 * np;
+        }
+        KRN_TRY(launch_diff_partial(ctx->stream, b.e, b.f.rhsi, b.f.w_flux, ctx->s_dpart));
+        std::vector<double> h(2 * np);
+        HIP_TRY(hipMemcpyAsync(h.data(), ctx->s_dpart, 2 * np * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        double d2 = 0.0, r2 = 0.0;
+        for (long x = 0; x < np; x++) {
+            d2 += h[2 * x];
+            r2 += h[2 * x + 1];
+        }
+        const double dev = r2 > 0.0 ? std::sqrt(d2 / r2) : std::sqrt(d2);
+        ctx->s_audits++;
+        ctx->s_audit_max = std::max(ctx->s_audit_max, dev);
+        if (!(dev <= ctx->s_bound)) {
+            ctx->s_off[m] = 1;
+            KRN_TRY(launch_copy(ctx->stream, b.f.w_flux, b.f.rhsi, 5 * b.e.n));
+        }
+        return 0;
+    }
+    KRN_TRY(launch_inviscid_wcns_sensor(ctx->stream, b.e, b.f, ctx->c, ctx->cfg.precision, sa));
+    return 0;
+}
+
 int st_inviscid(acfd_ctx *ctx, int m)
 {
     StageTimer t(ctx, ACFD_STAGE_INVISCID);
+    if (ctx->s_on && ctx->cfg.inviscid_scheme == 2 && !ctx->s_off[m]) return inviscid_with_sensor(ctx, m);
     if (ctx->cfg.inviscid_scheme == 0)
         KRN_TRY(launch_inviscid(ctx->stream, ctx->blk[m].e, ctx->blk[m].f, ctx->c, ctx->cfg.precision));
     else
@@ -652,6 +717,37 @@ int one_step(acfd_ctx *ctx)
     return rc;
 }
 
+// Record one step into a graph (every launch of one_step on the context's
+// stream, no host synchronisation inside) and instantiate it.
+int capture_step(acfd_ctx *ctx)
+{
+    const bool timing = ctx->timing;
+    ctx->timing = false;  // no event records inside the graph
+    HIP_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    const int rc = one_step(ctx);
+    hipGraph_t g = nullptr;
+    const hipError_t ec = hipStreamEndCapture(ctx->stream, &g);
+    ctx->timing = timing;
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (ec != hipSuccess || !g) return fail(ACFD_EDEVICE, "acfd_step: stream capture failed");
+    const hipError_t ei = hipGraphInstantiate(&ctx->graph_exec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ei != hipSuccess) {
+        ctx->graph_exec = nullptr;
+        return fail(ACFD_EDEVICE, "acfd_step: graph instantiation failed");
+    }
+    return 0;
+}
+
+void drop_graph(acfd_ctx *ctx)
+{
+    if (ctx->graph_exec) (void)hipGraphExecDestroy(ctx->graph_exec);
+    ctx->graph_exec = nullptr;
+}
+
 }  // namespace
 
 // ===================================================================== C ABI
@@ -719,10 +815,15 @@ int acfd_destroy(acfd_ctx *ctx)
 {
     if (!ctx) return 0;
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->graph_exec) (void)hipGraphExecDestroy(ctx->graph_exec);
     for (auto &b : ctx->blk) free_block(b);
     if (ctx->lu.ex) uc_release(ctx->lu.ex);
     if (ctx->lu.sl) (void)hipFree(ctx->lu.sl);
     if (ctx->lu.err) (void)hipFree(ctx->lu.err);
+    void *sp[] = {ctx->s_w, ctx->s_m, ctx->s_v, ctx->s_grad, ctx->s_loss, ctx->s_part, ctx->s_lpart, ctx->s_net,
+                  ctx->s_X, ctx->s_Y, ctx->s_stats, ctx->s_dpart};
+    for (void *p : sp)
+        if (p) (void)hipFree(p);
     void *ptrs[] = {ctx->d_cfl, ctx->d_stats, ctx->d_nneg, ctx->d_res, ctx->d_partial,
                     ctx->d_stage, ctx->d_qtab, ctx->d_nuttab, ctx->d_ntab, ctx->d_h_sb, ctx->d_h_db,
                     ctx->d_h_sn, ctx->d_h_dn, ctx->d_a_sb, ctx->d_a_gb, ctx->d_a_gs, ctx->d_a_sn,
@@ -998,7 +1099,25 @@ int acfd_prepare(acfd_ctx *ctx)
 int acfd_step(acfd_ctx *ctx, int nsteps)
 {
     if (!ctx || !ctx->topo) return fail(ACFD_ESTATE, "acfd_step: no topology");
-    for (int s = 0; s < nsteps; s++) {
+    int s = 0;
+    // graph mode: the first step runs eagerly (it also tunes the tile
+    // sweeps), the next one is captured, and every later step replays it;
+    // audited sensor steps read results on the host and stay eager
+    const bool graph = ctx->cfg.use_graph && !(ctx->s_on && ctx->s_audit > 0);
+    if (graph && !ctx->graph_exec && nsteps > 0) {
+        int rc = one_step(ctx);
+        if (rc) return rc;
+        s = 1;
+        if (s < nsteps) {
+            rc = capture_step(ctx);
+            if (rc) return rc;
+        }
+    }
+    for (; s < nsteps; s++) {
+        if (graph && ctx->graph_exec) {
+            HIP_TRY(hipGraphLaunch(ctx->graph_exec, ctx->stream));
+            continue;
+        }
         int rc = one_step(ctx);
         if (rc) return rc;
     }
@@ -1085,6 +1204,265 @@ int acfd_exchange_coordinates(acfd_ctx *ctx, double **xyz, const double *shift)
         }
     }
     return 0;
+}
+
+
+// ------------------------------------------------------- learned sensor
+}  // extern "C"
+namespace {
+
+int sensor_ready(acfd_ctx *ctx)
+{
+    if (ctx->s_w) return 0;
+    const size_t pb = kSensorParams * sizeof(float);
+    HIP_TRY(hipMalloc((void **)&ctx->s_w, pb));
+    HIP_TRY(hipMalloc((void **)&ctx->s_m, pb));
+    HIP_TRY(hipMalloc((void **)&ctx->s_v, pb));
+    HIP_TRY(hipMalloc((void **)&ctx->s_grad, pb));
+    HIP_TRY(hipMalloc((void **)&ctx->s_loss, sizeof(float)));
+    HIP_TRY(hipMalloc((void **)&ctx->s_net, sensor_net_bytes()));
+    HIP_TRY(hipMalloc((void **)&ctx->s_stats, 2 * sizeof(unsigned long long)));
+    HIP_TRY(hipMemset(ctx->s_stats, 0, 2 * sizeof(unsigned long long)));
+    HIP_TRY(hipMemset(ctx->s_m, 0, pb));
+    HIP_TRY(hipMemset(ctx->s_v, 0, pb));
+    // He initialisation from a fixed generator (64-bit LCG + Box-Muller)
+    std::vector<float> w(kSensorParams, 0.f);
+    unsigned long long st = 0x9E3779B97F4A7C15ull;
+    auto uni = [&]() {
+        st = st * 6364136223846793005ull + 1442695040888963407ull;
+        return ((st >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+    };
+    auto gauss = [&]() { return std::sqrt(-2.0 * std::log(uni())) * std::cos(6.283185307179586 * uni()); };
+    const int nf = 12, h = 32;
+    for (int e = 0; e < nf * h; e++) w[e] = (float)(gauss() * std::sqrt(2.0 / nf));
+    for (int e = 0; e < h * h; e++) w[nf * h + h + e] = (float)(gauss() * std::sqrt(2.0 / h));
+    for (int e = 0; e < h; e++) w[nf * h + h + h * h + h + e] = (float)(gauss() * std::sqrt(1.0 / h));
+    HIP_TRY(hipMemcpy(ctx->s_w, w.data(), pb, hipMemcpyHostToDevice));
+    KRN_TRY(launch_sensor_pack(ctx->stream, ctx->s_w, ctx->s_net));
+    ctx->s_calls.assign(ctx->blk.size(), 0);
+    ctx->s_off.assign(ctx->blk.size(), 0);
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+int ensure_mlp_partials(acfd_ctx *ctx, long count)
+{
+    const long need = mlp_partials_needed(count);
+    if (ctx->s_part_cap >= need) return 0;
+    if (ctx->s_part) (void)hipFree(ctx->s_part);
+    if (ctx->s_lpart) (void)hipFree(ctx->s_lpart);
+    ctx->s_part = ctx->s_lpart = nullptr;
+    HIP_TRY(hipMalloc((void **)&ctx->s_part, need * sizeof(float)));
+    HIP_TRY(hipMalloc((void **)&ctx->s_lpart, (need / kSensorParams + 1) * sizeof(float)));
+    ctx->s_part_cap = need;
+    return 0;
+}
+
+}  // namespace
+extern "C" {
+
+int acfd_sensor_weights(acfd_ctx *ctx, float *w, int set)
+{
+    if (!ctx || !ctx->topo || !w) return fail(ACFD_EINVAL, "acfd_sensor_weights: bad arguments");
+    int rc = sensor_ready(ctx);
+    if (rc) return rc;
+    const size_t pb = kSensorParams * sizeof(float);
+    if (set) {
+        HIP_TRY(hipMemcpy(ctx->s_w, w, pb, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemset(ctx->s_m, 0, pb));
+        HIP_TRY(hipMemset(ctx->s_v, 0, pb));
+        ctx->s_adam_t = 0;
+        KRN_TRY(launch_sensor_pack(ctx->stream, ctx->s_w, ctx->s_net));
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+    } else {
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        HIP_TRY(hipMemcpy(w, ctx->s_w, pb, hipMemcpyDeviceToHost));
+    }
+    return 0;
+}
+
+int acfd_sensor_collect(acfd_ctx *ctx, int stride, long *count)
+{
+    if (!ctx || !ctx->topo || stride < 1) return fail(ACFD_EINVAL, "acfd_sensor_collect: bad arguments");
+    int rc = sensor_ready(ctx);
+    if (rc) return rc;
+    long total = 0;
+    for (auto &b : ctx->blk)
+        for (int ax = 0; ax < 3; ax++) total += sensor_samples(b.e, ax, stride);
+    if (total > ctx->s_cap) {
+        if (ctx->s_X) (void)hipFree(ctx->s_X);
+        if (ctx->s_Y) (void)hipFree(ctx->s_Y);
+        ctx->s_X = ctx->s_Y = nullptr;
+        HIP_TRY(hipMalloc((void **)&ctx->s_X, (size_t)std::max(total, 1L) * kSensorFeatures * sizeof(float)));
+        HIP_TRY(hipMalloc((void **)&ctx->s_Y, (size_t)std::max(total, 1L) * sizeof(float)));
+        ctx->s_cap = total;
+    }
+    long off = 0;
+    for (auto &b : ctx->blk)
+        for (int ax = 0; ax < 3; ax++) {
+            KRN_TRY(launch_sensor_collect(ctx->stream, b.e, b.f, ctx->c.gamma, ax, stride,
+                                          ctx->s_X + off * kSensorFeatures, ctx->s_Y + off));
+            off += sensor_samples(b.e, ax, stride);
+        }
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    ctx->s_n = total;
+    if (count) *count = total;
+    return 0;
+}
+
+int acfd_sensor_train(acfd_ctx *ctx, int iters, int batch, float lr, unsigned seed, float *losses)
+{
+    if (!ctx || !ctx->topo || iters < 0 || batch < 1) return fail(ACFD_EINVAL, "acfd_sensor_train: bad arguments");
+    int rc = sensor_ready(ctx);
+    if (rc) return rc;
+    const long n = ctx->s_n;
+    if (n <= 0) return fail(ACFD_ESTATE, "acfd_sensor_train: no samples (call acfd_sensor_collect)");
+    const long count = std::min<long>(batch, n);
+    rc = ensure_mlp_partials(ctx, count);
+    if (rc) return rc;
+    // a fixed pseudo-random visiting order: k -> (k * stride + offset) mod n
+    const long primes[] = {1000003, 999983, 999979, 999961, 999959, 999953};
+    long stride = 1;
+    for (long p : primes)
+        if (n % p != 0) {
+            stride = p % n == 0 ? 1 : p;
+            break;
+        }
+    const long offset = (long)(seed % (unsigned long)n);
+    float *hist = nullptr;
+    HIP_TRY(hipMalloc((void **)&hist, std::max(iters, 1) * sizeof(float)));
+    for (int it = 0; it < iters; it++) {
+        KRN_TRY(launch_mlp_grad(ctx->stream, ctx->s_w, ctx->s_X, ctx->s_Y, n, (long)it * count, count, stride, offset,
+                                ctx->s_part, ctx->s_lpart, ctx->s_grad, hist + it));
+        KRN_TRY(launch_adam(ctx->stream, ctx->s_w, ctx->s_grad, ctx->s_m, ctx->s_v, lr, ++ctx->s_adam_t));
+    }
+    KRN_TRY(launch_sensor_pack(ctx->stream, ctx->s_w, ctx->s_net));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (losses && iters > 0) HIP_TRY(hipMemcpy(losses, hist, iters * sizeof(float), hipMemcpyDeviceToHost));
+    (void)hipFree(hist);
+    return 0;
+}
+
+int acfd_sensor_evaluate(acfd_ctx *ctx, float threshold, double *out)
+{
+    if (!ctx || !ctx->topo || !out) return fail(ACFD_EINVAL, "acfd_sensor_evaluate: bad arguments");
+    int rc = sensor_ready(ctx);
+    if (rc) return rc;
+    const long n = ctx->s_n;
+    if (n <= 0) return fail(ACFD_ESTATE, "acfd_sensor_evaluate: no samples");
+    float *prob = nullptr;
+    HIP_TRY(hipMalloc((void **)&prob, n * sizeof(float)));
+    KRN_TRY(launch_sensor_infer(ctx->stream, ctx->s_net, ctx->s_X, n, prob));
+    std::vector<float> p(n), y(n);
+    HIP_TRY(hipMemcpyAsync(p.data(), prob, n * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(y.data(), ctx->s_Y, n * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    (void)hipFree(prob);
+    long correct = 0, lin = 0, false_lin = 0, pos = 0;
+    for (long k = 0; k < n; k++) {
+        const bool need = y[k] > 0.5f, pred = p[k] >= threshold;
+        correct += need == pred;
+        lin += !pred;
+        pos += need;
+        false_lin += need && !pred;
+    }
+    out[0] = (double)correct / n;            // accuracy
+    out[1] = (double)lin / n;                // faces given linear weights
+    out[2] = (double)false_lin / n;          // nonlinear faces given linear weights
+    out[3] = (double)pos / n;                // faces whose label is nonlinear
+    return 0;
+}
+
+int acfd_sensor_enable(acfd_ctx *ctx, int on, float threshold, int audit_interval, double audit_bound)
+{
+    if (!ctx || !ctx->topo) return fail(ACFD_EINVAL, "acfd_sensor_enable: no topology");
+    if (on && ctx->cfg.inviscid_scheme != 2)
+        return fail(ACFD_EINVAL, "acfd_sensor_enable: the sensor drives the WCNS scheme (inviscid_scheme = 2)");
+    int rc = sensor_ready(ctx);
+    if (rc) return rc;
+    drop_graph(ctx);  // the captured step used the other inviscid kernels
+    ctx->s_on = on != 0;
+    ctx->s_thr = threshold;
+    ctx->s_audit = std::max(audit_interval, 0);
+    ctx->s_bound = audit_bound;
+    ctx->s_calls.assign(ctx->blk.size(), 0);
+    ctx->s_off.assign(ctx->blk.size(), 0);
+    ctx->s_audits = 0;
+    ctx->s_audit_max = 0.0;
+    HIP_TRY(hipMemset(ctx->s_stats, 0, 2 * sizeof(unsigned long long)));
+    return 0;
+}
+
+int acfd_sensor_stats(acfd_ctx *ctx, double *out)
+{
+    if (!ctx || !out) return fail(ACFD_EINVAL, "acfd_sensor_stats: bad arguments");
+    unsigned long long st[2] = {0, 0};
+    if (ctx->s_stats) {
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        HIP_TRY(hipMemcpy(st, ctx->s_stats, sizeof(st), hipMemcpyDeviceToHost));
+    }
+    long off = 0;
+    for (char c : ctx->s_off) off += c;
+    out[0] = (double)st[0];
+    out[1] = (double)st[1];
+    out[2] = ctx->s_audits;
+    out[3] = ctx->s_audit_max;
+    out[4] = (double)off;
+    out[5] = 0.0;
+    return 0;
+}
+
+int acfd_mlp_grad(const float *w, const float *X, const float *Y, long n, float *loss, float *grad)
+{
+    if (!w || !X || !Y || n <= 0 || !loss || !grad) return fail(ACFD_EINVAL, "acfd_mlp_grad: bad arguments");
+    float *dw = nullptr, *dx = nullptr, *dy = nullptr, *dg = nullptr, *dl = nullptr, *dp = nullptr, *dlp = nullptr;
+    const long np = mlp_partials_needed(n);
+    int rc = 0;
+    if (hipMalloc((void **)&dw, kSensorParams * sizeof(float)) != hipSuccess ||
+        hipMalloc((void **)&dx, n * kSensorFeatures * sizeof(float)) != hipSuccess ||
+        hipMalloc((void **)&dy, n * sizeof(float)) != hipSuccess ||
+        hipMalloc((void **)&dg, kSensorParams * sizeof(float)) != hipSuccess ||
+        hipMalloc((void **)&dl, sizeof(float)) != hipSuccess ||
+        hipMalloc((void **)&dp, np * sizeof(float)) != hipSuccess ||
+        hipMalloc((void **)&dlp, (np / kSensorParams + 1) * sizeof(float)) != hipSuccess)
+        rc = fail(ACFD_EDEVICE, "acfd_mlp_grad: device memory");
+    if (!rc && (hipMemcpy(dw, w, kSensorParams * sizeof(float), hipMemcpyHostToDevice) != hipSuccess ||
+                hipMemcpy(dx, X, n * kSensorFeatures * sizeof(float), hipMemcpyHostToDevice) != hipSuccess ||
+                hipMemcpy(dy, Y, n * sizeof(float), hipMemcpyHostToDevice) != hipSuccess))
+        rc = fail(ACFD_EDEVICE, "acfd_mlp_grad: copy");
+    if (!rc && launch_mlp_grad(nullptr, dw, dx, dy, n, 0, n, 1, 0, dp, dlp, dg, dl))
+        rc = fail(ACFD_EDEVICE, "acfd_mlp_grad: launch");
+    if (!rc && (hipMemcpy(grad, dg, kSensorParams * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess ||
+                hipMemcpy(loss, dl, sizeof(float), hipMemcpyDeviceToHost) != hipSuccess))
+        rc = fail(ACFD_EDEVICE, "acfd_mlp_grad: copy back");
+    void *all[] = {dw, dx, dy, dg, dl, dp, dlp};
+    for (void *p : all)
+        if (p) (void)hipFree(p);
+    return rc;
+}
+
+int acfd_sensor_infer(const float *w, const float *X, long n, float *prob)
+{
+    if (!w || !X || n <= 0 || !prob) return fail(ACFD_EINVAL, "acfd_sensor_infer: bad arguments");
+    float *dw = nullptr, *dx = nullptr, *dp = nullptr;
+    SensorNet *net = nullptr;
+    int rc = 0;
+    if (hipMalloc((void **)&dw, kSensorParams * sizeof(float)) != hipSuccess ||
+        hipMalloc((void **)&dx, n * kSensorFeatures * sizeof(float)) != hipSuccess ||
+        hipMalloc((void **)&dp, n * sizeof(float)) != hipSuccess ||
+        hipMalloc((void **)&net, sensor_net_bytes()) != hipSuccess)
+        rc = fail(ACFD_EDEVICE, "acfd_sensor_infer: device memory");
+    if (!rc && (hipMemcpy(dw, w, kSensorParams * sizeof(float), hipMemcpyHostToDevice) != hipSuccess ||
+                hipMemcpy(dx, X, n * kSensorFeatures * sizeof(float), hipMemcpyHostToDevice) != hipSuccess))
+        rc = fail(ACFD_EDEVICE, "acfd_sensor_infer: copy");
+    if (!rc && (launch_sensor_pack(nullptr, dw, net) || launch_sensor_infer(nullptr, net, dx, n, dp)))
+        rc = fail(ACFD_EDEVICE, "acfd_sensor_infer: launch");
+    if (!rc && hipMemcpy(prob, dp, n * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess)
+        rc = fail(ACFD_EDEVICE, "acfd_sensor_infer: copy back");
+    void *all[] = {dw, dx, dp, net};
+    for (void *p : all)
+        if (p) (void)hipFree(p);
+    return rc;
 }
 
 }  // extern "C"

@@ -12,9 +12,12 @@
 // order (WENO3 splitting / the reference's WENO3 states) and then first
 // order, exactly as the oracle does.  Everything is templated on the flux
 // arithmetic R (double, or float for the FP32-flux variant).
+#include <type_traits>
+
 #include "common.hpp"
 #include "flux_common.hpp"
 #include "kernels.hpp"
+#include "sensor.hpp"
 
 namespace acfd {
 
@@ -141,7 +144,7 @@ __device__ inline void node_quantity(const double *__restrict__ q, const double 
 // face flux at a+1/2 from node quantities W[0..5] at a-2..a+3 (order ord);
 // scheme 2 also needs the face metric m
 template <typename R, int SCHEME, int NQ>
-__device__ inline void ext_face(const R (*W)[NQ], int ord, const R *m, R gamma, R *F)
+__device__ inline void ext_face(const R (*W)[NQ], int ord, const R *m, R gamma, R *F, bool lin = false)
 {
     if (SCHEME == 1) {
 #pragma unroll
@@ -163,7 +166,13 @@ __device__ inline void ext_face(const R (*W)[NQ], int ord, const R *m, R gamma, 
         R ul[5], ur[5];
 #pragma unroll
         for (int x = 0; x < 5; x++) {
-            if (ord == 5) {
+            if (ord == 5 && lin) {
+                // linear weights: the fifth-order midpoint interpolation
+                ul[x] = ((R)3.0 * W[0][x] - (R)20.0 * W[1][x] + (R)90.0 * W[2][x] + (R)60.0 * W[3][x] -
+                         (R)5.0 * W[4][x]) / (R)128.0;
+                ur[x] = ((R)3.0 * W[5][x] - (R)20.0 * W[4][x] + (R)90.0 * W[3][x] + (R)60.0 * W[2][x] -
+                         (R)5.0 * W[1][x]) / (R)128.0;
+            } else if (ord == 5) {
                 ul[x] = weno5_interp<R>(W[0][x], W[1][x], W[2][x], W[3][x], W[4][x]);
                 ur[x] = weno5_interp<R>(W[5][x], W[4][x], W[3][x], W[2][x], W[1][x]);
             } else if (ord == 3) {
@@ -207,20 +216,32 @@ __device__ inline double ext_diff(const R (*FR)[5], int x, int dord)
 // j and k directions: each thread marches along the axis over kExtRun nodes
 // with rolling windows of node quantities and face fluxes.
 constexpr int kExtRun = 32;
-template <typename R, int AXIS, int SCHEME>
+template <typename R, int AXIS, int SCHEME, bool SENS>
 __global__ void __launch_bounds__(256) k_ext_jk(Ext e, const double *__restrict__ q,
                                                const double *__restrict__ dxidx,
                                                const double *__restrict__ inv_j, double *__restrict__ rhsi,
-                                               double gamma_d)
+                                               double gamma_d, SensorArgs sa)
 {
     constexpr int NQ = SCHEME == 1 ? 10 : 5;
-    const int i = e.is1 + (int)(blockIdx.x * blockDim.x + threadIdx.x);
-    const int o = (int)(blockIdx.y * blockDim.y + threadIdx.y);
+    const int ir = e.is1 + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    const int orr = (int)(blockIdx.y * blockDim.y + threadIdx.y);
     const int a0 = e.lo1(AXIS) + (int)blockIdx.z * kExtRun;
     const int a1 = min(a0 + kExtRun - 1, e.hi1(AXIS));
+    const bool active = ir <= e.ie1 && (AXIS == 1 ? e.ks1 + orr <= e.ke1 : e.js1 + orr <= e.je1);
+    // with the sensor every lane takes part in the wavefront-wide network;
+    // lanes outside the range work on a valid line and store nothing
+    __shared__ typename std::conditional<SENS, SensorLds, char>::type slds;
+    if constexpr (SENS) {
+        sensor_load_net(sa.net, slds);
+        __syncthreads();
+    } else {
+        if (!active) return;
+    }
+    const int i = min(ir, e.ie1);
+    const int o = min(orr, AXIS == 1 ? e.ke1 - e.ks1 : e.je1 - e.js1);
     const int j = AXIS == 1 ? e.lo(1) : e.js1 + o;
     const int k = AXIS == 2 ? e.lo(2) : e.ks1 + o;
-    if (i > e.ie1 || (AXIS == 1 ? k > e.ke1 : j > e.je1)) return;
+    unsigned long long n_eval = 0, n_lin = 0;
     const R gamma = (R)gamma_d;
     const long N = e.n, st = e.stride(AXIS);
     const int lo = e.lo(AXIS), hi = e.hi(AXIS);
@@ -256,18 +277,36 @@ __global__ void __launch_bounds__(256) k_ext_jk(Ext e, const double *__restrict_
         for (int f = 0; f < 5; f++)
 #pragma unroll
             for (int x = 0; x < 5; x++) FR[f][x] = FR[f + 1][x];
+        const int ord = face_order(b, lo, hi);
+        bool lin = false;
+        if constexpr (SENS) {
+            float feat[kSensorNF];
+            sensor_features<R>(W, gamma_d, feat);
+            const float p = sensor_eval_wave(slds, threadIdx.y, feat);
+            lin = p < sa.threshold;
+            if (active && ord == 5) {
+                n_eval++;
+                n_lin += lin ? 1 : 0;
+            }
+        }
         if (b >= lo && b + 1 <= hi) {
             R m[3] = {(R)0, (R)0, (R)0};
             if (SCHEME == 2) face_metric_r<R>(dxidx, inv_j, N, nat(b), st, c0, m);
-            ext_face<R, SCHEME, NQ>(W, face_order(b, lo, hi), m, gamma, FR[5]);
+            ext_face<R, SCHEME, NQ>(W, ord, m, gamma, FR[5], lin);
         }
         const int a = SCHEME == 1 ? b : b - 2;
-        if (a >= a0 && a <= a1) {
+        if (active && a >= a0 && a <= a1) {
             const int dord = SCHEME == 2 ? wcns_order(a, lo, hi) : 2;
             const long n = nat(a);
 #pragma unroll
             for (int x = 0; x < 5; x++)
                 rhsi[x * N + n] = rhsi[x * N + n] - ext_diff<R, SCHEME, SCHEME == 1 ? 5 : 3>(FR, x, dord);
+        }
+    }
+    if constexpr (SENS) {
+        if (n_eval) {
+            atomicAdd(&sa.stats[0], n_eval);
+            atomicAdd(&sa.stats[1], n_lin);
         }
     }
 }
@@ -280,15 +319,17 @@ template <int SCHEME> struct ExtI {
     static constexpr int NPW = SCHEME == 1 ? 58 : 54;
 };
 
-template <typename R, int SCHEME>
+template <typename R, int SCHEME, bool SENS>
 __global__ void __launch_bounds__(256) k_ext_i(Ext e, const double *__restrict__ q,
                                               const double *__restrict__ dxidx,
                                               const double *__restrict__ inv_j, double *__restrict__ rhsi,
-                                              double gamma_d)
+                                              double gamma_d, SensorArgs sa)
 {
     constexpr int NQ = SCHEME == 1 ? 10 : 5;
     __shared__ R nq[4][NQ][64];
     __shared__ R fq[4][5][64];
+    __shared__ typename std::conditional<SENS, SensorLds, char>::type slds;
+    if constexpr (SENS) sensor_load_net(sa.net, slds);
     const int lane = threadIdx.x, row = threadIdx.y;
     const int jr = e.js1 + (int)(blockIdx.y * blockDim.y + threadIdx.y);
     const bool row_ok = jr <= e.je1;
@@ -310,15 +351,30 @@ __global__ void __launch_bounds__(256) k_ext_i(Ext e, const double *__restrict__
     __syncthreads();
     {
         R F[5] = {(R)0, (R)0, (R)0, (R)0, (R)0};
-        if (lane >= 2 && lane <= 60 && p >= lo && p + 1 <= hi) {
-            R W[6][NQ];
+        const bool face_ok = lane >= 2 && lane <= 60 && p >= lo && p + 1 <= hi;
+        R W[6][NQ];
 #pragma unroll
-            for (int w = 0; w < 6; w++)
+        for (int w = 0; w < 6; w++)
 #pragma unroll
-                for (int x = 0; x < NQ; x++) W[w][x] = nq[row][x][lane - 2 + w];
+            for (int x = 0; x < NQ; x++) W[w][x] = nq[row][x][min(max(lane - 2 + w, 0), 63)];
+        const int ord = face_order(p, lo, hi);
+        bool lin = false;
+        if constexpr (SENS) {
+            float feat[kSensorNF];
+            sensor_features<R>(W, gamma_d, feat);
+            const float pr = sensor_eval_wave(slds, row, feat);
+            lin = pr < sa.threshold;
+            const bool counted = face_ok && ord == 5 && row_ok && p >= e.is1 - 1 && p <= e.ie1;
+            const unsigned long long ne = __popcll(__ballot(counted)), nl = __popcll(__ballot(counted && lin));
+            if (lane == 0 && ne) {
+                atomicAdd(&sa.stats[0], ne);
+                atomicAdd(&sa.stats[1], nl);
+            }
+        }
+        if (face_ok) {
             R m[3] = {(R)0, (R)0, (R)0};
             if (SCHEME == 2) face_metric_r<R>(dxidx, inv_j, N, n, 1, 0, m);
-            ext_face<R, SCHEME, NQ>(W, face_order(p, lo, hi), m, gamma, F);
+            ext_face<R, SCHEME, NQ>(W, ord, m, gamma, F, lin);
         }
 #pragma unroll
         for (int x = 0; x < 5; x++) fq[row][x][lane] = F[x];
@@ -340,17 +396,29 @@ __global__ void __launch_bounds__(256) k_ext_i(Ext e, const double *__restrict__
     }
 }
 
-template <typename R, int SCHEME>
-static void launch_ext_t(hipStream_t s, const Ext &e, const Fields &f, double gamma)
+template <typename R, int SCHEME, bool SENS = false>
+static void launch_ext_t(hipStream_t s, const Ext &e, const Fields &f, double gamma, SensorArgs sa = {})
 {
     const int Li = e.ie1 - e.is1 + 1, Lj = e.je1 - e.js1 + 1, Lk = e.ke1 - e.ks1 + 1;
     dim3 t(64, 4, 1);
     dim3 gi((Li + ExtI<SCHEME>::NPW - 1) / ExtI<SCHEME>::NPW, (Lj + 3) / 4, Lk);
-    hipLaunchKernelGGL((k_ext_i<R, SCHEME>), gi, t, 0, s, e, f.q, f.dxidx, f.inv_j, f.rhsi, gamma);
+    hipLaunchKernelGGL((k_ext_i<R, SCHEME, SENS>), gi, t, 0, s, e, f.q, f.dxidx, f.inv_j, f.rhsi, gamma, sa);
     dim3 gj((Li + 63) / 64, (Lk + 3) / 4, (Lj + kExtRun - 1) / kExtRun);
-    hipLaunchKernelGGL((k_ext_jk<R, 1, SCHEME>), gj, t, 0, s, e, f.q, f.dxidx, f.inv_j, f.rhsi, gamma);
+    hipLaunchKernelGGL((k_ext_jk<R, 1, SCHEME, SENS>), gj, t, 0, s, e, f.q, f.dxidx, f.inv_j, f.rhsi, gamma, sa);
     dim3 gk((Li + 63) / 64, (Lj + 3) / 4, (Lk + kExtRun - 1) / kExtRun);
-    hipLaunchKernelGGL((k_ext_jk<R, 2, SCHEME>), gk, t, 0, s, e, f.q, f.dxidx, f.inv_j, f.rhsi, gamma);
+    hipLaunchKernelGGL((k_ext_jk<R, 2, SCHEME, SENS>), gk, t, 0, s, e, f.q, f.dxidx, f.inv_j, f.rhsi, gamma, sa);
+}
+
+int launch_inviscid_wcns_sensor(hipStream_t s, const Ext &e, const Fields &f, const Consts &c, int precision,
+                                const SensorArgs &sa)
+{
+    int rc = launch_fill(s, f.rhsi, 5 * e.n, 0.0);
+    if (rc) return rc;
+    if (e.ie1 < e.is1 || e.je1 < e.js1 || e.ke1 < e.ks1) return 0;
+    if (precision == 1) launch_ext_t<float, 2, true>(s, e, f, c.gamma, sa);
+    else launch_ext_t<double, 2, true>(s, e, f, c.gamma, sa);
+    ACFD_CHECK_LAUNCH();
+    return 0;
 }
 
 int launch_inviscid_ext(hipStream_t s, const Ext &e, const Fields &f, const Consts &c, int precision, int scheme)

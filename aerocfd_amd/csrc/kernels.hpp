@@ -20,6 +20,28 @@ int launch_inviscid_ext(hipStream_t s, const Ext &e, const Fields &f, const Cons
 int launch_rk3_stage(hipStream_t s, const Ext &e, const Fields &f, const double *qn, int stage, bool viscous);
 int launch_copy(hipStream_t s, const double *a, double *b, long n);
 
+// k_sensor.hip: learned WCNS sensor (training, data, inference)
+struct SensorNet;
+struct SensorArgs {            // passed to the WCNS kernels when the sensor is on
+    const SensorNet *net;
+    float threshold;           // use the linear weights where p < threshold
+    unsigned long long *stats; // [0] faces evaluated, [1] faces given linear weights
+};
+int launch_mlp_grad(hipStream_t s, const float *w, const float *X, const float *Y, long n, long first, long count,
+                    long stride, long offset, float *part, float *lpart, float *grad, float *loss);
+long mlp_partials_needed(long count);
+int launch_adam(hipStream_t s, float *w, const float *g, float *m, float *v, float lr, int t);
+int launch_sensor_pack(hipStream_t s, const float *w, SensorNet *net);
+int launch_sensor_infer(hipStream_t s, const SensorNet *net, const float *X, long n, float *prob);
+long sensor_samples(const Ext &e, int axis, int stride);
+int launch_sensor_collect(hipStream_t s, const Ext &e, const Fields &f, double gamma, int axis, int stride,
+                          float *X, float *Y);
+size_t sensor_net_bytes();
+long diff_partials(const Ext &e);
+int launch_diff_partial(hipStream_t s, const Ext &e, const double *a, const double *b, double *part);
+int launch_inviscid_wcns_sensor(hipStream_t s, const Ext &e, const Fields &f, const Consts &c, int precision,
+                                const SensorArgs &sa);
+
 // k_lusgs.hip
 // workspace of the tile sweeps: sweep-layout operands (normal memory),
 // exchange slots (uncached memory, sentinel-filled) and an error word

@@ -65,6 +65,7 @@ class FlowConfig:
     precision: int = 0           # 0 FP64, 1 FP32 flux / FP64 update
     lusgs_kernel: int = 0        # 0 hyperplane launches, 1 pipelined tiles
     inviscid_scheme: int = 0     # 0 WENO3 + Roe (reference); 1 Steger-Warming + WENO5, 2 WCNS
+    use_graph: int = 0           # 1 replay one captured HIP graph per step
 
     # non-dimensional freestream, sub_init_flowfield.f90:30-39
     @property
@@ -197,6 +198,41 @@ class Solver:
         names.update(TIMED_KERNELS)
         return {name: (float(ms[s]), int(calls[s])) for name, s in names.items()}
 
+    # ------------------------------------------------ learned WCNS sensor
+    def sensor_weights(self, w=None):
+        """Return the 1505 FP32 parameters, or set them when w is given."""
+        if w is None:
+            out = np.zeros(SENSOR_PARAMS, dtype=np.float32)
+            _lib.check(self.lib.acfd_sensor_weights(self.h, out.ctypes.data_as(C.POINTER(C.c_float)), C.c_int(0)))
+            return out
+        a = np.ascontiguousarray(w, dtype=np.float32)
+        _lib.check(self.lib.acfd_sensor_weights(self.h, a.ctypes.data_as(C.POINTER(C.c_float)), C.c_int(1)))
+
+    def sensor_collect(self, stride=1):
+        n = C.c_long(0)
+        _lib.check(self.lib.acfd_sensor_collect(self.h, C.c_int(stride), C.byref(n)))
+        return n.value
+
+    def sensor_train(self, iters, batch=4096, lr=1e-3, seed=0):
+        losses = np.zeros(max(iters, 1), dtype=np.float32)
+        _lib.check(self.lib.acfd_sensor_train(self.h, C.c_int(iters), C.c_int(batch), C.c_float(lr),
+                                              C.c_uint(seed), losses.ctypes.data_as(C.POINTER(C.c_float))))
+        return losses[:iters]
+
+    def sensor_evaluate(self, threshold=0.5):
+        out = np.zeros(4)
+        _lib.check(self.lib.acfd_sensor_evaluate(self.h, C.c_float(threshold), _dp(out)))
+        return dict(zip(("accuracy", "linear", "false_linear", "labelled_nonlinear"), out))
+
+    def sensor_enable(self, on=True, threshold=0.5, audit_interval=0, audit_bound=1e-6):
+        _lib.check(self.lib.acfd_sensor_enable(self.h, C.c_int(int(on)), C.c_float(threshold),
+                                               C.c_int(audit_interval), C.c_double(audit_bound)))
+
+    def sensor_stats(self):
+        out = np.zeros(6)
+        _lib.check(self.lib.acfd_sensor_stats(self.h, _dp(out)))
+        return dict(zip(("faces", "linear", "audits", "audit_max_dev", "blocks_fallen_back"), out[:5]))
+
     def exchange_coordinates(self, xyz, shifts=None):
         """Fill the coordinate halos of every block (UpdateBufferCoordinate).
         xyz: list per block of array (3, nk, nj, ni), updated in place;
@@ -325,6 +361,36 @@ def sa_model(cfg, ranges, nut, dt, pri_v, amu, dudx, vor, dxidx, inv_j, aalpha, 
                                 _dp(_arr(ggamma, field_shape(r, 3))), _dp(length), _dp(_arr(dist, s1)),
                                 _dp(_arr(spacing, s1)), C.c_double(cfl)))
     return nut, sa_rhs, length
+
+
+SENSOR_PARAMS = 1505
+
+
+def mlp_grad(w, X, Y):
+    """Mean loss and gradient of the sensor network over samples X[n][12],
+    Y[n] computed by the GPU training kernels."""
+    lib = _lib.load()
+    w = np.ascontiguousarray(w, dtype=np.float32)
+    X = np.ascontiguousarray(X, dtype=np.float32)
+    Y = np.ascontiguousarray(Y, dtype=np.float32)
+    loss = C.c_float(0)
+    grad = np.zeros(SENSOR_PARAMS, dtype=np.float32)
+    fp = C.POINTER(C.c_float)
+    _lib.check(lib.acfd_mlp_grad(w.ctypes.data_as(fp), X.ctypes.data_as(fp), Y.ctypes.data_as(fp),
+                                 C.c_long(X.shape[0]), C.byref(loss), grad.ctypes.data_as(fp)))
+    return loss.value, grad
+
+
+def sensor_infer(w, X):
+    """Probabilities of the sensor network through its BF16 MFMA routine."""
+    lib = _lib.load()
+    w = np.ascontiguousarray(w, dtype=np.float32)
+    X = np.ascontiguousarray(X, dtype=np.float32)
+    p = np.zeros(X.shape[0], dtype=np.float32)
+    fp = C.POINTER(C.c_float)
+    _lib.check(lib.acfd_sensor_infer(w.ctypes.data_as(fp), X.ctypes.data_as(fp), C.c_long(X.shape[0]),
+                                     p.ctypes.data_as(fp)))
+    return p
 
 
 def mcell_updates_per_s(totalnodes, steps, seconds):

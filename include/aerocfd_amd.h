@@ -52,7 +52,8 @@ typedef struct acfd_config {
     int inviscid_scheme;            /* 0 = WENO3 + Roe (the reference's inviscid_flux_1); no counterpart
                                        in the reference: 1 = Steger-Warming splitting + WENO5,
                                        2 = WCNS (WENO5 midpoint interpolation + Roe, 6th-order difference) */
-    int reserved[4];
+    int use_graph;                  /* 1 = acfd_step replays one captured HIP graph per step (same results) */
+    int reserved[3];
 } acfd_config;
 
 /* One subface record, as read from bcinfo (sub_init_readbcinfo.f90:96-98):
@@ -180,6 +181,44 @@ int acfd_sa_step(const acfd_config *cfg, const int *ranges, double *nut, double 
  * acfd_set_topology (NULL = none).  xyz[m] points to 3 consecutive extended
  * scalar fields x,y,z of block m+1, updated in place on the host. */
 int acfd_exchange_coordinates(acfd_ctx *ctx, double **xyz, const double *shift);
+
+/* --------------------------------------------------- learned WCNS sensor */
+/* No counterpart in the reference.  A network 12 -> 32 -> 32 -> 1 reads the
+ * smoothness indicators and jumps of density and pressure at the six nodes
+ * of a WCNS interface and predicts whether the WENO5 nonlinear weights are
+ * needed (label: they differ from the linear weights by more than 1e-3);
+ * where it says no, the reconstruction uses the linear weights.  Inference
+ * runs as BF16 MFMA with FP32 accumulation inside the WCNS kernels;
+ * training runs on the GPU (FP32 forward/backward, Adam).  Parameters: 1505
+ * floats W1[12][32], b1[32], W2[32][32], b2[32], w3[32], b3. */
+
+/* get (set = 0) or set (set = 1) the FP32 parameters */
+int acfd_sensor_weights(acfd_ctx *ctx, float *w, int set);
+/* training samples from the current state: every face with a full
+ * six-node stencil on every stride-th grid line, all blocks and
+ * directions; *count = number of samples */
+int acfd_sensor_collect(acfd_ctx *ctx, int stride, long *count);
+/* iters Adam steps on minibatches of the samples (binary cross-entropy);
+ * losses[it] receives each minibatch loss when not NULL */
+int acfd_sensor_train(acfd_ctx *ctx, int iters, int batch, float lr, unsigned seed, float *losses);
+/* out[4] on the collected samples at the given threshold: accuracy,
+ * fraction given linear weights, fraction of nonlinear-labelled faces given
+ * linear weights, fraction labelled nonlinear */
+int acfd_sensor_evaluate(acfd_ctx *ctx, float threshold, double *out);
+/* use the sensor in the WCNS residual (inviscid_scheme 2): linear weights
+ * where its probability is below threshold; every audit_interval
+ * evaluations of a block the plain scheme runs too, and a relative
+ * deviation of the residual above audit_bound makes that block fall back
+ * to plain WCNS for good (audit_interval 0: no audits) */
+int acfd_sensor_enable(acfd_ctx *ctx, int on, float threshold, int audit_interval, double audit_bound);
+/* out[6]: faces evaluated, faces given linear weights, audits, largest
+ * audit deviation, blocks fallen back, 0 */
+int acfd_sensor_stats(acfd_ctx *ctx, double *out);
+/* stateless, for checking the training kernels: mean loss and its
+ * gradient over n samples X[n][12], Y[n] */
+int acfd_mlp_grad(const float *w, const float *X, const float *Y, long n, float *loss, float *grad);
+/* stateless batch inference through the BF16 MFMA routine of the WCNS kernels */
+int acfd_sensor_infer(const float *w, const float *X, long n, float *prob);
 
 #ifdef __cplusplus
 }
