@@ -52,7 +52,7 @@ constexpr int kSensorParams = 1505, kSensorFeatures = 12;  // the sensor network
 struct Block {
     Ext e;
     Fields f;
-    double *arena = nullptr;
+    long arena_off = 0;     // doubles from the context's arena base
     int tile_shape = -1;    // chosen by the autotuner on first use
     std::vector<acfd_subface> sub;
 };
@@ -93,6 +93,8 @@ struct acfd_ctx {
     Consts c;
     hipStream_t stream = nullptr;
     std::vector<Block> blk;
+    double *arena = nullptr;  // every block's fields
+    long arena_n = 0;
     long totalnodes = 0;
     bool topo = false;
     // device scalars
@@ -101,8 +103,12 @@ struct acfd_ctx {
     unsigned int *d_nneg = nullptr;
     double *d_res = nullptr;       // 6 sums
     double *d_partial = nullptr;   // residual partials
-    LuWs lu{nullptr, nullptr, nullptr};  // tile-sweep workspace (lusgs_kernel 1)
+    LuWs lu{nullptr, nullptr, nullptr};  // tile-sweep workspace (lusgs_kernel 1, 2)
     long lu_ex_n = 0;
+    LuBatch lub[4];          // lusgs_kernel 2: every block in one launch, per tile shape
+    int lub_shape = -1;      // chosen by the autotuner on first use
+    int *lu_ctr = nullptr;   // ticket counters of the batched sweeps
+    double *d_cfl_blk = nullptr;  // cfl after each block's find_dt
     double *d_stage = nullptr;     // upload/download staging
     long stage_n = 0;
     // pointer tables for the multi-block copy kernels
@@ -154,11 +160,6 @@ struct acfd_ctx {
 
 namespace {
 
-void free_block(Block &b)
-{
-    if (b.arena) (void)hipFree(b.arena);
-    b.arena = nullptr;
-}
 
 template <typename T>
 int dev_upload(T **dst, const std::vector<T> &src)
@@ -282,7 +283,9 @@ void inward(int face, int &di, int &dj, int &dk)
     dk = axis == 2 ? s : 0;
 }
 
-int alloc_block(acfd_ctx *ctx, Block &b)
+// Field pointers of block b from `base` (null: only count); returns the
+// block's doubles, rounded up to 512-byte multiples.
+long place_block(acfd_ctx *ctx, Block &b, double *base)
 {
     const Ext &e = b.e;
     const bool sa = ctx->c.turb == 2 && ctx->c.solver == 0;
@@ -300,15 +303,27 @@ int alloc_block(acfd_ctx *ctx, Block &b)
         {&b.f.q0, bdf ? 10 : 0}, {&b.f.w_flux, 12}, {&b.f.w_dqs, 5}, {&b.f.w_dq, 5},
         {&b.f.w_diag, 1}, {&b.f.w_r, 3}, {&b.f.w_sa, sa ? 9 : 0},
     };
-    long tot = 0;
-    for (auto &s : slots) tot += (long)s.nc * e.n;
-    HIP_TRY(hipMalloc((void **)&b.arena, tot * sizeof(double)));
-    HIP_TRY(hipMemsetAsync(b.arena, 0, tot * sizeof(double), ctx->stream));
     long off = 0;
     for (auto &s : slots) {
-        *s.p = s.nc ? b.arena + off : nullptr;
-        off += (long)s.nc * e.n;
+        if (base) *s.p = s.nc ? base + off : nullptr;
+        off += ((long)s.nc * e.n + 63) / 64 * 64;
     }
+    return off;
+}
+
+// Every block's fields in one allocation (the arena), so device tables can
+// address any block's field as an offset from one base pointer.
+int alloc_blocks(acfd_ctx *ctx)
+{
+    long tot = 0;
+    for (auto &b : ctx->blk) {
+        b.arena_off = tot;
+        tot += place_block(ctx, b, nullptr);
+    }
+    HIP_TRY(hipMalloc((void **)&ctx->arena, tot * sizeof(double)));
+    HIP_TRY(hipMemsetAsync(ctx->arena, 0, tot * sizeof(double), ctx->stream));
+    ctx->arena_n = tot;
+    for (auto &b : ctx->blk) place_block(ctx, b, ctx->arena + b.arena_off);
     return 0;
 }
 
@@ -356,13 +371,34 @@ int ensure_stage(acfd_ctx *ctx, long n)
 // leaves them that way) and an error word.  Nothing for hyperplane sweeps.
 int alloc_lu_ws(acfd_ctx *ctx)
 {
-    if (ctx->cfg.lusgs_kernel != 1) return 0;
-    long sl_n = 0, ex_n = 0;
+    const int nb = (int)ctx->blk.size();
+    HIP_TRY(hipMalloc((void **)&ctx->d_cfl_blk, nb * sizeof(double)));
+    for (int m = 0; m < nb; m++) HIP_TRY(hipMemcpy(ctx->d_cfl_blk + m, &ctx->cfg.cfl, sizeof(double), hipMemcpyHostToDevice));
+    if (ctx->cfg.lusgs_kernel != 1 && ctx->cfg.lusgs_kernel != 2) return 0;
+    const bool batch = ctx->cfg.lusgs_kernel == 2;
+    std::vector<Ext> es;
+    std::vector<Fields> fs;
     for (auto &b : ctx->blk) {
-        long a = 0, x = 0;
-        lusgs_ws_sizes(b.e, &a, &x);
-        sl_n = std::max(sl_n, a);
-        ex_n = std::max(ex_n, x);
+        es.push_back(b.e);
+        fs.push_back(b.f);
+    }
+    long sl_n = 0, ex_n = 0;
+    if (batch) {
+        // every block has its own part of the workspace; sized for the
+        // largest total over the tile shapes
+        for (int sh = 0; sh < lusgs_tile_shapes(); sh++) {
+            long a = 0, x = 0;
+            lusgs_batch_ws(nb, es.data(), sh, &a, &x);
+            sl_n = std::max(sl_n, a);
+            ex_n = std::max(ex_n, x);
+        }
+    } else {
+        for (auto &b : ctx->blk) {
+            long a = 0, x = 0;
+            lusgs_ws_sizes(b.e, &a, &x);
+            sl_n = std::max(sl_n, a);
+            ex_n = std::max(ex_n, x);
+        }
     }
     if (sl_n <= 0) return 0;
     HIP_TRY(hipMalloc((void **)&ctx->lu.sl, (size_t)sl_n * sizeof(double)));
@@ -372,6 +408,13 @@ int alloc_lu_ws(acfd_ctx *ctx)
     if (!ctx->lu.ex) return fail(ACFD_EDEVICE, "uncached device memory for the tile exchange is unavailable");
     ctx->lu_ex_n = ex_n;
     KRN_TRY(launch_fill_sentinel(ctx->stream, ctx->lu.ex, ex_n));
+    if (batch) {
+        HIP_TRY(hipMalloc((void **)&ctx->lu_ctr, 2 * sizeof(int)));
+        HIP_TRY(hipMemset(ctx->lu_ctr, 0, 2 * sizeof(int)));
+        for (int sh = 0; sh < lusgs_tile_shapes(); sh++)
+            if (lusgs_batch_make(nb, es.data(), fs.data(), ctx->arena, sh, ctx->lu.sl, ctx->lu.ex, &ctx->lub[sh]))
+                return fail(ACFD_EDEVICE, "batched LU-SGS tables: device allocation failed");
+    }
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     return 0;
 }
@@ -386,6 +429,7 @@ int tile_error(acfd_ctx *ctx)
     if (err) {
         (void)launch_fill_sentinel(ctx->stream, ctx->lu.ex, ctx->lu_ex_n);
         (void)hipMemsetAsync(ctx->lu.err, 0, sizeof(int), ctx->stream);
+        if (ctx->lu_ctr) (void)hipMemsetAsync(ctx->lu_ctr, 0, 2 * sizeof(int), ctx->stream);
         (void)hipStreamSynchronize(ctx->stream);
         return fail(ACFD_EDEVICE, "pipelined LU-SGS sweep timed out waiting for a neighbouring tile");
     }
@@ -516,7 +560,8 @@ int st_find_dt(acfd_ctx *ctx, int m)
 {
     StageTimer t(ctx, ACFD_STAGE_FIND_DT);
     if (ctx->c.itime == 0) KRN_TRY(launch_reset_stats(ctx->stream, ctx->d_stats));
-    KRN_TRY(launch_find_dt(ctx->stream, ctx->blk[m].e, ctx->blk[m].f, ctx->c, ctx->d_cfl, ctx->d_stats));
+    KRN_TRY(launch_find_dt(ctx->stream, ctx->blk[m].e, ctx->blk[m].f, ctx->c, ctx->d_cfl, ctx->d_cfl_blk + m,
+                           ctx->d_stats));
     return 0;
 }
 
@@ -565,7 +610,7 @@ int tune_tiles(acfd_ctx *ctx, Block &b)
     return 0;
 }
 
-int st_implicit(acfd_ctx *ctx, int m)
+int st_implicit(acfd_ctx *ctx, int m, const double *cfl)
 {
     StageTimer t(ctx, ACFD_STAGE_IMPLICIT1ST);
     Block &b = ctx->blk[m];
@@ -577,16 +622,93 @@ int st_implicit(acfd_ctx *ctx, int m)
         }
         StageTimer ts(ctx, ph == 2 ? kTimeLuForward : ph == 3 ? kTimeLuBackward
                            : ph == 1 ? kTimeLuGather : ph == 4 ? kTimeLuAdd : kTimeLuOther);
-        KRN_TRY(launch_lusgs_phase(ctx->stream, b.e, b.f, ctx->c, ctx->d_cfl, variant, &ctx->lu, ph,
+        KRN_TRY(launch_lusgs_phase(ctx->stream, b.e, b.f, ctx->c, cfl, variant, &ctx->lu, ph,
                                    b.tile_shape < 0 ? 0 : b.tile_shape));
     }
     return 0;
 }
 
-int st_sa(acfd_ctx *ctx, int m)
+// Autotuner of the batched sweep: the same timing as tune_tiles over the
+// whole batch, one shape for all blocks.
+int tune_batch(acfd_ctx *ctx)
+{
+    if (const char *fix = std::getenv("ACFD_TILE_SHAPE")) {
+        int s = std::atoi(fix);
+        if (s >= 0 && s < lusgs_tile_shapes()) {
+            ctx->lub_shape = s;
+            return 0;
+        }
+    }
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    float best = 1e30f;
+    int best_s = 0;
+    for (int s = 0; s < lusgs_tile_shapes(); s++) {
+        const LuBatch &lb = ctx->lub[s];
+        if (lb.total <= 0) continue;
+        (void)hipEventRecord(e0, ctx->stream);
+        int rc = launch_lusgs_batch(ctx->stream, lb, 1, ctx->c.gamma, ctx->lu.err, ctx->lu_ctr);
+        if (!rc) rc = launch_lusgs_batch(ctx->stream, lb, 2, ctx->c.gamma, ctx->lu.err, ctx->lu_ctr);
+        (void)hipEventRecord(e1, ctx->stream);
+        if (rc) return fail(ACFD_EDEVICE, "batched tile sweep launch failed during tuning");
+        HIP_TRY(hipEventSynchronize(e1));
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        if (std::getenv("ACFD_VERBOSE"))
+            std::fprintf(stderr, "acfd: batched sweep shape %d (%d tiles, %d blocks): %.3f ms\n", s, lb.total,
+                         lb.nent, ms);
+        if (ms < best) {
+            best = ms;
+            best_s = s;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    ctx->lub_shape = best_s;
+    return 0;
+}
+
+// implicit1st of every block, the sweeps batched over the blocks.  Blocks
+// only meet at the halo refresh, so updating all of them after all their
+// residuals equals updating each after its own; block m's update reads the
+// cfl its find_dt left (cfl + m), or the global one when cfl_per_block is
+// null.
+int st_implicit_all(acfd_ctx *ctx, const double *cfl_per_block)
+{
+    const int nb = (int)ctx->blk.size();
+    if (ctx->cfg.lusgs_kernel != 2) {
+        for (int m = 0; m < nb; m++) {
+            int rc = st_implicit(ctx, m, cfl_per_block ? cfl_per_block + m : ctx->d_cfl);
+            if (rc) return rc;
+        }
+        return 0;
+    }
+    StageTimer t(ctx, ACFD_STAGE_IMPLICIT1ST);
+    {
+        StageTimer ts(ctx, kTimeLuOther);
+        for (int m = 0; m < nb; m++) {
+            const Block &b = ctx->blk[m];
+            KRN_TRY(launch_lusgs_phase(ctx->stream, b.e, b.f, ctx->c, cfl_per_block ? cfl_per_block + m : ctx->d_cfl,
+                                       1, &ctx->lu, 0, 0));
+        }
+    }
+    if (ctx->lub_shape < 0) {
+        int rc = tune_batch(ctx);
+        if (rc) return rc;
+    }
+    const LuBatch &lb = ctx->lub[ctx->lub_shape];
+    for (int ph = 1; ph < kLusgsPhases; ph++) {
+        StageTimer ts(ctx, ph == 2 ? kTimeLuForward : ph == 3 ? kTimeLuBackward : ph == 1 ? kTimeLuGather : kTimeLuAdd);
+        KRN_TRY(launch_lusgs_batch(ctx->stream, lb, ph, ctx->c.gamma, ctx->lu.err, ctx->lu_ctr));
+    }
+    return 0;
+}
+
+int st_sa(acfd_ctx *ctx, int m, const double *cfl)
 {
     StageTimer t(ctx, ACFD_STAGE_SA);
-    KRN_TRY(launch_sa(ctx->stream, ctx->blk[m].e, ctx->blk[m].f, ctx->c, ctx->d_cfl));
+    KRN_TRY(launch_sa(ctx->stream, ctx->blk[m].e, ctx->blk[m].f, ctx->c, cfl));
     return 0;
 }
 
@@ -668,11 +790,21 @@ int one_step(acfd_ctx *ctx)
     const int aux = sa ? (ctx->c.des == 2 ? 2 : 1) : 0;
     const int nb = (int)ctx->blk.size();
     int rc = 0;
-    if (ctx->cfg.iflag_timeadvance == 2) {
+    // block m's update reads the cfl its own find_dt left (the reference
+    // runs each block's find_dt, implicit1st and sa_model back to back)
+    const double *cfl = ctx->c.itime == 0 ? nullptr : ctx->d_cfl_blk;
+    auto cfl_of = [&](int m) { return cfl ? cfl + m : ctx->d_cfl; };
+    if (ctx->cfg.iflag_timeadvance == 2 && ctx->cfg.lusgs_kernel == 2) {
+        for (int m = 0; m < nb && !rc; m++) rc = residual_stage(ctx, m, aux);
+        if (!rc) rc = st_implicit_all(ctx, cfl ? cfl : nullptr);
+        for (int m = 0; m < nb && !rc && sa; m++) rc = st_sa(ctx, m, cfl_of(m));
+        if (!rc) rc = refresh(ctx);
+        for (int m = 0; m < nb && !rc; m++) rc = st_getpri(ctx, m);
+    } else if (ctx->cfg.iflag_timeadvance == 2) {
         for (int m = 0; m < nb && !rc; m++) {
             rc = residual_stage(ctx, m, aux);
-            if (!rc) rc = st_implicit(ctx, m);
-            if (!rc && sa) rc = st_sa(ctx, m);
+            if (!rc) rc = st_implicit(ctx, m, cfl_of(m));
+            if (!rc && sa) rc = st_sa(ctx, m, cfl_of(m));
         }
         if (!rc) rc = refresh(ctx);
         for (int m = 0; m < nb && !rc; m++) rc = st_getpri(ctx, m);
@@ -685,7 +817,8 @@ int one_step(acfd_ctx *ctx)
             }
         }
         for (int m = 0; m < nb && !rc; m++) rc = st_update_q0(ctx, m);
-        for (int m = 0; m < nb && !rc && sa; m++) rc = st_sa(ctx, m);
+        // after the sub-iterations the reference's cfl is the last block's
+        for (int m = 0; m < nb && !rc && sa; m++) rc = st_sa(ctx, m, ctx->d_cfl);
         if (!rc) rc = refresh(ctx);
         for (int m = 0; m < nb && !rc; m++) rc = st_getpri(ctx, m);
     } else if (ctx->cfg.iflag_timeadvance == 4) {
@@ -765,6 +898,9 @@ int acfd_create(const acfd_config *cfg, acfd_ctx **out)
     if (cfg->inviscid_scheme < 0 || cfg->inviscid_scheme > 2)
         return fail(ACFD_EINVAL, "acfd_create: inviscid_scheme must be 0 (WENO3 + Roe), 1 (Steger-Warming + WENO5) "
                                  "or 2 (WCNS)");
+    if (cfg->lusgs_kernel < 0 || cfg->lusgs_kernel > 2)
+        return fail(ACFD_EINVAL, "acfd_create: lusgs_kernel must be 0 (hyperplanes), 1 (tile sweep per block) or 2 "
+                                 "(tile sweeps of all blocks in one launch)");
     if (cfg->iflag_timeadvance == 4 && cfg->iflag_turbulence == 2 && cfg->iflag_solver == 0)
         return fail(ACFD_EINVAL, "acfd_create: the explicit RK3 advance is for laminar or Euler flow");
     if (cfg->iflag_turbulence != 0 && cfg->iflag_turbulence != 2)
@@ -816,10 +952,13 @@ int acfd_destroy(acfd_ctx *ctx)
     if (!ctx) return 0;
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     if (ctx->graph_exec) (void)hipGraphExecDestroy(ctx->graph_exec);
-    for (auto &b : ctx->blk) free_block(b);
+    if (ctx->arena) (void)hipFree(ctx->arena);
     if (ctx->lu.ex) uc_release(ctx->lu.ex);
     if (ctx->lu.sl) (void)hipFree(ctx->lu.sl);
     if (ctx->lu.err) (void)hipFree(ctx->lu.err);
+    for (auto &b : ctx->lub) lusgs_batch_free(&b);
+    if (ctx->lu_ctr) (void)hipFree(ctx->lu_ctr);
+    if (ctx->d_cfl_blk) (void)hipFree(ctx->d_cfl_blk);
     void *sp[] = {ctx->s_w, ctx->s_m, ctx->s_v, ctx->s_grad, ctx->s_loss, ctx->s_part, ctx->s_lpart, ctx->s_net,
                   ctx->s_X, ctx->s_Y, ctx->s_stats, ctx->s_dpart};
     for (void *p : sp)
@@ -885,8 +1024,8 @@ int acfd_set_topology(acfd_ctx *ctx, int nblk, const int *dims, const int *nsub,
                 return fail(ACFD_EINVAL, "acfd_set_topology: block " + std::to_string(m + 1) +
                                              " mixes physical and interface subfaces on one face");
     }
-    for (auto &b : ctx->blk) {
-        int rc = alloc_block(ctx, b);
+    {
+        int rc = alloc_blocks(ctx);
         if (rc) return rc;
     }
     // halo copy list: every (sender inner layer n, receiver halo layer n)
@@ -1063,11 +1202,11 @@ int acfd_run_stage(acfd_ctx *ctx, int stage)
         for (int m = 0; m < nb && !rc; m++) rc = st_find_dt(ctx, m);
         break;
     case ACFD_STAGE_IMPLICIT1ST:
-        for (int m = 0; m < nb && !rc; m++) rc = st_implicit(ctx, m);
+        rc = st_implicit_all(ctx, nullptr);
         break;
     case ACFD_STAGE_SA:
         if (!(ctx->c.turb == 2 && ctx->c.solver == 0)) return fail(ACFD_EINVAL, "SA stage needs NS + SA");
-        for (int m = 0; m < nb && !rc; m++) rc = st_sa(ctx, m);
+        for (int m = 0; m < nb && !rc; m++) rc = st_sa(ctx, m, ctx->d_cfl);
         break;
     case ACFD_STAGE_PHYSICAL_BC: rc = st_physical_bc(ctx); break;
     case ACFD_STAGE_UPDATE_BUFFER: rc = st_update_buffer(ctx); break;
@@ -1102,8 +1241,9 @@ int acfd_step(acfd_ctx *ctx, int nsteps)
     int s = 0;
     // graph mode: the first step runs eagerly (it also tunes the tile
     // sweeps), the next one is captured, and every later step replays it;
-    // audited sensor steps read results on the host and stay eager
-    const bool graph = ctx->cfg.use_graph && !(ctx->s_on && ctx->s_audit > 0);
+    // audited sensor steps read results on the host, and steps under the
+    // per-stage timers record events between launches: both stay eager
+    const bool graph = ctx->cfg.use_graph && !ctx->timing && !(ctx->s_on && ctx->s_audit > 0);
     if (graph && !ctx->graph_exec && nsteps > 0) {
         int rc = one_step(ctx);
         if (rc) return rc;
@@ -1492,7 +1632,7 @@ int make_single(const acfd_config *cfg, const int *r, acfd_ctx **out)
         acfd_destroy(ctx);
         return fail(ACFD_EINVAL, "ranges: block too small");
     }
-    rc = alloc_block(ctx, ctx->blk[0]);
+    rc = alloc_blocks(ctx);
     if (rc) {
         acfd_destroy(ctx);
         return rc;

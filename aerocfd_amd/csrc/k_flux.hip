@@ -404,10 +404,12 @@ __global__ void k_find_dt_steady(Ext e, const double *__restrict__ pri, const do
 
 __global__ void k_find_dt_unsteady(Ext e, const double *__restrict__ pri, const double *__restrict__ amu,
                                    const double *__restrict__ dxidx, const double *__restrict__ dt,
-                                   double *cfl, double re)
+                                   double *cfl, double *cfl_blk, double re)
 {
     long n = e.n - 1;  // node (ie, je, ke)
-    *cfl = dt[n] * local_rate(e, n, pri, amu, dxidx, re);
+    const double v = dt[n] * local_rate(e, n, pri, amu, dxidx, re);
+    *cfl = v;
+    if (cfl_blk) *cfl_blk = v;
 }
 
 __global__ void k_reset_stats(unsigned long long *st)
@@ -423,7 +425,7 @@ int launch_reset_stats(hipStream_t s, unsigned long long *stats)
     return 0;
 }
 
-int launch_find_dt(hipStream_t s, const Ext &e, const Fields &f, const Consts &c, double *cfl,
+int launch_find_dt(hipStream_t s, const Ext &e, const Fields &f, const Consts &c, double *cfl, double *cfl_blk,
                    unsigned long long *stats)
 {
     if (c.itime == 0) {
@@ -431,7 +433,8 @@ int launch_find_dt(hipStream_t s, const Ext &e, const Fields &f, const Consts &c
         hipLaunchKernelGGL(k_find_dt_steady, dim3((unsigned)((e.n + bs - 1) / bs)), dim3(bs), 0, s, e,
                            f.pri, f.amu, f.dxidx, f.dt, cfl, c.re, stats);
     } else {
-        hipLaunchKernelGGL(k_find_dt_unsteady, dim3(1), dim3(1), 0, s, e, f.pri, f.amu, f.dxidx, f.dt, cfl, c.re);
+        hipLaunchKernelGGL(k_find_dt_unsteady, dim3(1), dim3(1), 0, s, e, f.pri, f.amu, f.dxidx, f.dt, cfl, cfl_blk,
+                           c.re);
     }
     ACFD_CHECK_LAUNCH();
     return 0;

@@ -46,6 +46,8 @@
 // divisions of forming calc_jm's matrix); the values agree with the matrix
 // form to rounding.
 #include <algorithm>
+#include <cstdlib>
+#include <vector>
 
 #include "common.hpp"
 #include "kernels.hpp"
@@ -359,16 +361,14 @@ __device__ inline const double *gather_src(int f, long N, const double *q, const
 // at a time (each lane then stores whole 16-byte pairs).  TT is a template
 // parameter so that a thread's CT loads are issued back to back.
 template <int TT>
-__global__ void __launch_bounds__(TT) k_sl_gather(Ext e, SlGeom g, const double *__restrict__ q,
-                                                  const double *__restrict__ dxidx,
-                                                  const double *__restrict__ inv_j,
-                                                  const double *__restrict__ rr,
-                                                  const double *__restrict__ diag,
-                                                  const double *__restrict__ rhs, double *__restrict__ sl)
+__device__ inline void sl_gather_tile(const Ext &e, const SlGeom &g, int tile, const double *__restrict__ q,
+                                      const double *__restrict__ dxidx, const double *__restrict__ inv_j,
+                                      const double *__restrict__ rr, const double *__restrict__ diag,
+                                      const double *__restrict__ rhs, double *__restrict__ sl,
+                                      double *buf /* [2][CT][TT] LDS, swizzled */)
 {
     constexpr int CT = sl_ct_of(TT), lstep = TT / CT;
-    extern __shared__ double buf[];  // [2][CT][TT], swizzled
-    const int tile = blockIdx.x, x = threadIdx.x;
+    const int x = threadIdx.x;
     const int ii_n = x % CT;
     const int tj = x & (g.TJ - 1), tk = x >> g.lgTJ;
     const bool me_ok = sl_line_ok(g, tile, x);
@@ -412,15 +412,26 @@ __global__ void __launch_bounds__(TT) k_sl_gather(Ext e, SlGeom g, const double 
     }
 }
 
+template <int TT>
+__global__ void __launch_bounds__(TT) k_sl_gather(Ext e, SlGeom g, const double *__restrict__ q,
+                                                  const double *__restrict__ dxidx,
+                                                  const double *__restrict__ inv_j,
+                                                  const double *__restrict__ rr,
+                                                  const double *__restrict__ diag,
+                                                  const double *__restrict__ rhs, double *__restrict__ sl)
+{
+    extern __shared__ double buf[];
+    sl_gather_tile<TT>(e, g, (int)blockIdx.x, q, dxidx, inv_j, rr, diag, rhs, sl, buf);
+}
+
 // q += dq from the sweep layout (:568-574), same walk as the gather, one
 // component at a time.
 template <int TT>
-__global__ void __launch_bounds__(TT) k_sl_add(Ext e, SlGeom g, const double *__restrict__ sl,
-                                               double *__restrict__ q)
+__device__ inline void sl_add_tile(const Ext &e, const SlGeom &g, int tile, const double *__restrict__ sl,
+                                   double *__restrict__ q, double *buf /* [CT][TT] LDS, swizzled */)
 {
     constexpr int CT = sl_ct_of(TT), lstep = TT / CT;
-    extern __shared__ double buf[];  // [CT][TT], swizzled
-    const int tile = blockIdx.x, x = threadIdx.x;
+    const int x = threadIdx.x;
     const int ii_n = x % CT;
     const int tj = x & (g.TJ - 1), tk = x >> g.lgTJ;
     const bool me_ok = sl_line_ok(g, tile, x);
@@ -457,6 +468,14 @@ __global__ void __launch_bounds__(TT) k_sl_add(Ext e, SlGeom g, const double *__
             __syncthreads();
         }
     }
+}
+
+template <int TT>
+__global__ void __launch_bounds__(TT) k_sl_add(Ext e, SlGeom g, const double *__restrict__ sl,
+                                               double *__restrict__ q)
+{
+    extern __shared__ double buf[];
+    sl_add_tile<TT>(e, g, (int)blockIdx.x, sl, q, buf);
 }
 
 template <int TT>
@@ -504,12 +523,10 @@ __device__ inline void lu_node(const double *op, const double *ta, const double 
 // is dispatched in reverse order.  Either way every wait is on a tile that
 // was dispatched earlier, so it is running or done.
 template <int DIR, int TJ, int TK>
-__global__ void __launch_bounds__(TJ *TK) k_lu_tiles(SlGeom g, double *__restrict__ sl, double gamma,
-                                                      double *ex, int *err)
+__device__ inline void lu_tiles_body(const SlGeom &g, int tile, double *__restrict__ sl, double gamma, double *ex,
+                                     int *err, double (*pj)[5][TJ * TK], double (*pk)[5][TJ * TK])
 {
     constexpr int TT = TJ * TK;
-    __shared__ double pj[2][5][TT], pk[2][5][TT];
-    const int tile = DIR > 0 ? (int)blockIdx.x : g.ntiles - 1 - (int)blockIdx.x;
     const int bj = tile % g.nbj, bk = tile / g.nbj;
     const int l = threadIdx.x, tj = l % TJ, tk = l / TJ;
     const int lj = bj * TJ + tj, lk = bk * TK + tk;
@@ -623,6 +640,94 @@ __global__ void __launch_bounds__(TJ *TK) k_lu_tiles(SlGeom g, double *__restric
     }
 }
 
+template <int DIR, int TJ, int TK>
+__global__ void __launch_bounds__(TJ *TK) k_lu_tiles(SlGeom g, double *__restrict__ sl, double gamma,
+                                                      double *ex, int *err)
+{
+    __shared__ double pj[2][5][TJ * TK], pk[2][5][TJ * TK];
+    const int tile = DIR > 0 ? (int)blockIdx.x : g.ntiles - 1 - (int)blockIdx.x;
+    lu_tiles_body<DIR, TJ, TK>(g, tile, sl, gamma, ex, err, pj, pk);
+}
+
+// ------------------------------------------------------------ batched tiles
+// Every block of a context in one launch per phase.  A table holds one
+// entry per block (ranges, tiling, operand fields, its part of the
+// workspace) and the launch's flat tile numbers run over the blocks one
+// after another, tile0 being a block's first.  Fields are offsets from the
+// base of the blocks' arena and workspace parts offsets from the
+// workspace's bases, all passed as kernel arguments: pointers derived from
+// kernel arguments are known to be global memory, so the accesses compile
+// to global loads and stores (a pointer read from a table would compile to
+// flat ones, which the LDS barriers of the sweep would also wait for).
+struct LuBatchEnt {
+    Ext e;
+    SlGeom g;
+    long q, dxidx, inv_j, rr, diag, rhs;
+    long sl, ex;
+    int tile0;
+};
+
+template <int TT>
+__global__ void __launch_bounds__(TT) k_sl_gather_batch(const LuBatchEnt *__restrict__ ents,
+                                                        const int *__restrict__ tile_ent, double *arena,
+                                                        double *slb)
+{
+    extern __shared__ double buf[];
+    const int flat = (int)blockIdx.x;
+    const LuBatchEnt &b = ents[tile_ent[flat]];
+    const Ext e = b.e;
+    const SlGeom g = b.g;
+    sl_gather_tile<TT>(e, g, flat - b.tile0, arena + b.q, arena + b.dxidx, arena + b.inv_j, arena + b.rr,
+                       arena + b.diag, arena + b.rhs, slb + b.sl, buf);
+}
+
+template <int TT>
+__global__ void __launch_bounds__(TT) k_sl_add_batch(const LuBatchEnt *__restrict__ ents,
+                                                     const int *__restrict__ tile_ent, double *arena,
+                                                     const double *slb)
+{
+    extern __shared__ double buf[];
+    const int flat = (int)blockIdx.x;
+    const LuBatchEnt &b = ents[tile_ent[flat]];
+    const Ext e = b.e;
+    const SlGeom g = b.g;
+    sl_add_tile<TT>(e, g, flat - b.tile0, slb + b.sl, arena + b.q, buf);
+}
+
+// Flat tile numbers.  When every workgroup of the launch fits on the device
+// at once (`resident`), tile = workgroup index: all tiles run together and
+// every wait ends.  Otherwise the tiles are handed out in the order
+// workgroups start (a ticket counter): a tile then only ever waits on tiles
+// whose workgroups started before it, so the sweep progresses however the
+// launch is dispatched; the last workgroup to finish resets both counters
+// for the next launch.  (The first form is the faster one where it applies:
+// the hardware's placement of consecutive workgroups suits the sweep.)
+template <int DIR, int TJ, int TK>
+__global__ void __launch_bounds__(TJ *TK) k_lu_tiles_batch(const LuBatchEnt *__restrict__ ents,
+                                                            const int *__restrict__ tile_ent, int total,
+                                                            int resident, double *slb, double *exb,
+                                                            double gamma, int *err, int *ctr)
+{
+    __shared__ double pj[2][5][TJ * TK], pk[2][5][TJ * TK];
+    __shared__ int ticket;
+    if (!resident) {
+        if (threadIdx.x == 0)
+            ticket = __hip_atomic_fetch_add(ctr, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __syncthreads();
+    }
+    const int flat = resident ? (int)blockIdx.x : __builtin_amdgcn_readfirstlane(ticket);
+    const LuBatchEnt &b = ents[tile_ent[flat]];
+    const SlGeom g = b.g;
+    const int local = flat - b.tile0;
+    lu_tiles_body<DIR, TJ, TK>(g, DIR > 0 ? local : g.ntiles - 1 - local, slb + b.sl, gamma, exb + b.ex, err, pj,
+                               pk);
+    if (!resident && threadIdx.x == 0 &&
+        __hip_atomic_fetch_add(ctr + 1, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == total - 1) {
+        __hip_atomic_store(ctr, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(ctr + 1, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
 int lusgs_tile_shapes() { return kNumTileShapes; }
 
 long lusgs_tiles(const Ext &e, int shape)
@@ -642,6 +747,163 @@ void lusgs_ws_sizes(const Ext &e, long *sl_n, long *ex_n)
     }
     *sl_n = a;
     *ex_n = b;
+}
+
+static long round_up32(long n) { return (n + 31) / 32 * 32; }
+static void batch_residency(LuBatch *b);
+
+// blocks of the batch in launch order: longest sweep first
+static std::vector<int> batch_order(int nblk, const Ext *e, int shape)
+{
+    std::vector<int> order;
+    for (int m = 0; m < nblk; m++)
+        if (lusgs_tiles(e[m], shape) > 0) order.push_back(m);
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+        const SlGeom ga = sl_geom(e[a], shape), gb = sl_geom(e[b], shape);
+        return ga.nsteps + ga.nbj + ga.nbk > gb.nsteps + gb.nbj + gb.nbk;
+    });
+    return order;
+}
+
+void lusgs_batch_ws(int nblk, const Ext *e, int shape, long *sl_n, long *ex_n)
+{
+    long a = 0, b = 0;
+    for (int m : batch_order(nblk, e, shape)) {
+        const SlGeom g = sl_geom(e[m], shape);
+        a += round_up32((long)g.ntiles * g.nsteps * kSlF * g.TT);
+        b += round_up32((long)g.ntiles * (g.exj_tile + g.exk_tile));
+    }
+    *sl_n = a;
+    *ex_n = b;
+}
+
+int lusgs_batch_make(int nblk, const Ext *e, const Fields *f, double *arena, int shape, double *sl, double *ex,
+                     LuBatch *out)
+{
+    std::vector<LuBatchEnt> ents;
+    std::vector<int> te;
+    long so = 0, xo = 0;
+    for (int m : batch_order(nblk, e, shape)) {
+        const SlGeom g = sl_geom(e[m], shape);
+        LuBatchEnt b;
+        b.e = e[m];
+        b.g = g;
+        b.q = f[m].q - arena;
+        b.dxidx = f[m].dxidx - arena;
+        b.inv_j = f[m].inv_j - arena;
+        b.rr = f[m].w_r - arena;
+        b.diag = f[m].w_diag - arena;
+        b.rhs = f[m].rhs - arena;
+        b.sl = so;
+        b.ex = xo;
+        b.tile0 = (int)te.size();
+        so += round_up32((long)g.ntiles * g.nsteps * kSlF * g.TT);
+        xo += round_up32((long)g.ntiles * (g.exj_tile + g.exk_tile));
+        for (int t = 0; t < g.ntiles; t++) te.push_back((int)ents.size());
+        ents.push_back(b);
+    }
+    out->shape = shape;
+    out->arena = arena;
+    out->sl = sl;
+    out->ex = ex;
+    out->nent = (int)ents.size();
+    out->total = (int)te.size();
+    if (ents.empty()) return 0;
+    batch_residency(out);
+    if (hipMalloc(&out->ents, ents.size() * sizeof(LuBatchEnt)) != hipSuccess) return 1;
+    if (hipMalloc((void **)&out->tile_ent, te.size() * sizeof(int)) != hipSuccess) return 1;
+    if (hipMemcpy(out->ents, ents.data(), ents.size() * sizeof(LuBatchEnt), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(out->tile_ent, te.data(), te.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess)
+        return 1;
+    return 0;
+}
+
+void lusgs_batch_free(LuBatch *b)
+{
+    if (b->ents) (void)hipFree(b->ents);
+    if (b->tile_ent) (void)hipFree(b->tile_ent);
+    b->ents = nullptr;
+    b->tile_ent = nullptr;
+    b->nent = b->total = 0;
+}
+
+// whether all `total` workgroups of the sweep kernel fit on the device at once
+template <int DIR, int TJ, int TK>
+static bool fits_resident(int total)
+{
+    int dev = 0, cus = 0, per_cu = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_lu_tiles_batch<DIR, TJ, TK>, TJ * TK, 0) !=
+            hipSuccess)
+        return false;
+    return (long)cus * per_cu >= total;
+}
+
+template <int DIR, int TJ, int TK>
+static void launch_tiles_batch(hipStream_t s, const LuBatch &b, double gamma, int *err, int *ctr)
+{
+    hipLaunchKernelGGL((k_lu_tiles_batch<DIR, TJ, TK>), dim3((unsigned)b.total), dim3(TJ * TK), 0, s,
+                       (const LuBatchEnt *)b.ents, b.tile_ent, b.total, b.resident[DIR > 0 ? 0 : 1], b.sl, b.ex,
+                       gamma, err, ctr);
+}
+
+static void batch_residency(LuBatch *b)
+{
+    bool f = false, r = false;
+    switch (b->shape) {
+    case 0: f = fits_resident<1, 16, 16>(b->total); r = fits_resident<-1, 16, 16>(b->total); break;
+    case 1: f = fits_resident<1, 16, 8>(b->total); r = fits_resident<-1, 16, 8>(b->total); break;
+    case 2: f = fits_resident<1, 8, 8>(b->total); r = fits_resident<-1, 8, 8>(b->total); break;
+    default: f = fits_resident<1, 16, 4>(b->total); r = fits_resident<-1, 16, 4>(b->total); break;
+    }
+    // ACFD_LU_TICKETS=1 hands tiles out by ticket even when all would fit
+    const char *t = std::getenv("ACFD_LU_TICKETS");
+    const bool tickets = t && std::atoi(t) == 1;
+    b->resident[0] = f && !tickets ? 1 : 0;
+    b->resident[1] = r && !tickets ? 1 : 0;
+}
+
+template <int DIR>
+static void launch_tiles_batch_shape(hipStream_t s, const LuBatch &b, double gamma, int *err, int *ctr)
+{
+    switch (b.shape) {
+    case 0: launch_tiles_batch<DIR, 16, 16>(s, b, gamma, err, ctr); break;
+    case 1: launch_tiles_batch<DIR, 16, 8>(s, b, gamma, err, ctr); break;
+    case 2: launch_tiles_batch<DIR, 8, 8>(s, b, gamma, err, ctr); break;
+    default: launch_tiles_batch<DIR, 16, 4>(s, b, gamma, err, ctr); break;
+    }
+}
+
+template <int TT>
+static void launch_layout_batch(hipStream_t s, int which, const LuBatch &b)
+{
+    const size_t lds = 2 * (size_t)kSlElems * sizeof(double);
+    const LuBatchEnt *ents = (const LuBatchEnt *)b.ents;
+    if (which == 0)
+        hipLaunchKernelGGL(k_sl_gather_batch<TT>, dim3((unsigned)b.total), dim3(TT), lds, s, ents, b.tile_ent,
+                           b.arena, b.sl);
+    else
+        hipLaunchKernelGGL(k_sl_add_batch<TT>, dim3((unsigned)b.total), dim3(TT), lds, s, ents, b.tile_ent,
+                           b.arena, (const double *)b.sl);
+}
+
+// Phases 1-4 of the LU-SGS update for every block of the batch at once
+// (phase 0, the per-node preparation, stays per block).
+int launch_lusgs_batch(hipStream_t s, const LuBatch &b, int phase, double gamma, int *err, int *ctr)
+{
+    if (b.total <= 0 || phase < 1 || phase > 4) return 0;
+    const int tt = kTileShapes[b.shape][0] * kTileShapes[b.shape][1];
+    if (phase == 2) launch_tiles_batch_shape<1>(s, b, gamma, err, ctr);
+    else if (phase == 3) launch_tiles_batch_shape<-1>(s, b, gamma, err, ctr);
+    else {
+        const int which = phase == 1 ? 0 : 1;
+        if (tt == 256) launch_layout_batch<256>(s, which, b);
+        else if (tt == 128) launch_layout_batch<128>(s, which, b);
+        else launch_layout_batch<64>(s, which, b);
+    }
+    ACFD_CHECK_LAUNCH();
+    return 0;
 }
 
 int launch_fill_sentinel(hipStream_t s, double *p, long n)

@@ -10,7 +10,9 @@ int launch_fill(hipStream_t s, double *p, long n, double v);
 int launch_inviscid(hipStream_t s, const Ext &e, const Fields &f, const Consts &c, int precision);
 int launch_viscous(hipStream_t s, const Ext &e, const Fields &f, const Consts &c, int precision,
                    int store_aux);
-int launch_find_dt(hipStream_t s, const Ext &e, const Fields &f, const Consts &c, double *cfl,
+// unsteady: the local cfl of the block's last node goes to *cfl and, when
+// not null, *cfl_blk
+int launch_find_dt(hipStream_t s, const Ext &e, const Fields &f, const Consts &c, double *cfl, double *cfl_blk,
                    unsigned long long *stats);
 int launch_reset_stats(hipStream_t s, unsigned long long *stats);
 
@@ -59,6 +61,20 @@ int lusgs_tile_shapes();
 long lusgs_tiles(const Ext &e, int shape);
 // doubles of the sweep layout and of the exchange slots (max over shapes)
 void lusgs_ws_sizes(const Ext &e, long *sl_n, long *ex_n);
+// all blocks' tile sweeps in one launch per phase (one tile shape for all)
+struct LuBatch {
+    void *ents = nullptr;     // device table, one entry per block
+    int *tile_ent = nullptr;  // device, entry of each flat tile number
+    double *arena = nullptr, *sl = nullptr, *ex = nullptr;  // bases of the table's offsets
+    int nent = 0, total = 0, shape = 0;
+    int resident[2] = {0, 0};  // forward / backward sweep: every workgroup fits on the device at once
+};
+void lusgs_batch_ws(int nblk, const Ext *e, int shape, long *sl_n, long *ex_n);
+// f[m]'s fields must lie in the allocation starting at arena
+int lusgs_batch_make(int nblk, const Ext *e, const Fields *f, double *arena, int shape, double *sl, double *ex,
+                     LuBatch *out);
+void lusgs_batch_free(LuBatch *b);
+int launch_lusgs_batch(hipStream_t s, const LuBatch &b, int phase, double gamma, int *err, int *ctr);
 int launch_fill_sentinel(hipStream_t s, double *p, long n);
 
 // k_bdf2.hip

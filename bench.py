@@ -126,7 +126,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--precision", type=int, default=0, help="0 FP64, 1 FP32 flux / FP64 update")
-    ap.add_argument("--lusgs", type=int, default=1, help="0 hyperplane launches, 1 pipelined tile sweep")
+    ap.add_argument("--lusgs", type=int, default=1, help="0 hyperplane launches, 1 pipelined tile sweep per block, 2 tile sweeps of all blocks in one launch")
+    ap.add_argument("--graph", action="store_true",
+                    help="replay one captured HIP graph per step in the timed region (per-kernel times then "
+                         "come from two eager steps after it)")
+    ap.add_argument("--cuts", type=int, default=1, help="blocks per axis (the workload split into cuts^3 blocks)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB (a known-size kernel that "
@@ -149,9 +153,11 @@ def main():
         torch.cuda.synchronize()
         del src, dst
 
-    case = cases.tgv_box(args.n)
+    cut = tuple(round(args.n * c / args.cuts) for c in range(1, args.cuts))
+    case = cases.tgv_box(args.n, blocks=(cut, cut, cut))
     case.cfg.precision = args.precision
     case.cfg.lusgs_kernel = args.lusgs
+    case.cfg.use_graph = 1 if args.graph else 0
     sol = Solver(case.cfg, case.dims, case.subfaces)
     case.upload(sol)
     del case.fields
@@ -165,7 +171,7 @@ def main():
         sync = torch.cuda.synchronize if torch.cuda.is_available() else (lambda: None)
     except Exception:
         sync = lambda: None  # noqa: E731
-    sol.timing(True)
+    sol.timing(not args.graph)
     sol.sync()
     sync()
     t0 = time.perf_counter()
@@ -174,23 +180,33 @@ def main():
     sol.sync()
     sync()
     t1 = time.perf_counter()
+    tsteps = args.steps
+    if args.graph:
+        # the per-stage timers run eager steps
+        tsteps = 2
+        sol.timing(True)
+        sol.step(tsteps)
     tm = sol.timing(None)
     res = sol.residual()
+    rr = [sol.ranges(m + 1) for m in range(len(case.dims))]
     sol.close()
     secs = t1 - t0
-    nodes = case.totalnodes
+    nodes = args.n ** 3  # the cut workload counts the same nodes as the single block
     value = nodes * args.steps / secs / 1e6
     # dominant kernel among the timed ones (per launch averages)
-    comp = (args.n) ** 3  # computed nodes (all faces are interfaces)
+    # computed nodes per launch: is1..ie1 x js1..je1 x ks1..ke1 of one block, or of
+    # every block for the batched LU-SGS phases
+    comp_all = sum((r[13] - r[12] + 1) * (r[15] - r[14] + 1) * (r[17] - r[16] + 1) for r in rr)
+    batched = {"lusgs_forward", "lusgs_backward", "lusgs_gather", "lusgs_add"} if args.lusgs == 2 else set()
     table = dict(BYTES_PER_NODE)
     if args.lusgs == 0:
         table.update(HYPERPLANE_BYTES)
     cand = {k: v for k, v in tm.items() if k in table and v[1] > 0}
     dom = max(cand, key=lambda k: cand[k][0])
     avg_ms = cand[dom][0] / cand[dom][1]
-    bytes_launch = table[dom] * comp
+    bytes_launch = table[dom] * (comp_all if dom in batched else comp_all / len(rr))
     achieved = bytes_launch / (avg_ms * 1e-3) / 1e9
-    default_cfg = args.n == 256 and args.lusgs == 1 and args.precision == 0
+    default_cfg = args.n == 256 and args.lusgs == 1 and args.precision == 0 and args.cuts == 1
     traffic = MEASURED_TRAFFIC_256.get(dom) if default_cfg else None
     out = {
         "metric": "Mcell-updates/s",
@@ -206,13 +222,13 @@ def main():
         "dtype": "f64" if args.precision == 0 else "f32-flux/f64-update",
         "data": "synthetic",
         "config": {"workload": "tgv_%d^3_periodic_laminar_ns_weno3_roe_lusgs" % args.n, "nodes": nodes,
-                   "blocks": 1, "mach": 0.1, "re": 1600, "scheme": "WENO3+Roe, central viscous, LU-SGS",
-                   "lusgs_kernel": "pipelined-tiles" if args.lusgs == 1 else "hyperplanes"},
+                   "blocks": len(case.dims), "graph": bool(args.graph), "mach": 0.1, "re": 1600, "scheme": "WENO3+Roe, central viscous, LU-SGS",
+                   "lusgs_kernel": {0: "hyperplanes", 1: "pipelined-tiles", 2: "batched-pipelined-tiles"}[args.lusgs]},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": PEAK_HBM_GBS,
                      "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4),
                      "traffic": traffic,
                      "bytes_per_node": table[dom], "avg_launch_ms": round(avg_ms, 4)},
-        "stage_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in tm.items() if v[1] > 0},
+        "stage_ms_per_step": {k: round(v[0] / tsteps, 4) for k, v in tm.items() if v[1] > 0},
         "residual": [float("%.6e" % x) for x in res[:5]],
     }
     if not args.no_cpu:

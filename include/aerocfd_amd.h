@@ -48,7 +48,8 @@ typedef struct acfd_config {
                                        4 = explicit three-stage Shu-Osher RK3 (no counterpart in the
                                        reference; laminar or Euler) */
     int precision;                  /* 0 = FP64 everywhere; 1 = FP32 reconstruction/splitting/viscous, FP64 update */
-    int lusgs_kernel;               /* 0 = hyperplane launches, 1 = pipelined tile sweep (same results) */
+    int lusgs_kernel;               /* 0 = hyperplane launches, 1 = pipelined tile sweep per block,
+                                       2 = tile sweeps of all blocks in one launch (all the same results) */
     int inviscid_scheme;            /* 0 = WENO3 + Roe (the reference's inviscid_flux_1); no counterpart
                                        in the reference: 1 = Steger-Warming splitting + WENO5,
                                        2 = WCNS (WENO5 midpoint interpolation + Roe, 6th-order difference) */
