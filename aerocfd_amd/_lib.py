@@ -28,8 +28,8 @@ class Config(C.Structure):
     _fields_ = [(n, C.c_double) for n in ("ma", "re", "tinf", "aoa_deg", "cfl", "dt", "prl", "prt")] + [
         (n, C.c_int) for n in ("iflag_solver", "iflag_time", "iflag_turbulence", "iflag_des",
                                "iflag_timeadvance", "precision", "lusgs_kernel", "inviscid_scheme",
-                               "use_graph")] + [
-        ("reserved", C.c_int * 3)]
+                               "use_graph", "batch_blocks")] + [
+        ("reserved", C.c_int * 2)]
 
 
 class Subface(C.Structure):

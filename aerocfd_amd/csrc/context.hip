@@ -109,6 +109,9 @@ struct acfd_ctx {
     int lub_shape = -1;      // chosen by the autotuner on first use
     int *lu_ctr = nullptr;   // ticket counters of the batched sweeps
     double *d_cfl_blk = nullptr;  // cfl after each block's find_dt
+    // batch_blocks: block table and the per-kind grids of the batched launches
+    BlockRec *d_blocks = nullptr;
+    BatchGrid bg_getpri, bg_inv[4], bg_visc, bg_dt, bg_prep;
     double *d_stage = nullptr;     // upload/download staging
     long stage_n = 0;
     // pointer tables for the multi-block copy kernels
@@ -364,6 +367,73 @@ int ensure_stage(acfd_ctx *ctx, long n)
     ctx->stage_n = n;
     return 0;
 }
+
+// One batched grid from the blocks' own grids (device arrays).
+int make_grid(BatchGrid &bg, const std::vector<dim3> &g)
+{
+    std::vector<KGrid> kg;
+    std::vector<int> map;
+    for (size_t m = 0; m < g.size(); m++) {
+        KGrid k{(int)map.size(), (int)g[m].x, (int)g[m].y, (int)g[m].z};
+        const long cnt = (long)g[m].x * g[m].y * g[m].z;
+        if (map.size() + cnt > (size_t)INT32_MAX) return fail(ACFD_EINVAL, "batched grid too large");
+        for (long x = 0; x < cnt; x++) map.push_back((int)m);
+        kg.push_back(k);
+    }
+    bg.total = (int)map.size();
+    if (bg.total == 0) return 0;
+    int rc = dev_upload(&bg.kg, kg);
+    if (!rc) rc = dev_upload(&bg.map, map);
+    return rc;
+}
+
+void free_grid(BatchGrid &bg)
+{
+    if (bg.kg) (void)hipFree(bg.kg);
+    if (bg.map) (void)hipFree(bg.map);
+    bg = BatchGrid();
+}
+
+long field_off(const acfd_ctx *ctx, const double *p) { return p ? (long)(p - ctx->arena) : -1; }
+
+// Block table and grids of the batched stage launches (batch_blocks).
+int build_batch(acfd_ctx *ctx)
+{
+    if (!ctx->cfg.batch_blocks) return 0;
+    std::vector<BlockRec> rec;
+    std::vector<dim3> g_pri, g_zero, g_inv[3], g_visc, g_prep;
+    for (auto &b : ctx->blk) {
+        const Fields &f = b.f;
+        BlockRec r;
+        r.e = b.e;
+        FieldOff &o = r.o;
+        o.q = field_off(ctx, f.q); o.pri = field_off(ctx, f.pri); o.rhsi = field_off(ctx, f.rhsi);
+        o.rhsv = field_off(ctx, f.rhsv); o.rhs = field_off(ctx, f.rhs); o.dxidx = field_off(ctx, f.dxidx);
+        o.inv_j = field_off(ctx, f.inv_j); o.amu = field_off(ctx, f.amu); o.dt = field_off(ctx, f.dt);
+        o.dudx = field_off(ctx, f.dudx); o.tao = field_off(ctx, f.tao); o.vor = field_off(ctx, f.vor);
+        o.aal = field_off(ctx, f.aal); o.bbe = field_off(ctx, f.bbe); o.gga = field_off(ctx, f.gga);
+        o.dist = field_off(ctx, f.dist); o.spacing = field_off(ctx, f.spacing); o.length = field_off(ctx, f.length);
+        o.nut = field_off(ctx, f.nut); o.sa_rhs = field_off(ctx, f.sa_rhs); o.q0 = field_off(ctx, f.q0);
+        o.w_flux = field_off(ctx, f.w_flux); o.w_dqs = field_off(ctx, f.w_dqs); o.w_dq = field_off(ctx, f.w_dq);
+        o.w_diag = field_off(ctx, f.w_diag); o.w_r = field_off(ctx, f.w_r); o.w_sa = field_off(ctx, f.w_sa);
+        rec.push_back(r);
+        g_pri.push_back(grid_1d(b.e.n));
+        g_zero.push_back(grid_1d(5 * b.e.n));
+        for (int a = 0; a < 3; a++) g_inv[a].push_back(inviscid_grid(b.e, a));
+        g_visc.push_back(viscous_grid(b.e));
+        g_prep.push_back(lu_prep_grid(b.e));
+    }
+    int rc = dev_upload(&ctx->d_blocks, rec);
+    if (!rc) rc = make_grid(ctx->bg_getpri, g_pri);
+    if (!rc) rc = make_grid(ctx->bg_inv[0], g_zero);
+    for (int a = 0; a < 3 && !rc; a++) rc = make_grid(ctx->bg_inv[a + 1], g_inv[a]);
+    if (!rc) rc = make_grid(ctx->bg_visc, g_visc);
+    if (!rc) rc = make_grid(ctx->bg_dt, g_pri);
+    if (!rc) rc = make_grid(ctx->bg_prep, g_prep);
+    return rc;
+}
+
+bool batched(const acfd_ctx *ctx) { return ctx->cfg.batch_blocks && ctx->d_blocks; }
 
 // Workspace of the tile sweeps, sized for the largest block and every tile
 // shape: the sweep-layout operand array (ordinary device memory), the
@@ -685,7 +755,11 @@ int st_implicit_all(acfd_ctx *ctx, const double *cfl_per_block)
         return 0;
     }
     StageTimer t(ctx, ACFD_STAGE_IMPLICIT1ST);
-    {
+    if (batched(ctx)) {
+        StageTimer ts(ctx, kTimeLuOther);
+        KRN_TRY(launch_lu_prep_batch(ctx->stream, ctx->d_blocks, ctx->bg_prep, ctx->arena,
+                                     cfl_per_block ? cfl_per_block : ctx->d_cfl, cfl_per_block ? 1 : 0, 0));
+    } else {
         StageTimer ts(ctx, kTimeLuOther);
         for (int m = 0; m < nb; m++) {
             const Block &b = ctx->blk[m];
@@ -771,8 +845,88 @@ int residual_stage(acfd_ctx *ctx, int m, int store_aux)
     return rc;
 }
 
+// A stage over every block: one launch per kernel with batch_blocks, else
+// block after block.  Blocks only meet at the halo refresh, so running a
+// stage for all blocks before the next stage equals running each block's
+// stages back to back.
+int st_getpri_all(acfd_ctx *ctx)
+{
+    if (!batched(ctx)) {
+        for (int m = 0; m < (int)ctx->blk.size(); m++) {
+            int rc = st_getpri(ctx, m);
+            if (rc) return rc;
+        }
+        return 0;
+    }
+    StageTimer t(ctx, ACFD_STAGE_GETPRI);
+    KRN_TRY(launch_getpri_batch(ctx->stream, ctx->d_blocks, ctx->bg_getpri, ctx->arena, ctx->c, ctx->d_nneg));
+    return 0;
+}
+
+int st_inviscid_all(acfd_ctx *ctx)
+{
+    // the batched launch covers the reference scheme without the sensor
+    if (!batched(ctx) || ctx->cfg.inviscid_scheme != 0) {
+        for (int m = 0; m < (int)ctx->blk.size(); m++) {
+            int rc = st_inviscid(ctx, m);
+            if (rc) return rc;
+        }
+        return 0;
+    }
+    StageTimer t(ctx, ACFD_STAGE_INVISCID);
+    KRN_TRY(launch_inviscid_batch(ctx->stream, ctx->d_blocks, ctx->bg_inv, ctx->arena, ctx->c, ctx->cfg.precision));
+    return 0;
+}
+
+int st_viscous_all(acfd_ctx *ctx, int store_aux)
+{
+    if (!batched(ctx)) {
+        for (int m = 0; m < (int)ctx->blk.size(); m++) {
+            int rc = st_viscous(ctx, m, store_aux);
+            if (rc) return rc;
+        }
+        return 0;
+    }
+    StageTimer t(ctx, ACFD_STAGE_VISCOUS);
+    KRN_TRY(launch_viscous_batch(ctx->stream, ctx->d_blocks, ctx->bg_visc, ctx->arena, ctx->c, ctx->cfg.precision,
+                                 store_aux));
+    return 0;
+}
+
+int st_find_dt_all(acfd_ctx *ctx)
+{
+    if (!batched(ctx)) {
+        for (int m = 0; m < (int)ctx->blk.size(); m++) {
+            int rc = st_find_dt(ctx, m);
+            if (rc) return rc;
+        }
+        return 0;
+    }
+    StageTimer t(ctx, ACFD_STAGE_FIND_DT);
+    if (ctx->c.itime == 0) KRN_TRY(launch_reset_stats(ctx->stream, ctx->d_stats));
+    KRN_TRY(launch_find_dt_batch(ctx->stream, ctx->d_blocks, (int)ctx->blk.size(), ctx->bg_dt, ctx->arena, ctx->c,
+                                 ctx->d_cfl, ctx->d_cfl_blk, ctx->d_stats));
+    return 0;
+}
+
+int residual_stage_all(acfd_ctx *ctx, int store_aux)
+{
+    int rc = st_getpri_all(ctx);
+    if (!rc) rc = st_inviscid_all(ctx);
+    if (!rc && ctx->c.solver == 0) rc = st_viscous_all(ctx, store_aux);
+    if (!rc) rc = st_find_dt_all(ctx);
+    return rc;
+}
+
 int launch_residual_all(acfd_ctx *ctx)
 {
+    if (batched(ctx)) {
+        const int nb = (int)ctx->blk.size();
+        KRN_TRY(launch_residual_batch(ctx->stream, ctx->d_blocks, nb, ctx->arena, ctx->d_partial,
+                                      ctx->c.turb == 2 && ctx->c.solver == 0));
+        KRN_TRY(launch_residual_final(ctx->stream, ctx->d_partial, nb * residual_partials(ctx->blk[0].e), ctx->d_res));
+        return 0;
+    }
     int parts = 0;
     for (auto &b : ctx->blk) {
         int p = residual_partials(b.e);
@@ -794,12 +948,24 @@ int one_step(acfd_ctx *ctx)
     // runs each block's find_dt, implicit1st and sa_model back to back)
     const double *cfl = ctx->c.itime == 0 ? nullptr : ctx->d_cfl_blk;
     auto cfl_of = [&](int m) { return cfl ? cfl + m : ctx->d_cfl; };
-    if (ctx->cfg.iflag_timeadvance == 2 && ctx->cfg.lusgs_kernel == 2) {
-        for (int m = 0; m < nb && !rc; m++) rc = residual_stage(ctx, m, aux);
+    const bool stagewise = ctx->cfg.batch_blocks || ctx->cfg.lusgs_kernel == 2;
+    if (ctx->cfg.iflag_timeadvance == 2 && stagewise) {
+        rc = residual_stage_all(ctx, aux);
         if (!rc) rc = st_implicit_all(ctx, cfl ? cfl : nullptr);
         for (int m = 0; m < nb && !rc && sa; m++) rc = st_sa(ctx, m, cfl_of(m));
         if (!rc) rc = refresh(ctx);
-        for (int m = 0; m < nb && !rc; m++) rc = st_getpri(ctx, m);
+        if (!rc) rc = st_getpri_all(ctx);
+    } else if (ctx->cfg.iflag_timeadvance == 3 && stagewise) {
+        for (int sub = 0; sub < 3 && !rc; sub++) {
+            rc = refresh(ctx);
+            if (!rc) rc = residual_stage_all(ctx, aux);
+            for (int m = 0; m < nb && !rc; m++) rc = st_crank(ctx, m);
+        }
+        for (int m = 0; m < nb && !rc; m++) rc = st_update_q0(ctx, m);
+        // after the sub-iterations the reference's cfl is the last block's
+        for (int m = 0; m < nb && !rc && sa; m++) rc = st_sa(ctx, m, ctx->d_cfl);
+        if (!rc) rc = refresh(ctx);
+        if (!rc) rc = st_getpri_all(ctx);
     } else if (ctx->cfg.iflag_timeadvance == 2) {
         for (int m = 0; m < nb && !rc; m++) {
             rc = residual_stage(ctx, m, aux);
@@ -829,6 +995,18 @@ int one_step(acfd_ctx *ctx)
         }
         for (int stage = 0; stage < 3 && !rc; stage++) {
             if (stage > 0) rc = refresh(ctx);
+            if (stagewise) {
+                if (!rc) rc = st_getpri_all(ctx);
+                if (!rc) rc = st_inviscid_all(ctx);
+                if (!rc && ctx->c.solver == 0) rc = st_viscous_all(ctx, 0);
+                if (!rc && stage == 0) rc = st_find_dt_all(ctx);
+                for (int m = 0; m < nb && !rc; m++) {
+                    const Block &b = ctx->blk[m];
+                    rc = launch_rk3_stage(ctx->stream, b.e, b.f, b.f.w_dqs, stage, ctx->c.solver == 0)
+                             ? fail(ACFD_EDEVICE, "rk3 stage launch failed") : 0;
+                }
+                continue;
+            }
             for (int m = 0; m < nb && !rc; m++) {
                 rc = st_getpri(ctx, m);
                 if (!rc) rc = st_inviscid(ctx, m);
@@ -842,7 +1020,7 @@ int one_step(acfd_ctx *ctx)
             }
         }
         if (!rc) rc = refresh(ctx);
-        for (int m = 0; m < nb && !rc; m++) rc = st_getpri(ctx, m);
+        if (!rc) rc = st_getpri_all(ctx);
     } else {
         return fail(ACFD_EINVAL, "iflag_timeadvance must be 2, 3 or 4");
     }
@@ -959,6 +1137,10 @@ int acfd_destroy(acfd_ctx *ctx)
     for (auto &b : ctx->lub) lusgs_batch_free(&b);
     if (ctx->lu_ctr) (void)hipFree(ctx->lu_ctr);
     if (ctx->d_cfl_blk) (void)hipFree(ctx->d_cfl_blk);
+    if (ctx->d_blocks) (void)hipFree(ctx->d_blocks);
+    for (BatchGrid *g : {&ctx->bg_getpri, &ctx->bg_inv[0], &ctx->bg_inv[1], &ctx->bg_inv[2], &ctx->bg_inv[3],
+                         &ctx->bg_visc, &ctx->bg_dt, &ctx->bg_prep})
+        free_grid(*g);
     void *sp[] = {ctx->s_w, ctx->s_m, ctx->s_v, ctx->s_grad, ctx->s_loss, ctx->s_part, ctx->s_lpart, ctx->s_net,
                   ctx->s_X, ctx->s_Y, ctx->s_stats, ctx->s_dpart};
     for (void *p : sp)
@@ -1131,6 +1313,7 @@ int acfd_set_topology(acfd_ctx *ctx, int nblk, const int *dims, const int *nsub,
         return rc;
     HIP_TRY(hipMalloc((void **)&ctx->d_partial, (size_t)parts * 6 * sizeof(double)));
     rc = alloc_lu_ws(ctx);
+    if (!rc) rc = build_batch(ctx);
     if (rc) return rc;
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     ctx->topo = true;

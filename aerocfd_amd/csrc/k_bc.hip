@@ -6,6 +6,7 @@
 // sub_bc_-30_symm.f90, sub_bc_-40_outflow.f90, sub_bc_-50_inflow.f90,
 // sub_bc_updatebuffer.f90:10-548, sub_bc_updatebufferaverage.f90:10-562,
 // sub_output_residual.f90:8-69.
+#include "batch.hpp"
 #include "common.hpp"
 #include "kernels.hpp"
 
@@ -258,18 +259,16 @@ int launch_average(hipStream_t s, const AvgList &al, double *const *q_tab, doubl
 // 6 sums per workgroup, reduced in a fixed order.
 constexpr int kResBlocks = 512;
 
-__global__ void __launch_bounds__(256) k_residual(Ext e, const double *__restrict__ rhs,
-                                                  const double *__restrict__ dt,
-                                                  const double *__restrict__ inv_j,
-                                                  const double *__restrict__ sa_rhs,
-                                                  double *__restrict__ partial, int with_sa)
+__device__ inline void residual_body(const Ext &e, int bx, const double *__restrict__ rhs,
+                                     const double *__restrict__ dt, const double *__restrict__ inv_j,
+                                     const double *__restrict__ sa_rhs, double *__restrict__ partial, int with_sa)
 {
     __shared__ double red[6][256];
     const long N = e.n;
     int Li = e.ie1 - e.is1 + 1, Lj = e.je1 - e.js1 + 1, Lk = e.ke1 - e.ks1 + 1;
     long total = (Li > 0 && Lj > 0 && Lk > 0) ? (long)Li * Lj * Lk : 0;
     double acc[6] = {0, 0, 0, 0, 0, 0};
-    for (long x = (long)blockIdx.x * blockDim.x + threadIdx.x; x < total; x += (long)gridDim.x * blockDim.x) {
+    for (long x = (long)bx * blockDim.x + threadIdx.x; x < total; x += (long)kResBlocks * blockDim.x) {
         int i = e.is1 + (int)(x % Li);
         int j = e.js1 + (int)((x / Li) % Lj);
         int k = e.ks1 + (int)(x / ((long)Li * Lj));
@@ -292,7 +291,28 @@ __global__ void __launch_bounds__(256) k_residual(Ext e, const double *__restric
             for (int v = 0; v < 6; v++) red[v][threadIdx.x] += red[v][threadIdx.x + o];
         __syncthreads();
     }
-    if (threadIdx.x < 6) partial[6 * blockIdx.x + threadIdx.x] = red[threadIdx.x][0];
+    if (threadIdx.x < 6) partial[6 * bx + threadIdx.x] = red[threadIdx.x][0];
+}
+
+__global__ void __launch_bounds__(256) k_residual(Ext e, const double *__restrict__ rhs,
+                                                  const double *__restrict__ dt,
+                                                  const double *__restrict__ inv_j,
+                                                  const double *__restrict__ sa_rhs,
+                                                  double *__restrict__ partial, int with_sa)
+{
+    residual_body(e, (int)blockIdx.x, rhs, dt, inv_j, sa_rhs, partial, with_sa);
+}
+
+// every block: workgroup x = m * kResBlocks + b writes block m's partial b
+// where the per-block launches would have put it
+__global__ void __launch_bounds__(256) k_residual_batch(const BlockRec *__restrict__ blks, double *arena,
+                                                        double *__restrict__ partial, int with_sa)
+{
+    const int m = (int)blockIdx.x / kResBlocks, bx = (int)blockIdx.x % kResBlocks;
+    const BlockRec &b = blks[m];
+    const Ext e = b.e;
+    residual_body(e, bx, arena + b.o.rhs, arena + b.o.dt, arena + b.o.inv_j, arena + b.o.sa_rhs,
+                  partial + 6L * kResBlocks * m, with_sa);
 }
 
 int residual_partials(const Ext &) { return kResBlocks; }
@@ -312,6 +332,14 @@ __global__ void k_residual_final(const double *__restrict__ partial, int count, 
     double s = 0.0;
     for (int x = 0; x < count; x++) s += partial[6 * x + v];
     out[v] = s;
+}
+
+int launch_residual_batch(hipStream_t s, const BlockRec *blks, int nblk, double *arena, double *partial, int with_sa)
+{
+    hipLaunchKernelGGL(k_residual_batch, dim3((unsigned)(nblk * kResBlocks)), dim3(256), 0, s, blks, arena, partial,
+                       with_sa);
+    ACFD_CHECK_LAUNCH();
+    return 0;
 }
 
 int launch_residual_final(hipStream_t s, const double *partial, int count, double *out)

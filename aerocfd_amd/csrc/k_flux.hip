@@ -8,6 +8,7 @@
 // The reconstruction and splitting are templated on the arithmetic type R:
 // R = double reproduces the reference in FP64, R = float is the FP32-flux
 // variant (inputs and the accumulated right-hand side stay FP64).
+#include "batch.hpp"
 #include "common.hpp"
 #include "flux_common.hpp"
 #include "kernels.hpp"
@@ -17,10 +18,10 @@ namespace acfd {
 // ------------------------------------------------------------------ getpri
 // sub_getpri_v.f90:33-59 over the whole extended block.  Counts nodes with
 // negative density into *nneg.
-__global__ void k_getpri(Ext e, const double *__restrict__ q, double *__restrict__ pri,
-                         double gamma, double cv, unsigned int *nneg)
+__device__ inline void getpri_body(const Ext &e, int bx, const double *__restrict__ q, double *__restrict__ pri,
+                                   double gamma, double cv, unsigned int *nneg)
 {
-    long n = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long n = (long)bx * blockDim.x + threadIdx.x;
     if (n >= e.n) return;
     const long N = e.n;
     double d = q[n];
@@ -40,11 +41,37 @@ __global__ void k_getpri(Ext e, const double *__restrict__ q, double *__restrict
     if (d < 0.0) atomicAdd(nneg, 1u);
 }
 
+__global__ void __launch_bounds__(256) k_getpri(Ext e, const double *__restrict__ q, double *__restrict__ pri,
+                                                double gamma, double cv, unsigned int *nneg)
+{
+    getpri_body(e, (int)blockIdx.x, q, pri, gamma, cv, nneg);
+}
+
+__global__ void __launch_bounds__(256) k_getpri_batch(const BlockRec *__restrict__ blks,
+                                                      const int *__restrict__ map, const KGrid *__restrict__ kg,
+                                                      double *arena, double gamma, double cv, unsigned int *nneg)
+{
+    int bx, by, bz;
+    const BlockRec &b = blks[batch_wg(map, kg, bx, by, bz)];
+    const Ext e = b.e;
+    getpri_body(e, bx, arena + b.o.q, arena + b.o.pri, gamma, cv, nneg);
+}
+
+dim3 grid_1d(long n) { return dim3((unsigned)((n + 255) / 256)); }
+
 int launch_getpri(hipStream_t s, const Ext &e, const Fields &f, const Consts &c, unsigned int *nneg)
 {
-    int bs = 256;
-    long nb = (e.n + bs - 1) / bs;
-    hipLaunchKernelGGL(k_getpri, dim3((unsigned)nb), dim3(bs), 0, s, e, f.q, f.pri, c.gamma, c.cv, nneg);
+    hipLaunchKernelGGL(k_getpri, grid_1d(e.n), dim3(256), 0, s, e, f.q, f.pri, c.gamma, c.cv, nneg);
+    ACFD_CHECK_LAUNCH();
+    return 0;
+}
+
+int launch_getpri_batch(hipStream_t s, const BlockRec *blks, const BatchGrid &bg, double *arena, const Consts &c,
+                        unsigned int *nneg)
+{
+    if (bg.total <= 0) return 0;
+    hipLaunchKernelGGL(k_getpri_batch, dim3((unsigned)bg.total), dim3(256), 0, s, blks, bg.map, bg.kg, arena,
+                       c.gamma, c.cv, nneg);
     ACFD_CHECK_LAUNCH();
     return 0;
 }
@@ -54,6 +81,17 @@ __global__ void k_fill(double *p, long n, double v)
 {
     long x = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (x < n) p[x] = v;
+}
+
+// rhsi = 0 on every block (1-D grid over 5n per block)
+__global__ void __launch_bounds__(256) k_zero_rhsi_batch(const BlockRec *__restrict__ blks,
+                                                         const int *__restrict__ map,
+                                                         const KGrid *__restrict__ kg, double *arena)
+{
+    int bx, by, bz;
+    const BlockRec &b = blks[batch_wg(map, kg, bx, by, bz)];
+    const long x = (long)bx * blockDim.x + threadIdx.x;
+    if (x < 5 * b.e.n) arena[b.o.rhsi + x] = 0.0;
 }
 
 int launch_fill(hipStream_t s, double *p, long n, double v)
@@ -101,15 +139,14 @@ __device__ inline void face_metric(const double *__restrict__ dxidx, const doubl
 // F(a-1/2) from lane L-1, so a wave produces 63 nodes.  rhsi = -(F+ - F-).
 constexpr int kInvI = 63;
 template <typename R>
-__global__ void __launch_bounds__(256) k_inviscid_i(Ext e, const double *__restrict__ q,
-                                                   const double *__restrict__ dxidx,
-                                                   const double *__restrict__ inv_j,
-                                                   double *__restrict__ rhsi, double gamma)
+__device__ inline void inviscid_i_body(const Ext &e, int bx, int by, int bz, const double *__restrict__ q,
+                                       const double *__restrict__ dxidx, const double *__restrict__ inv_j,
+                                       double *__restrict__ rhsi, double gamma)
 {
     const int lane = threadIdx.x;
-    const int j = e.js1 + (int)(blockIdx.y * blockDim.y + threadIdx.y);
-    const int k = e.ks1 + (int)blockIdx.z;
-    const int a_raw = e.is1 - 1 + (int)blockIdx.x * kInvI + lane;
+    const int j = e.js1 + (int)(by * blockDim.y + threadIdx.y);
+    const int k = e.ks1 + bz;
+    const int a_raw = e.is1 - 1 + bx * kInvI + lane;
     const bool row = j <= e.je1;
     // lanes past the range evaluate a valid face and discard it, so that
     // every lane takes part in the exchange
@@ -141,19 +178,27 @@ __global__ void __launch_bounds__(256) k_inviscid_i(Ext e, const double *__restr
     }
 }
 
+template <typename R>
+__global__ void __launch_bounds__(256) k_inviscid_i(Ext e, const double *__restrict__ q,
+                                                   const double *__restrict__ dxidx,
+                                                   const double *__restrict__ inv_j,
+                                                   double *__restrict__ rhsi, double gamma)
+{
+    inviscid_i_body<R>(e, (int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z, q, dxidx, inv_j, rhsi, gamma);
+}
+
 // The j and k directions (:160-243): each thread marches along the axis
 // over kInvRun nodes, keeping the four-value window and the previous face
 // flux in registers, so each face is evaluated once.  rhsi -= (F+ - F-).
 constexpr int kInvRun = 16;
 template <typename R, int AXIS>
-__global__ void __launch_bounds__(256) k_inviscid_jk(Ext e, const double *__restrict__ q,
-                                                    const double *__restrict__ dxidx,
-                                                    const double *__restrict__ inv_j,
-                                                    double *__restrict__ rhsi, double gamma)
+__device__ inline void inviscid_jk_body(const Ext &e, int bx, int by, int bz, const double *__restrict__ q,
+                                        const double *__restrict__ dxidx, const double *__restrict__ inv_j,
+                                        double *__restrict__ rhsi, double gamma)
 {
-    const int i = e.is1 + (int)(blockIdx.x * blockDim.x + threadIdx.x);
-    const int o = (int)(blockIdx.y * blockDim.y + threadIdx.y);  // the other transverse index
-    const int a0 = e.lo1(AXIS) + (int)blockIdx.z * kInvRun;
+    const int i = e.is1 + (int)(bx * blockDim.x + threadIdx.x);
+    const int o = (int)(by * blockDim.y + threadIdx.y);  // the other transverse index
+    const int a0 = e.lo1(AXIS) + bz * kInvRun;
     const int a1 = min(a0 + kInvRun - 1, e.hi1(AXIS));
     const int j = AXIS == 1 ? a0 : e.js1 + o;
     const int k = AXIS == 2 ? a0 : e.ks1 + o;
@@ -201,17 +246,80 @@ __global__ void __launch_bounds__(256) k_inviscid_jk(Ext e, const double *__rest
     }
 }
 
+template <typename R, int AXIS>
+__global__ void __launch_bounds__(256) k_inviscid_jk(Ext e, const double *__restrict__ q,
+                                                    const double *__restrict__ dxidx,
+                                                    const double *__restrict__ inv_j,
+                                                    double *__restrict__ rhsi, double gamma)
+{
+    inviscid_jk_body<R, AXIS>(e, (int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z, q, dxidx, inv_j, rhsi, gamma);
+}
+
+// AXIS 0 = the i kernel, 1 / 2 = the j / k kernels
+template <typename R, int AXIS>
+__global__ void __launch_bounds__(256) k_inviscid_batch(const BlockRec *__restrict__ blks,
+                                                        const int *__restrict__ map,
+                                                        const KGrid *__restrict__ kg, double *arena, double gamma)
+{
+    int bx, by, bz;
+    const BlockRec &b = blks[batch_wg(map, kg, bx, by, bz)];
+    const Ext e = b.e;
+    if (AXIS == 0)
+        inviscid_i_body<R>(e, bx, by, bz, arena + b.o.q, arena + b.o.dxidx, arena + b.o.inv_j, arena + b.o.rhsi,
+                           gamma);
+    else
+        inviscid_jk_body<R, AXIS>(e, bx, by, bz, arena + b.o.q, arena + b.o.dxidx, arena + b.o.inv_j,
+                                  arena + b.o.rhsi, gamma);
+}
+
+// per-block grids of the three inviscid kernels (64 x 4 threads)
+dim3 inviscid_grid(const Ext &e, int axis)
+{
+    const int Li = e.ie1 - e.is1 + 1, Lj = e.je1 - e.js1 + 1, Lk = e.ke1 - e.ks1 + 1;
+    if (Li <= 0 || Lj <= 0 || Lk <= 0) return dim3(0, 0, 0);
+    if (axis == 0) return dim3((Li + kInvI - 1) / kInvI, (Lj + 3) / 4, Lk);
+    if (axis == 1) return dim3((Li + 63) / 64, (Lk + 3) / 4, (Lj + kInvRun - 1) / kInvRun);
+    return dim3((Li + 63) / 64, (Lj + 3) / 4, (Lk + kInvRun - 1) / kInvRun);
+}
+
 template <typename R>
 static void launch_inviscid_t(hipStream_t s, const Ext &e, const Fields &f, double gamma)
 {
-    const int Li = e.ie1 - e.is1 + 1, Lj = e.je1 - e.js1 + 1, Lk = e.ke1 - e.ks1 + 1;
     dim3 t(64, 4, 1);
-    dim3 gi((Li + kInvI - 1) / kInvI, (Lj + 3) / 4, Lk);
-    hipLaunchKernelGGL((k_inviscid_i<R>), gi, t, 0, s, e, f.q, f.dxidx, f.inv_j, f.rhsi, gamma);
-    dim3 gj((Li + 63) / 64, (Lk + 3) / 4, (Lj + kInvRun - 1) / kInvRun);
-    hipLaunchKernelGGL((k_inviscid_jk<R, 1>), gj, t, 0, s, e, f.q, f.dxidx, f.inv_j, f.rhsi, gamma);
-    dim3 gk((Li + 63) / 64, (Lj + 3) / 4, (Lk + kInvRun - 1) / kInvRun);
-    hipLaunchKernelGGL((k_inviscid_jk<R, 2>), gk, t, 0, s, e, f.q, f.dxidx, f.inv_j, f.rhsi, gamma);
+    hipLaunchKernelGGL((k_inviscid_i<R>), inviscid_grid(e, 0), t, 0, s, e, f.q, f.dxidx, f.inv_j, f.rhsi, gamma);
+    hipLaunchKernelGGL((k_inviscid_jk<R, 1>), inviscid_grid(e, 1), t, 0, s, e, f.q, f.dxidx, f.inv_j, f.rhsi,
+                       gamma);
+    hipLaunchKernelGGL((k_inviscid_jk<R, 2>), inviscid_grid(e, 2), t, 0, s, e, f.q, f.dxidx, f.inv_j, f.rhsi,
+                       gamma);
+}
+
+// WENO3 + Roe residual of every block: bg[0] = rhsi zeroing (1-D over 5n),
+// bg[1..3] = the i, j, k kernels
+int launch_inviscid_batch(hipStream_t s, const BlockRec *blks, const BatchGrid *bg, double *arena,
+                          const Consts &c, int precision)
+{
+    dim3 t(64, 4, 1);
+    if (bg[0].total > 0)
+        hipLaunchKernelGGL(k_zero_rhsi_batch, dim3((unsigned)bg[0].total), dim3(256), 0, s, blks, bg[0].map,
+                           bg[0].kg, arena);
+    if (bg[1].total <= 0) return 0;
+    if (precision == 1) {
+        hipLaunchKernelGGL((k_inviscid_batch<float, 0>), dim3((unsigned)bg[1].total), t, 0, s, blks, bg[1].map,
+                           bg[1].kg, arena, c.gamma);
+        hipLaunchKernelGGL((k_inviscid_batch<float, 1>), dim3((unsigned)bg[2].total), t, 0, s, blks, bg[2].map,
+                           bg[2].kg, arena, c.gamma);
+        hipLaunchKernelGGL((k_inviscid_batch<float, 2>), dim3((unsigned)bg[3].total), t, 0, s, blks, bg[3].map,
+                           bg[3].kg, arena, c.gamma);
+    } else {
+        hipLaunchKernelGGL((k_inviscid_batch<double, 0>), dim3((unsigned)bg[1].total), t, 0, s, blks, bg[1].map,
+                           bg[1].kg, arena, c.gamma);
+        hipLaunchKernelGGL((k_inviscid_batch<double, 1>), dim3((unsigned)bg[2].total), t, 0, s, blks, bg[2].map,
+                           bg[2].kg, arena, c.gamma);
+        hipLaunchKernelGGL((k_inviscid_batch<double, 2>), dim3((unsigned)bg[3].total), t, 0, s, blks, bg[3].map,
+                           bg[3].kg, arena, c.gamma);
+    }
+    ACFD_CHECK_LAUNCH();
+    return 0;
 }
 
 int launch_inviscid(hipStream_t s, const Ext &e, const Fields &f, const Consts &c, int precision)
@@ -240,17 +348,15 @@ __device__ inline double ndiff(const double *a, long n, long st, int x, int x0, 
 // temperature gradients, Sutherland viscosity, stress, heat flux and the
 // contravariant node fluxes fv/gv/hv (components 2..5) into w (12 fields).
 template <typename R>
-__global__ void __launch_bounds__(256) k_viscous_node(Ext e, Box b, Consts c,
-                                                      const double *__restrict__ pri,
-                                                      double *__restrict__ amu,
-                                                      const double *__restrict__ dxidx,
-                                                      const double *__restrict__ inv_j,
-                                                      double *__restrict__ w, double *__restrict__ vor,
-                                                      double *__restrict__ dudx,
-                                                      double *__restrict__ tao, int store_aux)
+__device__ inline void viscous_node_body(const Ext &e, int wx, int wy, int wz, const Consts &c,
+                                         const double *__restrict__ pri, double *__restrict__ amu,
+                                         const double *__restrict__ dxidx, const double *__restrict__ inv_j,
+                                         double *__restrict__ w, double *__restrict__ vor, double *__restrict__ dudx,
+                                         double *__restrict__ tao, int store_aux)
 {
+    const Box b{e.is0, e.ie0, e.js0, e.je0, e.ks0, e.ke0};
     int i, j, k;
-    if (!box_node(b, i, j, k)) return;
+    if (!box_node_at(b, wx, wy, wz, i, j, k)) return;
     const long N = e.n;
     long n = e.ix(i, j, k);
     const long sj = e.ni, sk = (long)e.ni * e.nj;
@@ -316,12 +422,24 @@ __global__ void __launch_bounds__(256) k_viscous_node(Ext e, Box b, Consts c,
     }
 }
 
-// Stage 2 (sub_viscous.f90:296-365): rhsv = (d/dxi fv + d/deta gv + d/dzeta hv)/re
-__global__ void __launch_bounds__(256) k_viscous_diff(Ext e, Box b, const double *__restrict__ w,
-                                                      double *__restrict__ rhsv, double re)
+template <typename R>
+__global__ void __launch_bounds__(256) k_viscous_node(Ext e, Consts c, const double *__restrict__ pri,
+                                                      double *__restrict__ amu, const double *__restrict__ dxidx,
+                                                      const double *__restrict__ inv_j, double *__restrict__ w,
+                                                      double *__restrict__ vor, double *__restrict__ dudx,
+                                                      double *__restrict__ tao, int store_aux)
 {
+    viscous_node_body<R>(e, (int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z, c, pri, amu, dxidx, inv_j, w, vor,
+                         dudx, tao, store_aux);
+}
+
+// Stage 2 (sub_viscous.f90:296-365): rhsv = (d/dxi fv + d/deta gv + d/dzeta hv)/re
+__device__ inline void viscous_diff_body(const Ext &e, int bx, int by, int bz, const double *__restrict__ w,
+                                         double *__restrict__ rhsv, double re)
+{
+    const Box b{e.is0, e.ie0, e.js0, e.je0, e.ks0, e.ke0};
     int i, j, k;
-    if (!box_node(b, i, j, k)) return;
+    if (!box_node_at(b, bx, by, bz, i, j, k)) return;
     const long N = e.n;
     long n = e.ix(i, j, k);
     const long sj = e.ni, sk = (long)e.ni * e.nj;
@@ -335,18 +453,63 @@ __global__ void __launch_bounds__(256) k_viscous_diff(Ext e, Box b, const double
     }
 }
 
+__global__ void __launch_bounds__(256) k_viscous_diff(Ext e, const double *__restrict__ w,
+                                                      double *__restrict__ rhsv, double re)
+{
+    viscous_diff_body(e, (int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z, w, rhsv, re);
+}
+
+template <typename R>
+__global__ void __launch_bounds__(256) k_viscous_node_batch(const BlockRec *__restrict__ blks,
+                                                            const int *__restrict__ map,
+                                                            const KGrid *__restrict__ kg, double *arena, Consts c,
+                                                            int store_aux)
+{
+    int bx, by, bz;
+    const BlockRec &b = blks[batch_wg(map, kg, bx, by, bz)];
+    const Ext e = b.e;
+    viscous_node_body<R>(e, bx, by, bz, c, arena + b.o.pri, arena + b.o.amu, arena + b.o.dxidx, arena + b.o.inv_j,
+                         arena + b.o.w_flux, arena + b.o.vor, arena + b.o.dudx, arena + b.o.tao, store_aux);
+}
+
+__global__ void __launch_bounds__(256) k_viscous_diff_batch(const BlockRec *__restrict__ blks,
+                                                            const int *__restrict__ map,
+                                                            const KGrid *__restrict__ kg, double *arena, double re)
+{
+    int bx, by, bz;
+    const BlockRec &b = blks[batch_wg(map, kg, bx, by, bz)];
+    const Ext e = b.e;
+    viscous_diff_body(e, bx, by, bz, arena + b.o.w_flux, arena + b.o.rhsv, re);
+}
+
+// per-block grid of both viscous kernels: the physical range, 64 x 4 threads
+dim3 viscous_grid(const Ext &e) { return box_grid(Box{e.is0, e.ie0, e.js0, e.je0, e.ks0, e.ke0}); }
+
 int launch_viscous(hipStream_t s, const Ext &e, const Fields &f, const Consts &c, int precision,
                    int store_aux)
 {
-    Box b{e.is0, e.ie0, e.js0, e.je0, e.ks0, e.ke0};
-    dim3 g = box_grid(b), t(64, 4, 1);
+    dim3 g = viscous_grid(e), t(64, 4, 1);
     if (precision == 1)
-        hipLaunchKernelGGL((k_viscous_node<float>), g, t, 0, s, e, b, c, f.pri, f.amu, f.dxidx, f.inv_j,
-                           f.w_flux, f.vor, f.dudx, f.tao, store_aux);
+        hipLaunchKernelGGL((k_viscous_node<float>), g, t, 0, s, e, c, f.pri, f.amu, f.dxidx, f.inv_j, f.w_flux,
+                           f.vor, f.dudx, f.tao, store_aux);
     else
-        hipLaunchKernelGGL((k_viscous_node<double>), g, t, 0, s, e, b, c, f.pri, f.amu, f.dxidx, f.inv_j,
-                           f.w_flux, f.vor, f.dudx, f.tao, store_aux);
-    hipLaunchKernelGGL(k_viscous_diff, g, t, 0, s, e, b, f.w_flux, f.rhsv, c.re);
+        hipLaunchKernelGGL((k_viscous_node<double>), g, t, 0, s, e, c, f.pri, f.amu, f.dxidx, f.inv_j, f.w_flux,
+                           f.vor, f.dudx, f.tao, store_aux);
+    hipLaunchKernelGGL(k_viscous_diff, g, t, 0, s, e, f.w_flux, f.rhsv, c.re);
+    ACFD_CHECK_LAUNCH();
+    return 0;
+}
+
+int launch_viscous_batch(hipStream_t s, const BlockRec *blks, const BatchGrid &bg, double *arena, const Consts &c,
+                         int precision, int store_aux)
+{
+    if (bg.total <= 0) return 0;
+    dim3 g((unsigned)bg.total), t(64, 4, 1);
+    if (precision == 1)
+        hipLaunchKernelGGL((k_viscous_node_batch<float>), g, t, 0, s, blks, bg.map, bg.kg, arena, c, store_aux);
+    else
+        hipLaunchKernelGGL((k_viscous_node_batch<double>), g, t, 0, s, blks, bg.map, bg.kg, arena, c, store_aux);
+    hipLaunchKernelGGL(k_viscous_diff_batch, g, t, 0, s, blks, bg.map, bg.kg, arena, c.re);
     ACFD_CHECK_LAUNCH();
     return 0;
 }
@@ -377,11 +540,12 @@ __device__ inline double local_rate(const Ext &e, long n, const double *pri, con
     return txi * t1 + teta * t2 + tzeta * t3;
 }
 
-__global__ void k_find_dt_steady(Ext e, const double *__restrict__ pri, const double *__restrict__ amu,
-                                 const double *__restrict__ dxidx, double *__restrict__ dt,
-                                 const double *cfl, double re, unsigned long long *stats)
+__device__ inline void find_dt_steady_body(const Ext &e, int bx, const double *__restrict__ pri,
+                                           const double *__restrict__ amu, const double *__restrict__ dxidx,
+                                           double *__restrict__ dt, const double *cfl, double re,
+                                           unsigned long long *stats)
 {
-    long n = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long n = (long)bx * blockDim.x + threadIdx.x;
     double v = 0.0;
     bool ok = n < e.n;
     if (ok) {
@@ -396,10 +560,34 @@ __global__ void k_find_dt_steady(Ext e, const double *__restrict__ pri, const do
         mx = a > mx ? a : mx;
         mn = c < mn ? c : mn;
     }
-    if ((threadIdx.x & 63) == 0) {
+    if ((threadIdx.x & 63) == 0 && stats) {
         atomicMax(stats, mx);
         atomicMin(stats + 1, mn);
     }
+}
+
+__global__ void __launch_bounds__(256) k_find_dt_steady(Ext e, const double *__restrict__ pri,
+                                                        const double *__restrict__ amu,
+                                                        const double *__restrict__ dxidx, double *__restrict__ dt,
+                                                        const double *cfl, double re, unsigned long long *stats)
+{
+    find_dt_steady_body(e, (int)blockIdx.x, pri, amu, dxidx, dt, cfl, re, stats);
+}
+
+// every block; the dt statistics come from the last block, as when the
+// blocks run one after another with the statistics reset before each
+__global__ void __launch_bounds__(256) k_find_dt_steady_batch(const BlockRec *__restrict__ blks,
+                                                              const int *__restrict__ map,
+                                                              const KGrid *__restrict__ kg, double *arena,
+                                                              int nblk, const double *cfl, double re,
+                                                              unsigned long long *stats)
+{
+    int bx, by, bz;
+    const int m = batch_wg(map, kg, bx, by, bz);
+    const BlockRec &b = blks[m];
+    const Ext e = b.e;
+    find_dt_steady_body(e, bx, arena + b.o.pri, arena + b.o.amu, arena + b.o.dxidx, arena + b.o.dt, cfl, re,
+                        m == nblk - 1 ? stats : nullptr);
 }
 
 __global__ void k_find_dt_unsteady(Ext e, const double *__restrict__ pri, const double *__restrict__ amu,
@@ -412,6 +600,21 @@ __global__ void k_find_dt_unsteady(Ext e, const double *__restrict__ pri, const 
     if (cfl_blk) *cfl_blk = v;
 }
 
+// unsteady, every block: thread m writes block m's cfl to cfl_blk[m]; the
+// global cfl ends as the last block's, as after the blocks in order
+__global__ void k_find_dt_unsteady_batch(const BlockRec *__restrict__ blks, double *arena, int nblk,
+                                         double *cfl, double *cfl_blk, double re)
+{
+    const int m = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (m >= nblk) return;
+    const BlockRec &b = blks[m];
+    const Ext e = b.e;
+    const long n = e.n - 1;
+    const double v = arena[b.o.dt + n] * local_rate(e, n, arena + b.o.pri, arena + b.o.amu, arena + b.o.dxidx, re);
+    cfl_blk[m] = v;
+    if (m == nblk - 1) *cfl = v;
+}
+
 __global__ void k_reset_stats(unsigned long long *st)
 {
     st[0] = 0ull;
@@ -421,6 +624,21 @@ __global__ void k_reset_stats(unsigned long long *st)
 int launch_reset_stats(hipStream_t s, unsigned long long *stats)
 {
     hipLaunchKernelGGL(k_reset_stats, dim3(1), dim3(1), 0, s, stats);
+    ACFD_CHECK_LAUNCH();
+    return 0;
+}
+
+int launch_find_dt_batch(hipStream_t s, const BlockRec *blks, int nblk, const BatchGrid &bg, double *arena,
+                         const Consts &c, double *cfl, double *cfl_blk, unsigned long long *stats)
+{
+    if (c.itime == 0) {
+        if (bg.total > 0)
+            hipLaunchKernelGGL(k_find_dt_steady_batch, dim3((unsigned)bg.total), dim3(256), 0, s, blks, bg.map, bg.kg,
+                               arena, nblk, cfl, c.re, stats);
+    } else {
+        hipLaunchKernelGGL(k_find_dt_unsteady_batch, dim3((unsigned)((nblk + 63) / 64)), dim3(64), 0, s, blks,
+                           arena, nblk, cfl, cfl_blk, c.re);
+    }
     ACFD_CHECK_LAUNCH();
     return 0;
 }

@@ -49,6 +49,7 @@
 #include <cstdlib>
 #include <vector>
 
+#include "batch.hpp"
 #include "common.hpp"
 #include "kernels.hpp"
 
@@ -152,17 +153,16 @@ __device__ inline void lu_backward(const LuArgs &a, long n, const double *vi, co
 // rhs = rhsi + rhsv on the computed range; spectral radii ra, rb, rc and
 // diag on the ring is1-1..ie1+1 (:72-156); with clear_dq, dq* and dq cleared
 // on the ring so that out-of-range neighbours read 0 (:105-106).
-__global__ void __launch_bounds__(256) k_lu_prep(Ext e, Box ring, const double *__restrict__ pri,
-                                                 const double *__restrict__ dxidx,
-                                                 const double *__restrict__ dt,
-                                                 const double *__restrict__ rhsi,
-                                                 const double *__restrict__ rhsv,
-                                                 double *__restrict__ rhs, double *__restrict__ rr,
-                                                 double *__restrict__ diag, double *__restrict__ dqs,
-                                                 double *__restrict__ dq, const double *cfl, int clear_dq)
+__device__ inline void lu_prep_body(const Ext &e, int bx, int by, int bz, const double *__restrict__ pri,
+                                    const double *__restrict__ dxidx, const double *__restrict__ dt,
+                                    const double *__restrict__ rhsi, const double *__restrict__ rhsv,
+                                    double *__restrict__ rhs, double *__restrict__ rr, double *__restrict__ diag,
+                                    double *__restrict__ dqs, double *__restrict__ dq, const double *cfl,
+                                    int clear_dq)
 {
+    const Box ring{e.is1 - 1, e.ie1 + 1, e.js1 - 1, e.je1 + 1, e.ks1 - 1, e.ke1 + 1};
     int i, j, k;
-    if (!box_node(ring, i, j, k)) return;
+    if (!box_node_at(ring, bx, by, bz, i, j, k)) return;
     const long N = e.n;
     long n = e.ix(i, j, k);
     double phi = *cfl >= 1.0e3 ? 0.0 : 1.0;  // :98-102
@@ -187,6 +187,44 @@ __global__ void __launch_bounds__(256) k_lu_prep(Ext e, Box ring, const double *
         }
         if (inner) rhs[x * N + n] = rhsi[x * N + n] + rhsv[x * N + n];
     }
+}
+
+__global__ void __launch_bounds__(256) k_lu_prep(Ext e, const double *__restrict__ pri,
+                                                 const double *__restrict__ dxidx, const double *__restrict__ dt,
+                                                 const double *__restrict__ rhsi, const double *__restrict__ rhsv,
+                                                 double *__restrict__ rhs, double *__restrict__ rr,
+                                                 double *__restrict__ diag, double *__restrict__ dqs,
+                                                 double *__restrict__ dq, const double *cfl, int clear_dq)
+{
+    lu_prep_body(e, (int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z, pri, dxidx, dt, rhsi, rhsv, rhs, rr, diag,
+                 dqs, dq, cfl, clear_dq);
+}
+
+// every block; block m reads cfl[m * cfl_stride]
+__global__ void __launch_bounds__(256) k_lu_prep_batch(const BlockRec *__restrict__ blks,
+                                                       const int *__restrict__ map, const KGrid *__restrict__ kg,
+                                                       double *arena, const double *cfl, int cfl_stride,
+                                                       int clear_dq)
+{
+    int bx, by, bz;
+    const int m = batch_wg(map, kg, bx, by, bz);
+    const BlockRec &b = blks[m];
+    const Ext e = b.e;
+    lu_prep_body(e, bx, by, bz, arena + b.o.pri, arena + b.o.dxidx, arena + b.o.dt, arena + b.o.rhsi,
+                 arena + b.o.rhsv, arena + b.o.rhs, arena + b.o.w_r, arena + b.o.w_diag, arena + b.o.w_dqs,
+                 arena + b.o.w_dq, cfl + (long)m * cfl_stride, clear_dq);
+}
+
+dim3 lu_prep_grid(const Ext &e) { return box_grid(Box{e.is1 - 1, e.ie1 + 1, e.js1 - 1, e.je1 + 1, e.ks1 - 1, e.ke1 + 1}); }
+
+int launch_lu_prep_batch(hipStream_t s, const BlockRec *blks, const BatchGrid &bg, double *arena, const double *cfl,
+                         int cfl_stride, int clear_dq)
+{
+    if (bg.total <= 0) return 0;
+    hipLaunchKernelGGL(k_lu_prep_batch, dim3((unsigned)bg.total), dim3(64, 4, 1), 0, s, blks, bg.map, bg.kg, arena,
+                       cfl, cfl_stride, clear_dq);
+    ACFD_CHECK_LAUNCH();
+    return 0;
 }
 
 // One hyperplane l of the forward (DIR = +1) or backward (DIR = -1) sweep.
@@ -945,9 +983,8 @@ int launch_lusgs_phase(hipStream_t s, const Ext &e, const Fields &f, const Const
     const bool tiles = variant == 1 && ws && ws->sl && ws->ex;
     const bool empty = Li <= 0 || Lj <= 0 || Lk <= 0;
     if (phase == 0) {
-        Box ring{e.is1 - 1, e.ie1 + 1, e.js1 - 1, e.je1 + 1, e.ks1 - 1, e.ke1 + 1};
-        hipLaunchKernelGGL(k_lu_prep, box_grid(ring), dim3(64, 4, 1), 0, s, e, ring, f.pri, f.dxidx, f.dt,
-                           f.rhsi, f.rhsv, f.rhs, f.w_r, f.w_diag, f.w_dqs, f.w_dq, cfl, tiles ? 0 : 1);
+        hipLaunchKernelGGL(k_lu_prep, lu_prep_grid(e), dim3(64, 4, 1), 0, s, e, f.pri, f.dxidx, f.dt, f.rhsi,
+                           f.rhsv, f.rhs, f.w_r, f.w_diag, f.w_dqs, f.w_dq, cfl, tiles ? 0 : 1);
     } else if (empty) {
         return 0;
     } else if (tiles) {

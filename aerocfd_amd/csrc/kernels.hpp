@@ -1,5 +1,6 @@
 // Host-side launchers of the device kernels (one translation unit each).
 #pragma once
+#include "batch.hpp"
 #include "common.hpp"
 
 namespace acfd {
@@ -43,6 +44,24 @@ long diff_partials(const Ext &e);
 int launch_diff_partial(hipStream_t s, const Ext &e, const double *a, const double *b, double *part);
 int launch_inviscid_wcns_sensor(hipStream_t s, const Ext &e, const Fields &f, const Consts &c, int precision,
                                 const SensorArgs &sa);
+
+// batched launches over every block (block table, per-kind grids)
+dim3 grid_1d(long n);                    // 256-thread workgroups over n items
+dim3 inviscid_grid(const Ext &e, int axis);
+dim3 viscous_grid(const Ext &e);
+dim3 lu_prep_grid(const Ext &e);
+int launch_getpri_batch(hipStream_t s, const BlockRec *blks, const BatchGrid &bg, double *arena, const Consts &c,
+                        unsigned int *nneg);
+int launch_inviscid_batch(hipStream_t s, const BlockRec *blks, const BatchGrid *bg, double *arena,
+                          const Consts &c, int precision);
+int launch_viscous_batch(hipStream_t s, const BlockRec *blks, const BatchGrid &bg, double *arena, const Consts &c,
+                         int precision, int store_aux);
+int launch_find_dt_batch(hipStream_t s, const BlockRec *blks, int nblk, const BatchGrid &bg, double *arena,
+                         const Consts &c, double *cfl, double *cfl_blk, unsigned long long *stats);
+int launch_lu_prep_batch(hipStream_t s, const BlockRec *blks, const BatchGrid &bg, double *arena, const double *cfl,
+                         int cfl_stride, int clear_dq);
+int launch_residual_batch(hipStream_t s, const BlockRec *blks, int nblk, double *arena, double *partial,
+                          int with_sa);
 
 // k_lusgs.hip
 // workspace of the tile sweeps: sweep-layout operands (normal memory),

@@ -66,6 +66,7 @@ class FlowConfig:
     lusgs_kernel: int = 0        # 0 hyperplane launches, 1 pipelined tiles
     inviscid_scheme: int = 0     # 0 WENO3 + Roe (reference); 1 Steger-Warming + WENO5, 2 WCNS
     use_graph: int = 0           # 1 replay one captured HIP graph per step
+    batch_blocks: int = 0        # 1 one launch per stage over all blocks
 
     # non-dimensional freestream, sub_init_flowfield.f90:30-39
     @property

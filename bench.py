@@ -131,6 +131,7 @@ def main():
                     help="replay one captured HIP graph per step in the timed region (per-kernel times then "
                          "come from two eager steps after it)")
     ap.add_argument("--cuts", type=int, default=1, help="blocks per axis (the workload split into cuts^3 blocks)")
+    ap.add_argument("--batch", action="store_true", help="one launch per stage over all blocks")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB (a known-size kernel that "
@@ -158,6 +159,7 @@ def main():
     case.cfg.precision = args.precision
     case.cfg.lusgs_kernel = args.lusgs
     case.cfg.use_graph = 1 if args.graph else 0
+    case.cfg.batch_blocks = 1 if args.batch else 0
     sol = Solver(case.cfg, case.dims, case.subfaces)
     case.upload(sol)
     del case.fields
@@ -198,6 +200,8 @@ def main():
     # every block for the batched LU-SGS phases
     comp_all = sum((r[13] - r[12] + 1) * (r[15] - r[14] + 1) * (r[17] - r[16] + 1) for r in rr)
     batched = {"lusgs_forward", "lusgs_backward", "lusgs_gather", "lusgs_add"} if args.lusgs == 2 else set()
+    if args.batch:
+        batched |= {"lusgs_other", "getpri", "inviscid", "viscous", "find_dt"}
     table = dict(BYTES_PER_NODE)
     if args.lusgs == 0:
         table.update(HYPERPLANE_BYTES)
@@ -222,7 +226,7 @@ def main():
         "dtype": "f64" if args.precision == 0 else "f32-flux/f64-update",
         "data": "synthetic",
         "config": {"workload": "tgv_%d^3_periodic_laminar_ns_weno3_roe_lusgs" % args.n, "nodes": nodes,
-                   "blocks": len(case.dims), "graph": bool(args.graph), "mach": 0.1, "re": 1600, "scheme": "WENO3+Roe, central viscous, LU-SGS",
+                   "blocks": len(case.dims), "graph": bool(args.graph), "batch_blocks": bool(args.batch), "mach": 0.1, "re": 1600, "scheme": "WENO3+Roe, central viscous, LU-SGS",
                    "lusgs_kernel": {0: "hyperplanes", 1: "pipelined-tiles", 2: "batched-pipelined-tiles"}[args.lusgs]},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": PEAK_HBM_GBS,
                      "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4),

@@ -54,7 +54,8 @@ typedef struct acfd_config {
                                        in the reference: 1 = Steger-Warming splitting + WENO5,
                                        2 = WCNS (WENO5 midpoint interpolation + Roe, 6th-order difference) */
     int use_graph;                  /* 1 = acfd_step replays one captured HIP graph per step (same results) */
-    int reserved[3];
+    int batch_blocks;               /* 1 = each stage is one launch over all blocks (same results) */
+    int reserved[2];
 } acfd_config;
 
 /* One subface record, as read from bcinfo (sub_init_readbcinfo.f90:96-98):
