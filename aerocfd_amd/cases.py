@@ -391,3 +391,121 @@ def shock_vortex(ni=96, nj=48, nk=4, mach=1.5, seed=4, cfg=None):
     dxidx, inv_j, al, be, ga = uniform_metrics(rng, (h, h, h))
     _common_fields(case, 0, q, dxidx, inv_j, al, be, ga)
     return case
+
+
+NUT_INF = 1.341946  # freestream SA working variable (module_tursa.f90:11)
+
+
+def tanh_stretch(n, height, first):
+    """Node heights y_1..y_n of the one-sided tanh clustering
+    y = H (1 - tanh(d (1 - s)) / tanh d), s = (j - 1)/(n - 1), with d chosen
+    (bisection) so that y_2 - y_1 = first.  Returns (d, y(s), dy/dj(s))."""
+    def first_of(d):
+        s = 1.0 / (n - 1)
+        return height * (1.0 - math.tanh(d * (1.0 - s)) / math.tanh(d))
+    lo, hi = 1e-6, 30.0
+    if first_of(hi) > first:
+        lo = hi
+    for _ in range(200):
+        mid = 0.5 * (lo + hi)
+        if first_of(mid) > first:
+            lo = mid
+        else:
+            hi = mid
+    d = 0.5 * (lo + hi)
+
+    def y(s):
+        return height * (1.0 - np.tanh(d * (1.0 - s)) / math.tanh(d))
+
+    def dyds(s):
+        t = np.tanh(d * (1.0 - s))
+        return height * d * (1.0 - t * t) / math.tanh(d) / (n - 1)
+    return d, y, dyds
+
+
+def flat_plate(seed=3, nblk=(4, 3, 1), lo=(32, 32, 32), hi=(160, 96, 64), length=(1.0, 0.25, 0.1),
+               first=5.4e-6, cfg=None):
+    """Batched multi-block turbulent flat plate: a box of nblk[0] x nblk[1] x
+    nblk[2] point-matched blocks whose node counts per axis are drawn from a
+    seeded generator in [lo, hi]; x uniform over the plate length (inflow at
+    x = 0, outflow at x = length[0]), y tanh-clustered to the wall y = 0 with
+    first spacing `first` (y+ ~ 1 at Re_L = 5e6), z uniform (symmetry faces).
+    Mach 0.3, Re_L = 5e6, SA, local time step, first-order implicit; the
+    reference's native uniform start (sub_init_flowfield.f90:69-110).
+    Metrics are the exact ones of the separable mapping; wall distance is y;
+    spacing follows sub_init_spacing.f90 (largest forward edge length on the
+    physical range, last node copied, zero on halos)."""
+    from .solver import FlowConfig
+    rng = np.random.default_rng(seed)
+    sizes = [[int(rng.integers(lo[a], hi[a] + 1)) for _ in range(nblk[a])] for a in range(3)]
+    cuts, tot = [], []
+    for a in range(3):
+        c, pos = [], 1
+        for s in sizes[a][:-1]:
+            pos += s - 1
+            c.append(pos)
+        cuts.append(tuple(c))
+        tot.append(sum(sizes[a]) - (nblk[a] - 1))
+    NI, NJ, NK = tot
+    if cfg is None:
+        cfg = FlowConfig(ma=0.3, re=5.0e6, iflag_solver=0, iflag_turbulence=2, iflag_des=0, iflag_time=0,
+                         iflag_timeadvance=2, cfl=5.0, dt=1.0e-3, prl=0.72)
+    dims, subs, origins = split_box(NI, NJ, NK, cuts, phys=(-50, -40, -20, -10, -30, -30))
+    case = Case("flat_plate_%dblk_%dx%dx%d" % (len(dims), NI, NJ, NK), cfg, dims, subs)
+    hx, hz = length[0] / (NI - 1), length[2] / (NK - 1)
+    _, ymap, dymap = tanh_stretch(NJ, length[1], first)
+    g = GAMMA
+    cv = 1.0 / g / (g - 1.0) / cfg.ma ** 2
+    aoa = math.radians(cfg.aoa_deg)
+    for m in range(len(dims)):
+        r = case.ranges[m]
+        org, _ = origins[m]
+        shp = case.shape(m)
+        # global node indices of the extended range (halos lie inside the box)
+        gi = org[0] + np.arange(r[0], r[1] + 1) - 1
+        gj = org[1] + np.arange(r[2], r[3] + 1) - 1
+        gk = org[2] + np.arange(r[4], r[5] + 1) - 1
+        K, J, I = np.meshgrid(gk, gj, gi, indexing="ij")
+        sj = (J - 1.0) / (NJ - 1)
+        x = (I - 1.0) * hx
+        y = ymap(sj)
+        z = (K - 1.0) * hz
+        yj = dymap(sj)
+        dxidx = np.zeros(shp + (9,))
+        dxidx[..., 0] = 1.0 / hx
+        dxidx[..., 4] = 1.0 / yj
+        dxidx[..., 8] = 1.0 / hz
+        inv_j = hx * yj * hz
+        al = np.zeros(shp + (3,))
+        be = np.zeros(shp + (3,))
+        ga = np.zeros(shp + (3,))
+        al[..., 0] = inv_j / (hx * hx)
+        be[..., 1] = inv_j / (yj * yj)
+        ga[..., 2] = inv_j / (hz * hz)
+        q = np.zeros(shp + (5,))
+        q[..., 0] = 1.0
+        q[..., 1] = math.cos(aoa)
+        q[..., 2] = math.sin(aoa)
+        q[..., 4] = cv + 0.5
+        _common_fields(case, m, q, dxidx, inv_j, al, be, ga)
+        fl = case.fields[m]
+        amu = fl["amu"]
+        amu[..., 0] = 1.0
+        amu[..., 1] = 0.009
+        amu[..., 2] = 1.009
+        fl["nut"] = np.full(shp, NUT_INF)
+        fl["dist"] = y.copy()
+        # sub_init_spacing.f90:39-104 on the physical range
+        sp = np.zeros(shp)
+        p0 = (r[6] - r[0], r[7] - r[0] + 1, r[8] - r[2], r[9] - r[2] + 1, r[10] - r[4], r[11] - r[4] + 1)
+        xs, ys, zs = x[p0[4]:p0[5], p0[2]:p0[3], p0[0]:p0[1]], y[p0[4]:p0[5], p0[2]:p0[3], p0[0]:p0[1]], \
+            z[p0[4]:p0[5], p0[2]:p0[3], p0[0]:p0[1]]
+        spc = []
+        for ax in (2, 1, 0):  # i, j, k edges (arrays are k, j, i)
+            d = [np.diff(c, axis=ax) for c in (xs, ys, zs)]
+            e = np.sqrt(d[0] ** 2 + d[1] ** 2 + d[2] ** 2)
+            last = np.take(e, [-1], axis=ax)
+            spc.append(np.concatenate([e, last], axis=ax))
+        sp[p0[4]:p0[5], p0[2]:p0[3], p0[0]:p0[1]] = np.maximum(np.maximum(spc[0], spc[1]), spc[2])
+        fl["spacing"] = sp
+    return case

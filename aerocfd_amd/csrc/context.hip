@@ -106,6 +106,7 @@ struct acfd_ctx {
     LuWs lu{nullptr, nullptr, nullptr};  // tile-sweep workspace (lusgs_kernel 1, 2)
     long lu_ex_n = 0;
     LuBatch lub[4];          // lusgs_kernel 2: every block in one launch, per tile shape
+    LuBatch sab[4];          // the same for the SA model's scalar sweeps
     int lub_shape = -1;      // chosen by the autotuner on first use
     int *lu_ctr = nullptr;   // ticket counters of the batched sweeps
     double *d_cfl_blk = nullptr;  // cfl after each block's find_dt
@@ -481,9 +482,13 @@ int alloc_lu_ws(acfd_ctx *ctx)
     if (batch) {
         HIP_TRY(hipMalloc((void **)&ctx->lu_ctr, 2 * sizeof(int)));
         HIP_TRY(hipMemset(ctx->lu_ctr, 0, 2 * sizeof(int)));
-        for (int sh = 0; sh < lusgs_tile_shapes(); sh++)
+        const bool sa = ctx->c.turb == 2 && ctx->c.solver == 0;
+        for (int sh = 0; sh < lusgs_tile_shapes(); sh++) {
             if (lusgs_batch_make(nb, es.data(), fs.data(), ctx->arena, sh, ctx->lu.sl, ctx->lu.ex, &ctx->lub[sh]))
                 return fail(ACFD_EDEVICE, "batched LU-SGS tables: device allocation failed");
+            if (sa && sa_batch_make(nb, es.data(), fs.data(), ctx->arena, sh, ctx->lu.ex, &ctx->sab[sh]))
+                return fail(ACFD_EDEVICE, "batched SA sweep tables: device allocation failed");
+        }
     }
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     return 0;
@@ -782,9 +787,40 @@ int st_implicit_all(acfd_ctx *ctx, const double *cfl_per_block)
 int st_sa(acfd_ctx *ctx, int m, const double *cfl)
 {
     StageTimer t(ctx, ACFD_STAGE_SA);
-    KRN_TRY(launch_sa(ctx->stream, ctx->blk[m].e, ctx->blk[m].f, ctx->c, cfl));
+    const Block &b = ctx->blk[m];
+    if ((ctx->cfg.lusgs_kernel == 1 || ctx->cfg.lusgs_kernel == 2) && ctx->lu.ex) {
+        // tile sweeps in the flow sweep's exchange slots (idle, sentinel-filled)
+        const int shape = b.tile_shape >= 0 ? b.tile_shape : ctx->lub_shape >= 0 ? ctx->lub_shape : 0;
+        const SaSweep sw{shape, ctx->lu.ex, ctx->lu.err};
+        KRN_TRY(launch_sa(ctx->stream, b.e, b.f, ctx->c, cfl, &sw));
+    } else {
+        KRN_TRY(launch_sa(ctx->stream, b.e, b.f, ctx->c, cfl));
+    }
     return 0;
 }
+
+// sa_model of every block, the sweeps batched over the blocks with
+// lusgs_kernel 2 (block m reads cfl_of(m)).
+template <typename F>
+int st_sa_all(acfd_ctx *ctx, F cfl_of)
+{
+    const int nb = (int)ctx->blk.size();
+    const int shape = ctx->lub_shape >= 0 ? ctx->lub_shape : 0;
+    if (ctx->cfg.lusgs_kernel != 2 || ctx->sab[shape].total <= 0) {
+        for (int m = 0; m < nb; m++) {
+            int rc = st_sa(ctx, m, cfl_of(m));
+            if (rc) return rc;
+        }
+        return 0;
+    }
+    StageTimer t(ctx, ACFD_STAGE_SA);
+    for (int m = 0; m < nb; m++)
+        KRN_TRY(launch_sa_pre(ctx->stream, ctx->blk[m].e, ctx->blk[m].f, ctx->c, cfl_of(m)));
+    KRN_TRY(launch_sa_batch(ctx->stream, ctx->sab[shape], ctx->lu.err, ctx->lu_ctr));
+    for (int m = 0; m < nb; m++) KRN_TRY(launch_sa_post(ctx->stream, ctx->blk[m].e, ctx->blk[m].f));
+    return 0;
+}
+
 
 int st_crank(acfd_ctx *ctx, int m)
 {
@@ -952,7 +988,7 @@ int one_step(acfd_ctx *ctx)
     if (ctx->cfg.iflag_timeadvance == 2 && stagewise) {
         rc = residual_stage_all(ctx, aux);
         if (!rc) rc = st_implicit_all(ctx, cfl ? cfl : nullptr);
-        for (int m = 0; m < nb && !rc && sa; m++) rc = st_sa(ctx, m, cfl_of(m));
+        if (!rc && sa) rc = st_sa_all(ctx, cfl_of);
         if (!rc) rc = refresh(ctx);
         if (!rc) rc = st_getpri_all(ctx);
     } else if (ctx->cfg.iflag_timeadvance == 3 && stagewise) {
@@ -1135,6 +1171,7 @@ int acfd_destroy(acfd_ctx *ctx)
     if (ctx->lu.sl) (void)hipFree(ctx->lu.sl);
     if (ctx->lu.err) (void)hipFree(ctx->lu.err);
     for (auto &b : ctx->lub) lusgs_batch_free(&b);
+    for (auto &b : ctx->sab) lusgs_batch_free(&b);
     if (ctx->lu_ctr) (void)hipFree(ctx->lu_ctr);
     if (ctx->d_cfl_blk) (void)hipFree(ctx->d_cfl_blk);
     if (ctx->d_blocks) (void)hipFree(ctx->d_blocks);

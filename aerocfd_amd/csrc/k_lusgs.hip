@@ -766,6 +766,195 @@ __global__ void __launch_bounds__(TJ *TK) k_lu_tiles_batch(const LuBatchEnt *__r
     }
 }
 
+// ------------------------------------------------ scalar sweeps (SA model)
+// The LU-SGS of the SA working variable (sub_tur_sa.f90:755-812) has the
+// same dependencies with one unknown per node:
+//   forward   dn* = (rhs + a+ dn*_i-1 + b+ dn*_j-1 + c+ dn*_k-1) / diag
+//   backward  dn  = dn* - (a- dn_i+1 + b- dn_j+1 + c- dn_k+1) / diag
+// with the node's own a+- = (a +- |a|)/2 (a, b, c the contravariant
+// velocities), so a neighbour only supplies its value.  Same tiles and
+// skewed clock as the flow sweep; the operands (a, b, c, diag and rhs or
+// dn*) are read in place, one i-line per lane (a cache line serves 16
+// consecutive steps), and one value per exchange slot crosses a tile edge.
+struct SaGeom {
+    int TJ, TK, nbj, nbk, ntiles, nsteps, Li, Lj, Lk;
+    long exj_tile, exk_tile;  // doubles per tile of the j / k exchange regions
+};
+
+static SaGeom sa_geom(const Ext &e, int shape)
+{
+    const SlGeom s = sl_geom(e, shape);
+    SaGeom g;
+    g.TJ = s.TJ;
+    g.TK = s.TK;
+    g.nbj = s.nbj;
+    g.nbk = s.nbk;
+    g.ntiles = s.ntiles;
+    g.nsteps = s.nsteps;
+    g.Li = s.Li;
+    g.Lj = s.Lj;
+    g.Lk = s.Lk;
+    g.exj_tile = (long)(g.Li + g.TK - 1) * g.TK;
+    g.exk_tile = (long)(g.Li + g.TJ - 1) * g.TJ;
+    return g;
+}
+
+__device__ inline double ex_take1(double *slot, unsigned long long pre, int *err, bool &failed)
+{
+    unsigned long long b = pre;
+    if (b == kSentinel) {
+        for (long spins = 0; !failed; spins++) {
+            b = xload_bits(slot);
+            if (b != kSentinel) break;
+            if ((spins & 1023) == 1023 &&
+                (spins > (1L << 21) || __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
+                atomicOr(err, 1);
+                failed = true;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+        if (failed) b = 0;
+    }
+    xstore_bits(slot, kSentinel);
+    return __longlong_as_double((long long)b);
+}
+
+// cu = a, b, c, diag (component stride N); fwd reads rhs and writes dns,
+// bwd reads dns and writes dn
+template <int DIR, int TJ, int TK>
+__device__ inline void sa_tiles_body(const Ext &e, const SaGeom &g, int tile, const double *__restrict__ cu,
+                                     const double *__restrict__ rhs, double *__restrict__ dns,
+                                     double *__restrict__ dn, double *ex, int *err, double (*pj)[TJ * TK],
+                                     double (*pk)[TJ * TK])
+{
+    const int bj = tile % g.nbj, bk = tile / g.nbj;
+    const int l = threadIdx.x, tj = l % TJ, tk = l / TJ;
+    const int lj = bj * TJ + tj, lk = bk * TK + tk;
+    const bool active = lj < g.Lj && lk < g.Lk;
+    const int sj = DIR > 0 ? tj : TJ - 1 - tj, sk = DIR > 0 ? tk : TK - 1 - tk;
+    const bool j_ring = DIR > 0 ? lj == 0 : lj + 1 >= g.Lj;
+    const bool k_ring = DIR > 0 ? lk == 0 : lk + 1 >= g.Lk;
+    const bool j_in = !j_ring && sj == 0, k_in = !k_ring && sk == 0;
+    const int lpj = l - DIR, lpk = l - DIR * TJ;
+    const bool j_out = sj == TJ - 1 && (DIR > 0 ? bj + 1 < g.nbj : bj > 0);
+    const bool k_out = sk == TK - 1 && (DIR > 0 ? bk + 1 < g.nbk : bk > 0);
+    double *exj = ex, *exk = ex + (long)g.ntiles * g.exj_tile;
+    double *jin = exj + (long)(DIR > 0 ? tile - 1 : tile + 1) * g.exj_tile + tk;
+    double *kin = exk + (long)(DIR > 0 ? tile - g.nbj : tile + g.nbj) * g.exk_tile + tj;
+    double *jout = exj + (long)tile * g.exj_tile + tk;
+    double *kout = exk + (long)tile * g.exk_tile + tj;
+    const double *jsrc = j_in ? jin : ex;
+    const double *ksrc = k_in ? kin : ex;
+    const int xj_last = g.Li + TK - 2, xk_last = g.Li + TJ - 2;
+    const long N = e.n;
+    // first node of the lane's line in sweep order (inactive lanes read a
+    // valid node and ignore it)
+    const int cj = min(lj, g.Lj - 1), ck = min(lk, g.Lk - 1);
+    const long n0 = e.ix(e.is1, e.js1 + cj, e.ks1 + ck);
+    const double *src = DIR > 0 ? rhs : dns;
+    double *out = DIR > 0 ? dns : dn;
+    bool failed = false;
+    double vi = 0.0;
+    // operands of step s: a, b, c, diag, rhs / dn*, and the exchange bits
+    auto fetch = [&](int s, double *op, unsigned long long &xj, unsigned long long &xk) {
+        const int pos = min(max(s - sj - sk, 0), g.Li - 1);
+        const long n = n0 + (DIR > 0 ? pos : g.Li - 1 - pos);
+        op[0] = cu[n];
+        op[1] = cu[N + n];
+        op[2] = cu[2 * N + n];
+        op[3] = cu[3 * N + n];
+        op[4] = src[n];
+        const int p = s - sj - sk;
+        const int dj = j_in ? min(max(p + sk, 0), xj_last) : 0;
+        const int dk = k_in ? min(max(p + sj, 0), xk_last) : 0;
+        xj = xload_bits(jsrc + (long)dj * TK);
+        xk = xload_bits(ksrc + (long)dk * TJ);
+    };
+    auto step = [&](int s, const double *op, unsigned long long xj, unsigned long long xk) {
+        const int par = s & 1;
+        const int pos = s - sj - sk;
+        if (active && pos >= 0 && pos < g.Li) {
+            const long n = n0 + (DIR > 0 ? pos : g.Li - 1 - pos);
+            const double vj = j_ring ? 0.0 : j_in ? ex_take1(jin + (long)(pos + sk) * TK, xj, err, failed)
+                                                  : pj[par ^ 1][lpj];
+            const double vk = k_ring ? 0.0 : k_in ? ex_take1(kin + (long)(pos + sj) * TJ, xk, err, failed)
+                                                  : pk[par ^ 1][lpk];
+            double res;
+            if (DIR > 0) {
+                const double ta = 0.5 * (op[0] + fabs(op[0])) * vi;
+                const double tb = 0.5 * (op[1] + fabs(op[1])) * vj;
+                const double tc = 0.5 * (op[2] + fabs(op[2])) * vk;
+                res = (op[4] + (ta + tb + tc)) / op[3];
+            } else {
+                const double ta = 0.5 * (op[0] - fabs(op[0])) * vi;
+                const double tb = 0.5 * (op[1] - fabs(op[1])) * vj;
+                const double tc = 0.5 * (op[2] - fabs(op[2])) * vk;
+                res = op[4] - (ta + tb + tc) / op[3];
+            }
+            out[n] = res;
+            vi = res;
+            pj[par][l] = res;
+            pk[par][l] = res;
+            if (j_out) xstore(jout + (long)(pos + sk) * TK, res);
+            if (k_out) xstore(kout + (long)(pos + sj) * TJ, res);
+        }
+        lds_barrier();
+    };
+    double opA[5], opB[5];
+    unsigned long long xjA, xkA, xjB, xkB;
+    fetch(0, opA, xjA, xkA);
+    for (int s = 0; s < g.nsteps; s += 2) {
+        fetch(s + 1, opB, xjB, xkB);
+        step(s, opA, xjA, xkA);
+        fetch(s + 2, opA, xjA, xkA);
+        step(s + 1, opB, xjB, xkB);
+    }
+}
+
+template <int DIR, int TJ, int TK>
+__global__ void __launch_bounds__(TJ *TK) k_sa_tiles(Ext e, SaGeom g, const double *__restrict__ cu,
+                                                      const double *__restrict__ rhs, double *__restrict__ dns,
+                                                      double *__restrict__ dn, double *ex, int *err)
+{
+    __shared__ double pj[2][TJ * TK], pk[2][TJ * TK];
+    const int tile = DIR > 0 ? (int)blockIdx.x : g.ntiles - 1 - (int)blockIdx.x;
+    sa_tiles_body<DIR, TJ, TK>(e, g, tile, cu, rhs, dns, dn, ex, err, pj, pk);
+}
+
+struct SaBatchEnt {
+    Ext e;
+    SaGeom g;
+    long cu, rhs, dns, dn, ex;
+    int tile0;
+};
+
+template <int DIR, int TJ, int TK>
+__global__ void __launch_bounds__(TJ *TK) k_sa_tiles_batch(const SaBatchEnt *__restrict__ ents,
+                                                            const int *__restrict__ tile_ent, int total,
+                                                            int resident, double *arena, double *exb, int *err,
+                                                            int *ctr)
+{
+    __shared__ double pj[2][TJ * TK], pk[2][TJ * TK];
+    __shared__ int ticket;
+    if (!resident) {
+        if (threadIdx.x == 0)
+            ticket = __hip_atomic_fetch_add(ctr, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __syncthreads();
+    }
+    const int flat = resident ? (int)blockIdx.x : __builtin_amdgcn_readfirstlane(ticket);
+    const SaBatchEnt &b = ents[tile_ent[flat]];
+    const Ext e = b.e;
+    const SaGeom g = b.g;
+    const int local = flat - b.tile0;
+    sa_tiles_body<DIR, TJ, TK>(e, g, DIR > 0 ? local : g.ntiles - 1 - local, arena + b.cu, arena + b.rhs,
+                               arena + b.dns, arena + b.dn, exb + b.ex, err, pj, pk);
+    if (!resident && threadIdx.x == 0 &&
+        __hip_atomic_fetch_add(ctr + 1, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == total - 1) {
+        __hip_atomic_store(ctr, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(ctr + 1, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
 int lusgs_tile_shapes() { return kNumTileShapes; }
 
 long lusgs_tiles(const Ext &e, int shape)
@@ -939,6 +1128,116 @@ int launch_lusgs_batch(hipStream_t s, const LuBatch &b, int phase, double gamma,
         if (tt == 256) launch_layout_batch<256>(s, which, b);
         else if (tt == 128) launch_layout_batch<128>(s, which, b);
         else launch_layout_batch<64>(s, which, b);
+    }
+    ACFD_CHECK_LAUNCH();
+    return 0;
+}
+
+// ---- scalar sweeps: host side
+template <int DIR, int TJ, int TK>
+static bool sa_fits_resident(int total)
+{
+    int dev = 0, cus = 0, per_cu = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_sa_tiles_batch<DIR, TJ, TK>, TJ * TK, 0) !=
+            hipSuccess)
+        return false;
+    return (long)cus * per_cu >= total;
+}
+
+template <int TJ, int TK>
+static void sa_sweeps_t(hipStream_t s, const Ext &e, const SaGeom &g, const double *cu, const double *rhs,
+                        double *dns, double *dn, double *ex, int *err)
+{
+    hipLaunchKernelGGL((k_sa_tiles<1, TJ, TK>), dim3((unsigned)g.ntiles), dim3(TJ * TK), 0, s, e, g, cu, rhs, dns,
+                       dn, ex, err);
+    hipLaunchKernelGGL((k_sa_tiles<-1, TJ, TK>), dim3((unsigned)g.ntiles), dim3(TJ * TK), 0, s, e, g, cu, rhs,
+                       dns, dn, ex, err);
+}
+
+// both SA sweeps of one block (ex: sentinel-filled, at least
+// ntiles * (exj_tile + exk_tile) doubles -- less than the flow sweep's)
+int launch_sa_sweeps(hipStream_t s, const Ext &e, int shape, const double *cu, const double *rhs, double *dns,
+                     double *dn, double *ex, int *err)
+{
+    const SaGeom g = sa_geom(e, shape);
+    if (g.Li <= 0 || g.Lj <= 0 || g.Lk <= 0) return 0;
+    switch (shape) {
+    case 0: sa_sweeps_t<16, 16>(s, e, g, cu, rhs, dns, dn, ex, err); break;
+    case 1: sa_sweeps_t<16, 8>(s, e, g, cu, rhs, dns, dn, ex, err); break;
+    case 2: sa_sweeps_t<8, 8>(s, e, g, cu, rhs, dns, dn, ex, err); break;
+    default: sa_sweeps_t<16, 4>(s, e, g, cu, rhs, dns, dn, ex, err); break;
+    }
+    ACFD_CHECK_LAUNCH();
+    return 0;
+}
+
+int sa_batch_make(int nblk, const Ext *e, const Fields *f, double *arena, int shape, double *ex, LuBatch *out)
+{
+    std::vector<SaBatchEnt> ents;
+    std::vector<int> te;
+    long xo = 0;
+    for (int m : batch_order(nblk, e, shape)) {
+        const SaGeom g = sa_geom(e[m], shape);
+        if (!f[m].w_sa) continue;
+        SaBatchEnt b;
+        b.e = e[m];
+        b.g = g;
+        b.cu = f[m].w_dqs - arena;
+        b.rhs = f[m].sa_rhs - arena;
+        b.dns = f[m].w_sa + 8 * e[m].n - arena;
+        b.dn = f[m].w_dq - arena;
+        b.ex = xo;
+        b.tile0 = (int)te.size();
+        xo += round_up32((long)g.ntiles * (g.exj_tile + g.exk_tile));
+        for (int t = 0; t < g.ntiles; t++) te.push_back((int)ents.size());
+        ents.push_back(b);
+    }
+    out->shape = shape;
+    out->arena = arena;
+    out->ex = ex;
+    out->nent = (int)ents.size();
+    out->total = (int)te.size();
+    if (ents.empty()) return 0;
+    const char *t = std::getenv("ACFD_LU_TICKETS");
+    const bool tickets = t && std::atoi(t) == 1;
+    bool fwd = false, bwd = false;
+    switch (shape) {
+    case 0: fwd = sa_fits_resident<1, 16, 16>(out->total); bwd = sa_fits_resident<-1, 16, 16>(out->total); break;
+    case 1: fwd = sa_fits_resident<1, 16, 8>(out->total); bwd = sa_fits_resident<-1, 16, 8>(out->total); break;
+    case 2: fwd = sa_fits_resident<1, 8, 8>(out->total); bwd = sa_fits_resident<-1, 8, 8>(out->total); break;
+    default: fwd = sa_fits_resident<1, 16, 4>(out->total); bwd = sa_fits_resident<-1, 16, 4>(out->total); break;
+    }
+    out->resident[0] = fwd && !tickets ? 1 : 0;
+    out->resident[1] = bwd && !tickets ? 1 : 0;
+    if (hipMalloc(&out->ents, ents.size() * sizeof(SaBatchEnt)) != hipSuccess) return 1;
+    if (hipMalloc((void **)&out->tile_ent, te.size() * sizeof(int)) != hipSuccess) return 1;
+    if (hipMemcpy(out->ents, ents.data(), ents.size() * sizeof(SaBatchEnt), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(out->tile_ent, te.data(), te.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess)
+        return 1;
+    return 0;
+}
+
+template <int TJ, int TK>
+static void sa_batch_t(hipStream_t s, const LuBatch &b, int *err, int *ctr)
+{
+    const SaBatchEnt *ents = (const SaBatchEnt *)b.ents;
+    hipLaunchKernelGGL((k_sa_tiles_batch<1, TJ, TK>), dim3((unsigned)b.total), dim3(TJ * TK), 0, s, ents,
+                       b.tile_ent, b.total, b.resident[0], b.arena, b.ex, err, ctr);
+    hipLaunchKernelGGL((k_sa_tiles_batch<-1, TJ, TK>), dim3((unsigned)b.total), dim3(TJ * TK), 0, s, ents,
+                       b.tile_ent, b.total, b.resident[1], b.arena, b.ex, err, ctr);
+}
+
+// both SA sweeps of every block of the batch
+int launch_sa_batch(hipStream_t s, const LuBatch &b, int *err, int *ctr)
+{
+    if (b.total <= 0) return 0;
+    switch (b.shape) {
+    case 0: sa_batch_t<16, 16>(s, b, err, ctr); break;
+    case 1: sa_batch_t<16, 8>(s, b, err, ctr); break;
+    case 2: sa_batch_t<8, 8>(s, b, err, ctr); break;
+    default: sa_batch_t<16, 4>(s, b, err, ctr); break;
     }
     ACFD_CHECK_LAUNCH();
     return 0;

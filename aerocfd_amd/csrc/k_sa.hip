@@ -6,6 +6,10 @@
 #include "common.hpp"
 #include "kernels.hpp"
 
+// No implicit contraction: the hyperplane and tile sweeps evaluate the same
+// expressions and must round alike.
+#pragma clang fp contract(off)
+
 namespace acfd {
 
 namespace sa {
@@ -279,7 +283,8 @@ __global__ void __launch_bounds__(256) k_add1(Ext e, Box b, double *__restrict__
     x[n] = x[n] + d[n];
 }
 
-int launch_sa(hipStream_t s, const Ext &e, const Fields &f, const Consts &c, const double *cfl)
+// Everything before the sweeps: length scale, right-hand side, diagonal.
+int launch_sa_pre(hipStream_t s, const Ext &e, const Fields &f, const Consts &c, const double *cfl)
 {
     int bs = 256;
     // sa_rhs = 0 and zeroed temporaries (sub_tur_sa.f90:48, :276-291)
@@ -298,7 +303,16 @@ int launch_sa(hipStream_t s, const Ext &e, const Fields &f, const Consts &c, con
     Box ring{e.is1 - 1, e.ie1 + 1, e.js1 - 1, e.je1 + 1, e.ks1 - 1, e.ke1 + 1};
     hipLaunchKernelGGL(k_sa_prep, box_grid(ring), t, 0, s, e, ring, c, f.pri, f.dxidx, f.nut, f.dt, cfl, f.w_sa,
                        f.w_dqs, f.w_dq);
+    ACFD_CHECK_LAUNCH();
+    return 0;
+}
+
+// The two sweeps as hyperplane launches.
+int launch_sa_planes(hipStream_t s, const Ext &e, const Fields &f)
+{
+    dim3 t(64, 4, 1);
     int Li = e.ie1 - e.is1 + 1, Lj = e.je1 - e.js1 + 1, Lk = e.ke1 - e.ks1 + 1;
+    if (Li <= 0 || Lj <= 0 || Lk <= 0) return 0;
     int L = Li + Lj + Lk - 2;
     double *dn1 = f.w_sa + 8 * e.n;
     for (int dir = 0; dir < 2; dir++)
@@ -311,9 +325,36 @@ int launch_sa(hipStream_t s, const Ext &e, const Fields &f, const Consts &c, con
             else
                 hipLaunchKernelGGL((k_sa_plane<-1>), gg, t, 0, s, e, l, ja, ka, f.w_dqs, f.sa_rhs, dn1, f.w_dq);
         }
+    ACFD_CHECK_LAUNCH();
+    return 0;
+}
+
+// nut += dnut on the computed range.
+int launch_sa_post(hipStream_t s, const Ext &e, const Fields &f)
+{
+    Box b{e.is1, e.ie1, e.js1, e.je1, e.ks1, e.ke1};
+    dim3 g = box_grid(b), t(64, 4, 1);
+    if (g.x == 0) return 0;
     hipLaunchKernelGGL(k_add1, g, t, 0, s, e, b, f.nut, f.w_dq);
     ACFD_CHECK_LAUNCH();
     return 0;
+}
+
+// sa_model of one block (sub_tur_sa.f90:7-122) with hyperplane sweeps, or
+// with the tile sweeps when `sweep` is given (same results).
+int launch_sa(hipStream_t s, const Ext &e, const Fields &f, const Consts &c, const double *cfl,
+              const SaSweep *sweep)
+{
+    int rc = launch_sa_pre(s, e, f, c, cfl);
+    if (!rc) {
+        if (sweep)
+            rc = launch_sa_sweeps(s, e, sweep->shape, f.w_dqs, f.sa_rhs, f.w_sa + 8 * e.n, f.w_dq, sweep->ex,
+                                  sweep->err);
+        else
+            rc = launch_sa_planes(s, e, f);
+    }
+    if (!rc) rc = launch_sa_post(s, e, f);
+    return rc;
 }
 
 }  // namespace acfd

@@ -95,13 +95,26 @@ int lusgs_batch_make(int nblk, const Ext *e, const Fields *f, double *arena, int
 void lusgs_batch_free(LuBatch *b);
 int launch_lusgs_batch(hipStream_t s, const LuBatch &b, int phase, double gamma, int *err, int *ctr);
 int launch_fill_sentinel(hipStream_t s, double *p, long n);
+// tile sweeps of the SA model's scalar LU-SGS (cu = a, b, c, diag)
+int launch_sa_sweeps(hipStream_t s, const Ext &e, int shape, const double *cu, const double *rhs, double *dns,
+                     double *dn, double *ex, int *err);
+int sa_batch_make(int nblk, const Ext *e, const Fields *f, double *arena, int shape, double *ex, LuBatch *out);
+int launch_sa_batch(hipStream_t s, const LuBatch &b, int *err, int *ctr);
 
 // k_bdf2.hip
 int launch_crank(hipStream_t s, const Ext &e, const Fields &f, const Consts &c);
 int launch_update_q0(hipStream_t s, const Ext &e, const Fields &f);
 
 // k_sa.hip
-int launch_sa(hipStream_t s, const Ext &e, const Fields &f, const Consts &c, const double *cfl);
+struct SaSweep {
+    int shape;   // tile shape index
+    double *ex;  // sentinel-filled exchange slots
+    int *err;
+};
+int launch_sa(hipStream_t s, const Ext &e, const Fields &f, const Consts &c, const double *cfl,
+              const SaSweep *sweep = nullptr);
+int launch_sa_pre(hipStream_t s, const Ext &e, const Fields &f, const Consts &c, const double *cfl);
+int launch_sa_post(hipStream_t s, const Ext &e, const Fields &f);
 
 // k_bc.hip
 struct BcTask {       // one physical subface

@@ -1,12 +1,19 @@
 """Throughput of the residual-and-update path on one MI355X.
 
-Workload: Taylor-Green vortex, one block of 256^3 nodes on [0, 2 pi]^3 with
-every face pair self-connected (node 256 == node 1), Mach 0.1, Re 1600,
-laminar Navier-Stokes with the reference scheme (WENO3 + Roe inviscid flux,
-central viscous flux, first-order implicit LU-SGS), FP64.  One step is one
-pass of main.f90:343-447 plus the residual norm of that step.
+Default workload (--workload tgv): Taylor-Green vortex, one block of 256^3
+nodes on [0, 2 pi]^3 with every face pair self-connected (node 256 ==
+node 1), Mach 0.1, Re 1600, laminar Navier-Stokes with the reference scheme
+(WENO3 + Roe inviscid flux, central viscous flux, first-order implicit
+LU-SGS), FP64.  One step is one pass of main.f90:343-447 plus the residual
+norm of that step.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 256]
+--workload flat_plate: 12 seeded uneven blocks of a turbulent flat plate
+(tanh wall clustering, Mach 0.3, Re_L 5e6, SA, wall / inflow / outflow /
+symmetry / farfield faces, local time step, first-order implicit), by
+default with FP32 fluxes and the FP64 update, every stage and sweep one
+launch over all blocks.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 256] [--workload tgv|flat_plate]
 
 Prints one JSON line.  value = Mcell-updates/s = nodes * steps / seconds with
 the state resident in HBM; roofline = the dominant kernel's algorithmic bytes
@@ -60,9 +67,17 @@ MEASURED_TRAFFIC_256 = {
 }
 
 
-def cpu_baseline(n_sample=96, steps=8, timeout=300):
-    """Time the reference's own Fortran (or the oracle) on a TGV sample of
-    n_sample^3 nodes for `steps` steps in a child process (one core)."""
+CPU_SAMPLES = {
+    "tgv": ("cases.tgv_box(96)", 8, "TGV 96^3 periodic block"),
+    "flat_plate": ("cases.flat_plate(nblk=(4, 3, 1), lo=(24, 20, 12), hi=(48, 32, 16))", 60,
+                   "flat plate, the same 12-block layout at reduced node counts"),
+}
+
+
+def cpu_baseline(workload="tgv", timeout=300):
+    """Time the reference's own Fortran (or the oracle) on a reduced sample
+    of the workload for a few steps in a child process (one core)."""
+    make, steps, what = CPU_SAMPLES[workload]
     code = r"""
 import json, os, sys, time, tempfile
 sys.path.insert(0, %r)
@@ -70,19 +85,21 @@ import numpy as np
 from aerocfd_amd import cases
 from oracle import refwrap as R, oracle as O
 from tests.helpers import ora_config, ora_states
-n, steps = %d, %d
-case = cases.tgv_box(n)
+steps = %d
+case = %s
 kind = "reference" if R.available() else "port"
 if kind == "reference":
     os.chdir(tempfile.mkdtemp())
     c = case.cfg
-    R.setup(solver=0, itime=1, tadv=2, ma=c.ma, re=c.re, tinf=c.tinf, cfl=c.cfl, dt=c.dt)
+    R.setup(solver=c.iflag_solver, itime=c.iflag_time, turb=c.iflag_turbulence, des=c.iflag_des,
+            tadv=c.iflag_timeadvance, ma=c.ma, re=c.re, tinf=c.tinf, cfl=c.cfl, dt=c.dt)
     exts = R.begin(case.dims, case.subfaces)
-    fl = case.fields[0]
-    for k in ("q", "dxidx", "inv_j", "amu", "dt"):
-        R.put(1, k, fl[k])
     sc = R.scalars()
-    R.put(1, "pri_v", O.getpri(exts[0], fl["q"], sc["gamma"], sc["cv"])[0])
+    for m, fl in enumerate(case.fields):
+        for k, v in fl.items():
+            if k in R.XFER:
+                R.put(m + 1, k, v)
+        R.put(m + 1, "pri_v", O.getpri(exts[m], fl["q"], sc["gamma"], sc["cv"])[0])
     R.physical_bc(); R.update_buffer(); R.average()
     R.step()
     t0 = time.perf_counter()
@@ -92,7 +109,8 @@ if kind == "reference":
 else:
     cfg = ora_config(case.cfg, case.totalnodes)
     st = ora_states(case)
-    st[0]["pri_v"] = O.getpri(st[0].ext, st[0]["q"], cfg.gamma, cfg.cv)[0]
+    for b in st:
+        b["pri_v"] = O.getpri(b.ext, b["q"], cfg.gamma, cfg.cv)[0]
     O.prepare(st, cfg)
     O.step(st, cfg)
     t0 = time.perf_counter()
@@ -100,7 +118,7 @@ else:
         O.step(st, cfg)
     dt = time.perf_counter() - t0
 print(json.dumps({"kind": kind, "seconds": dt, "nodes": case.totalnodes, "steps": steps}))
-""" % (ROOT, n_sample, steps)
+""" % (ROOT, steps, make)
     env = dict(os.environ, OMP_NUM_THREADS="1")
     try:
         r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=timeout,
@@ -115,8 +133,8 @@ print(json.dumps({"kind": kind, "seconds": dt, "nodes": case.totalnodes, "steps"
                 "sample": "timed out after %d s" % timeout}
     return {"value": d["nodes"] * d["steps"] / d["seconds"] / 1e6, "unit": "Mcell-updates/s", "cores": 1,
             "kind": d["kind"],
-            "sample": "TGV %d^3 periodic block, %d LU-SGS steps (same scheme and flags), %.2f s on one core"
-                      % (n_sample, d["steps"], d["seconds"])}
+            "sample": "%s (%d nodes), %d steps of the same scheme and flags in FP64, %.2f s on one core"
+                      % (what, d["nodes"], d["steps"], d["seconds"])}
 
 
 def main():
@@ -125,18 +143,29 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=256)
-    ap.add_argument("--precision", type=int, default=0, help="0 FP64, 1 FP32 flux / FP64 update")
-    ap.add_argument("--lusgs", type=int, default=1, help="0 hyperplane launches, 1 pipelined tile sweep per block, 2 tile sweeps of all blocks in one launch")
+    ap.add_argument("--workload", choices=["tgv", "flat_plate"], default="tgv")
+    ap.add_argument("--precision", type=int, default=None,
+                    help="0 FP64, 1 FP32 flux / FP64 update (default: 0 for tgv, 1 for flat_plate)")
+    ap.add_argument("--lusgs", type=int, default=None, help="0 hyperplane launches, 1 pipelined tile sweep per block, 2 tile sweeps "
+                         "of all blocks in one launch (default: 1 for tgv, 2 for flat_plate)")
     ap.add_argument("--graph", action="store_true",
                     help="replay one captured HIP graph per step in the timed region (per-kernel times then "
                          "come from two eager steps after it)")
     ap.add_argument("--cuts", type=int, default=1, help="blocks per axis (the workload split into cuts^3 blocks)")
-    ap.add_argument("--batch", action="store_true", help="one launch per stage over all blocks")
+    ap.add_argument("--batch", type=int, default=None,
+                    help="1 = one launch per stage over all blocks (default: 0 for tgv, 1 for flat_plate)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB (a known-size kernel that "
                          "calibrates HBM byte counters)")
     args = ap.parse_args()
+    fp = args.workload == "flat_plate"
+    if args.precision is None:
+        args.precision = 1 if fp else 0
+    if args.lusgs is None:
+        args.lusgs = 2 if fp else 1
+    if args.batch is None:
+        args.batch = 1 if fp else 0
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; every block of the workload is resident on it. "
               "Sharding blocks over several GPUs is described in DESIGN.md but not built, so --gpus %d "
@@ -154,8 +183,11 @@ def main():
         torch.cuda.synchronize()
         del src, dst
 
-    cut = tuple(round(args.n * c / args.cuts) for c in range(1, args.cuts))
-    case = cases.tgv_box(args.n, blocks=(cut, cut, cut))
+    if fp:
+        case = cases.flat_plate()
+    else:
+        cut = tuple(round(args.n * c / args.cuts) for c in range(1, args.cuts))
+        case = cases.tgv_box(args.n, blocks=(cut, cut, cut))
     case.cfg.precision = args.precision
     case.cfg.lusgs_kernel = args.lusgs
     case.cfg.use_graph = 1 if args.graph else 0
@@ -193,7 +225,9 @@ def main():
     rr = [sol.ranges(m + 1) for m in range(len(case.dims))]
     sol.close()
     secs = t1 - t0
-    nodes = args.n ** 3  # the cut workload counts the same nodes as the single block
+    # the reference's totalnodes (sum of ni nj nk); the cut TGV counts the
+    # nodes of the single block
+    nodes = case.totalnodes if fp else args.n ** 3
     value = nodes * args.steps / secs / 1e6
     # dominant kernel among the timed ones (per launch averages)
     # computed nodes per launch: is1..ie1 x js1..je1 x ks1..ke1 of one block, or of
@@ -210,7 +244,7 @@ def main():
     avg_ms = cand[dom][0] / cand[dom][1]
     bytes_launch = table[dom] * (comp_all if dom in batched else comp_all / len(rr))
     achieved = bytes_launch / (avg_ms * 1e-3) / 1e9
-    default_cfg = args.n == 256 and args.lusgs == 1 and args.precision == 0 and args.cuts == 1
+    default_cfg = not fp and args.n == 256 and args.lusgs == 1 and args.precision == 0 and args.cuts == 1
     traffic = MEASURED_TRAFFIC_256.get(dom) if default_cfg else None
     out = {
         "metric": "Mcell-updates/s",
@@ -225,8 +259,12 @@ def main():
         "vs_baseline": None,
         "dtype": "f64" if args.precision == 0 else "f32-flux/f64-update",
         "data": "synthetic",
-        "config": {"workload": "tgv_%d^3_periodic_laminar_ns_weno3_roe_lusgs" % args.n, "nodes": nodes,
-                   "blocks": len(case.dims), "graph": bool(args.graph), "batch_blocks": bool(args.batch), "mach": 0.1, "re": 1600, "scheme": "WENO3+Roe, central viscous, LU-SGS",
+        "config": {"workload": (case.name + "_turbulent_sa_weno3_roe_lusgs" if fp else
+                                "tgv_%d^3_periodic_laminar_ns_weno3_roe_lusgs" % args.n),
+                   "nodes": nodes, "blocks": len(case.dims), "graph": bool(args.graph),
+                   "batch_blocks": bool(args.batch),
+                   "mach": case.cfg.ma, "re": case.cfg.re,
+                   "scheme": "WENO3+Roe, central viscous, %sLU-SGS" % ("SA, " if fp else ""),
                    "lusgs_kernel": {0: "hyperplanes", 1: "pipelined-tiles", 2: "batched-pipelined-tiles"}[args.lusgs]},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": PEAK_HBM_GBS,
                      "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4),
@@ -236,7 +274,7 @@ def main():
         "residual": [float("%.6e" % x) for x in res[:5]],
     }
     if not args.no_cpu:
-        cb = cpu_baseline()
+        cb = cpu_baseline(args.workload)
         out["cpu_baseline"] = cb
     print(json.dumps(out), flush=True)
 
