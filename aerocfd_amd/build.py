@@ -8,7 +8,8 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
-SOURCES = ["k_flux.hip", "k_flux5.hip", "k_sensor.hip", "k_lusgs.hip", "k_bdf2.hip", "k_sa.hip", "k_bc.hip", "context.hip"]
+SOURCES = ["k_flux.hip", "k_flux5.hip", "k_sensor.hip", "k_lusgs.hip", "k_bdf2.hip", "k_sa.hip", "k_bc.hip",
+           "k_grid.hip", "context.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-fPIC", "-shared", "-std=c++17",

@@ -452,6 +452,7 @@ def flat_plate(seed=3, nblk=(4, 3, 1), lo=(32, 32, 32), hi=(160, 96, 64), length
                          iflag_timeadvance=2, cfl=5.0, dt=1.0e-3, prl=0.72)
     dims, subs, origins = split_box(NI, NJ, NK, cuts, phys=(-50, -40, -20, -10, -30, -30))
     case = Case("flat_plate_%dblk_%dx%dx%d" % (len(dims), NI, NJ, NK), cfg, dims, subs)
+    case.coords = []  # (x, y, z) per block over the extended extents
     hx, hz = length[0] / (NI - 1), length[2] / (NK - 1)
     _, ymap, dymap = tanh_stretch(NJ, length[1], first)
     g = GAMMA
@@ -470,6 +471,7 @@ def flat_plate(seed=3, nblk=(4, 3, 1), lo=(32, 32, 32), hi=(160, 96, 64), length
         x = (I - 1.0) * hx
         y = ymap(sj)
         z = (K - 1.0) * hz
+        case.coords.append((x, y, z))
         yj = dymap(sj)
         dxidx = np.zeros(shp + (9,))
         dxidx[..., 0] = 1.0 / hx
@@ -509,3 +511,42 @@ def flat_plate(seed=3, nblk=(4, 3, 1), lo=(32, 32, 32), hi=(160, 96, 64), length
         sp[p0[4]:p0[5], p0[2]:p0[3], p0[0]:p0[1]] = np.maximum(np.maximum(spc[0], spc[1]), spc[2])
         fl["spacing"] = sp
     return case
+
+
+def warped_grid(NI=14, NJ=12, NK=10, cuts=((7,), (6,), ()), phys=(-10, -40, -20, -10, -30, -30), amp=0.08,
+                seed=0):
+    """A curvilinear multi-block grid: the box [0,1]^3 (NI x NJ x NK nodes)
+    cut into point-matched blocks and warped by a smooth seeded mapping
+    x + amp * sum of products of sines, so that every metric term is
+    non-trivial.  Halo nodes of interface faces carry the neighbour's
+    coordinates (what the coordinate exchange delivers).  Returns dims,
+    subfaces, the 18 ranges per block and (x, y, z) per block over the
+    extended extents."""
+    rng = np.random.default_rng(seed)
+    dims, subs, origins = split_box(NI, NJ, NK, cuts, phys=phys)
+    ph = rng.uniform(0, 2 * math.pi, (3, 3))
+    fr = rng.uniform(0.5, 1.5, (3, 3))
+
+    def mapping(gi, gj, gk):
+        s = [(gi - 1.0) / (NI - 1), (gj - 1.0) / (NJ - 1), (gk - 1.0) / (NK - 1)]
+        out = []
+        for a in range(3):
+            w = np.ones_like(s[0])
+            for b in range(3):
+                if b != a:
+                    w = w * np.sin(math.pi * fr[a][b] * s[b] + ph[a][b])
+            out.append(s[a] + amp * w * np.sin(math.pi * s[a]) + 0.3 * amp * s[(a + 1) % 3])
+        return out
+
+    ranges, coords = [], []
+    for m in range(len(dims)):
+        r = ranges_of(dims[m], subs[m])
+        org, _ = origins[m]
+        gi = org[0] + np.arange(r[0], r[1] + 1) - 1
+        gj = org[1] + np.arange(r[2], r[3] + 1) - 1
+        gk = org[2] + np.arange(r[4], r[5] + 1) - 1
+        K, J, I = np.meshgrid(gk, gj, gi, indexing="ij")
+        coords.append(tuple(np.ascontiguousarray(c) for c in mapping(I.astype(float), J.astype(float),
+                                                                      K.astype(float))))
+        ranges.append(r)
+    return dims, subs, ranges, coords

@@ -44,7 +44,8 @@ int fail(int code, const std::string &msg)
             return fail(ACFD_EDEVICE, std::string(#expr ": ") + hipGetErrorString((hipError_t)r__)); \
     } while (0)
 
-const int kNcomp[22] = {0, 5, 7, 5, 5, 5, 9, 6, 1, 3, 1, 9, 1, 3, 3, 3, 1, 1, 1, 1, 10, 1};
+const int kNcomp[28] = {0, 5, 7, 5, 5, 5, 9, 6, 1, 3, 1, 9, 1, 3, 3, 3, 1, 1, 1, 1, 10, 1, 1, 1, 1, 1, 1, 1};
+constexpr int kLastField = 27;
 
 constexpr int kTimeSlots = 20;  // acfd_timing entries
 constexpr int kSensorParams = 1505, kSensorFeatures = 12;  // the sensor network's size
@@ -53,6 +54,7 @@ struct Block {
     Ext e;
     Fields f;
     long arena_off = 0;     // doubles from the context's arena base
+    double *geo = nullptr;  // acfd_grid_metrics: x, y, z, spci, spcj, spck (n each)
     int tile_shape = -1;    // chosen by the autotuner on first use
     std::vector<acfd_subface> sub;
 };
@@ -95,6 +97,7 @@ struct acfd_ctx {
     std::vector<Block> blk;
     double *arena = nullptr;  // every block's fields
     long arena_n = 0;
+    double *geo = nullptr;    // acfd_grid_metrics: coordinates and edge lengths of every block
     long totalnodes = 0;
     bool topo = false;
     // device scalars
@@ -355,6 +358,8 @@ double *field_ptr(Block &b, int field)
     case ACFD_SA_RHS: return b.f.sa_rhs;
     case ACFD_Q0: return b.f.q0;
     case ACFD_LENGTH: return b.f.length;
+    case ACFD_X: case ACFD_Y: case ACFD_Z: case ACFD_SPCI: case ACFD_SPCJ: case ACFD_SPCK:
+        return b.geo ? b.geo + (long)(field - ACFD_X) * b.e.n : nullptr;
     }
     return nullptr;
 }
@@ -1167,6 +1172,7 @@ int acfd_destroy(acfd_ctx *ctx)
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     if (ctx->graph_exec) (void)hipGraphExecDestroy(ctx->graph_exec);
     if (ctx->arena) (void)hipFree(ctx->arena);
+    if (ctx->geo) (void)hipFree(ctx->geo);
     if (ctx->lu.ex) uc_release(ctx->lu.ex);
     if (ctx->lu.sl) (void)hipFree(ctx->lu.sl);
     if (ctx->lu.err) (void)hipFree(ctx->lu.err);
@@ -1370,7 +1376,7 @@ int acfd_block_ranges(acfd_ctx *ctx, int m, int *r)
 
 int acfd_upload(acfd_ctx *ctx, int m, int field, const double *host)
 {
-    if (!ctx || !ctx->topo || m < 1 || m > (int)ctx->blk.size() || !host || field < 1 || field > 21)
+    if (!ctx || !ctx->topo || m < 1 || m > (int)ctx->blk.size() || !host || field < 1 || field > kLastField)
         return fail(ACFD_EINVAL, "acfd_upload: bad arguments");
     Block &b = ctx->blk[m - 1];
     double *dst = field_ptr(b, field);
@@ -1387,7 +1393,7 @@ int acfd_upload(acfd_ctx *ctx, int m, int field, const double *host)
 
 int acfd_download(acfd_ctx *ctx, int m, int field, double *host)
 {
-    if (!ctx || !ctx->topo || m < 1 || m > (int)ctx->blk.size() || !host || field < 1 || field > 21)
+    if (!ctx || !ctx->topo || m < 1 || m > (int)ctx->blk.size() || !host || field < 1 || field > kLastField)
         return fail(ACFD_EINVAL, "acfd_download: bad arguments");
     Block &b = ctx->blk[m - 1];
     double *src = field_ptr(b, field);
@@ -1542,6 +1548,78 @@ int acfd_timing(acfd_ctx *ctx, int enable, double *ms, long *calls)
             ctx->stage_calls[s] = 0;
         }
     }
+    return 0;
+}
+
+int acfd_grid_metrics(acfd_ctx *ctx, double *const *xyz)
+{
+    if (!ctx || !ctx->topo || !xyz) return fail(ACFD_EINVAL, "acfd_grid_metrics: bad arguments");
+    const int nb = (int)ctx->blk.size();
+    for (int m = 0; m < nb; m++)
+        if (!xyz[m]) return fail(ACFD_EINVAL, "acfd_grid_metrics: missing coordinates of a block");
+    if (!ctx->geo) {
+        long tot = 0;
+        for (auto &b : ctx->blk) tot += 6 * b.e.n;
+        HIP_TRY(hipMalloc((void **)&ctx->geo, tot * sizeof(double)));
+        long off = 0;
+        for (auto &b : ctx->blk) {
+            b.geo = ctx->geo + off;
+            off += 6 * b.e.n;
+        }
+    }
+    const bool sa = ctx->c.turb == 2 && ctx->c.solver == 0;
+    for (int m = 0; m < nb; m++) {
+        Block &b = ctx->blk[m];
+        const long n = b.e.n;
+        HIP_TRY(hipMemcpyAsync(b.geo, xyz[m], 3 * n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+        const double *x = b.geo, *y = b.geo + n, *z = b.geo + 2 * n;
+        KRN_TRY(launch_grid_spacing(ctx->stream, b.e, x, y, z, b.geo + 3 * n, sa ? b.f.spacing : nullptr));
+        KRN_TRY(launch_grid_metrics(ctx->stream, b.e, x, y, z, b.f.w_flux, b.f.inv_j, b.f.dxidx,
+                                    sa ? b.f.aal : nullptr, sa ? b.f.bbe : nullptr, sa ? b.f.gga : nullptr));
+    }
+    if (sa) {
+        // wall points: the face nodes of every wall subface of every block
+        // (sub_init_readbcinfo.f90:294-400), SoA x | y | z
+        std::vector<double> wx, wy, wz;
+        for (int m = 0; m < nb; m++) {
+            const Block &b = ctx->blk[m];
+            const Ext &e = b.e;
+            for (const acfd_subface &f : b.sub) {
+                if (f.blk_t != -20) continue;
+                for (int k = f.ks; k <= f.ke; k++)
+                    for (int j = f.js; j <= f.je; j++)
+                        for (int i = f.is; i <= f.ie; i++) {
+                            const int ii = f.face == 1 ? e.is0 : f.face == 2 ? e.ie0 : i;
+                            const int jj = f.face == 3 ? e.js0 : f.face == 4 ? e.je0 : j;
+                            const int kk = f.face == 5 ? e.ks0 : f.face == 6 ? e.ke0 : k;
+                            const long n = e.ix(ii, jj, kk);
+                            wx.push_back(xyz[m][n]);
+                            wy.push_back(xyz[m][e.n + n]);
+                            wz.push_back(xyz[m][2 * e.n + n]);
+                        }
+            }
+        }
+        const long nw = (long)wx.size();
+        double *dw = nullptr;
+        if (nw > 0) {
+            std::vector<double> w(wx);
+            w.insert(w.end(), wy.begin(), wy.end());
+            w.insert(w.end(), wz.begin(), wz.end());
+            HIP_TRY(hipMalloc((void **)&dw, w.size() * sizeof(double)));
+            HIP_TRY(hipMemcpyAsync(dw, w.data(), w.size() * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+        }
+        int rc = 0;
+        for (int m = 0; m < nb && !rc; m++) {
+            const Block &b = ctx->blk[m];
+            const long n = b.e.n;
+            rc = launch_walldist(ctx->stream, b.e, b.geo, b.geo + n, b.geo + 2 * n, dw, nw, b.f.dist);
+        }
+        const hipError_t es = hipStreamSynchronize(ctx->stream);
+        if (dw) (void)hipFree(dw);
+        if (rc) return fail(ACFD_EDEVICE, "acfd_grid_metrics: wall distance launch failed");
+        if (es != hipSuccess) return fail(ACFD_EDEVICE, std::string("acfd_grid_metrics: ") + hipGetErrorString(es));
+    }
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
     return 0;
 }
 

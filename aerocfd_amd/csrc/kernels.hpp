@@ -63,6 +63,14 @@ int launch_lu_prep_batch(hipStream_t s, const BlockRec *blks, const BatchGrid &b
 int launch_residual_batch(hipStream_t s, const BlockRec *blks, int nblk, double *arena, double *partial,
                           int with_sa);
 
+// k_grid.hip
+int launch_grid_metrics(hipStream_t s, const Ext &e, const double *x, const double *y, const double *z, double *scratch9,
+                        double *inv_j, double *dxidx, double *aal, double *bbe, double *gga);
+int launch_grid_spacing(hipStream_t s, const Ext &e, const double *x, const double *y, const double *z, double *spc,
+                        double *spacing);
+int launch_walldist(hipStream_t s, const Ext &e, const double *x, const double *y, const double *z,
+                    const double *wall, long nw, double *dist);
+
 // k_lusgs.hip
 // workspace of the tile sweeps: sweep-layout operands (normal memory),
 // exchange slots (uncached memory, sentinel-filled) and an error word

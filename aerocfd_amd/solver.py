@@ -28,6 +28,8 @@ FIELDS = {  # name -> (acfd_field id, components)
     "tao": (7, 6), "vor": (8, 1), "amu": (9, 3), "dt": (10, 1), "dxidx": (11, 9), "inv_j": (12, 1),
     "aalpha": (13, 3), "bbeta": (14, 3), "ggamma": (15, 3), "dist": (16, 1), "spacing": (17, 1),
     "nut": (18, 1), "sa_rhs": (19, 1), "q0": (20, 10), "length": (21, 1),
+    # after grid_metrics
+    "x": (22, 1), "y": (23, 1), "z": (24, 1), "spci": (25, 1), "spcj": (26, 1), "spck": (27, 1),
 }
 
 STAGES = {
@@ -233,6 +235,20 @@ class Solver:
         out = np.zeros(6)
         _lib.check(self.lib.acfd_sensor_stats(self.h, _dp(out)))
         return dict(zip(("faces", "linear", "audits", "audit_max_dev", "blocks_fallen_back"), out[:5]))
+
+    def grid_metrics(self, xyz):
+        """Grid pre-processing on the device (init_spacing, init_walldist with
+        SA, gridtransformation): xyz = list per block of (x, y, z) arrays
+        over the extended extents, halos filled.  Writes the metric inputs
+        of the resident blocks."""
+        ptrs = (C.POINTER(C.c_double) * self.nblocks)()
+        keep = []
+        for m in range(self.nblocks):
+            a = np.ascontiguousarray(np.stack([np.asarray(c, dtype=np.float64) for c in xyz[m]]))
+            assert a.shape == (3,) + self.shape(m + 1, "x")
+            keep.append(a)
+            ptrs[m] = _dp(a)
+        _lib.check(self.lib.acfd_grid_metrics(self.h, ptrs))
 
     def exchange_coordinates(self, xyz, shifts=None):
         """Fill the coordinate halos of every block (UpdateBufferCoordinate).

@@ -74,7 +74,9 @@ enum acfd_field {
     ACFD_Q = 1, ACFD_PRI_V = 2, ACFD_RHSI = 3, ACFD_RHSV = 4, ACFD_RHS = 5, ACFD_DUDX = 6,
     ACFD_TAO = 7, ACFD_VOR = 8, ACFD_AMU = 9, ACFD_DT = 10, ACFD_DXIDX = 11, ACFD_INV_J = 12,
     ACFD_AALPHA = 13, ACFD_BBETA = 14, ACFD_GGAMMA = 15, ACFD_DIST = 16, ACFD_SPACING = 17,
-    ACFD_NUT = 18, ACFD_SA_RHS = 19, ACFD_Q0 = 20, ACFD_LENGTH = 21
+    ACFD_NUT = 18, ACFD_SA_RHS = 19, ACFD_Q0 = 20, ACFD_LENGTH = 21,
+    /* after acfd_grid_metrics: coordinates and edge lengths (sub_init_spacing.f90:29) */
+    ACFD_X = 22, ACFD_Y = 23, ACFD_Z = 24, ACFD_SPCI = 25, ACFD_SPCJ = 26, ACFD_SPCK = 27
 };
 
 /* Stages of one time step, for acfd_run_stage (reference routine in brackets). */
@@ -183,6 +185,20 @@ int acfd_sa_step(const acfd_config *cfg, const int *ranges, double *nut, double 
  * acfd_set_topology (NULL = none).  xyz[m] points to 3 consecutive extended
  * scalar fields x,y,z of block m+1, updated in place on the host. */
 int acfd_exchange_coordinates(acfd_ctx *ctx, double **xyz, const double *shift);
+
+/* Grid pre-processing on the device, replacing the start-up calls of
+ * main.f90:163-209: init_spacing (sub_init_spacing.f90:16-110), init_walldist
+ * (sub_init_walldist.f90:9-85, when the SA model is on; wall points are the
+ * face nodes of every subface with blk_t = -20) and gridtransformation
+ * (sub_init_gridtransformation.f90:27-874, 3-D).  xyz[m] points to the x, y,
+ * z fields of block m+1 over its extended extents with the halos filled
+ * (acfd_exchange_coordinates).  Writes dxidx and inv_j -- and with SA
+ * aalpha/bbeta/ggamma, spacing and dist -- of the resident blocks, and keeps
+ * the coordinates and edge lengths on the device (fields ACFD_X..ACFD_SPCK).
+ * Results equal the reference's bit for bit where it defines them; at halo
+ * nodes with two or three coordinates outside the physical range (never
+ * read by the path) the metrics are zero. */
+int acfd_grid_metrics(acfd_ctx *ctx, double *const *xyz);
 
 /* --------------------------------------------------- learned WCNS sensor */
 /* No counterpart in the reference.  A network 12 -> 32 -> 32 -> 1 reads the

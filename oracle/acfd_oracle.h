@@ -159,6 +159,19 @@ void ora_rk3_stage(const ora_ext *e, double *q, const double *qn, const double *
 /* one Shu-Osher RK3 step; scheme 0 = the reference's WENO3 + Roe */
 int ora_step_rk3(int nblk, ora_block *blk, ora_config *c, int scheme);
 
+/* Grid pre-processing (oracle_grid.c).  x, y, z are scalar fields over
+ * the extended extents. */
+/* sub_init_gridtransformation.f90:27-874 (3-D): written on the nodes with at
+ * most one coordinate outside the physical range; alpha may be NULL */
+void ora_gridtransformation(const ora_ext *e, const double *x, const double *y, const double *z, double *dxidx,
+                            double *inv_j, double *alpha, double *beta, double *gamma);
+/* sub_init_spacing.f90:16-110 (physical range); vol may be NULL */
+void ora_spacing(const ora_ext *e, const double *x, const double *y, const double *z, double *spci, double *spcj,
+                 double *spck, double *spacing, double *vol);
+/* sub_init_walldist.f90:9-85 against nw wall points (physical range) */
+void ora_walldist(const ora_ext *e, const double *x, const double *y, const double *z, long nw, const double *xw,
+                  const double *yw, const double *zw, double *dist);
+
 #ifdef __cplusplus
 }
 #endif

@@ -288,3 +288,69 @@ def inviscid_ext(ext, q, dxidx, inv_j, gamma, scheme):
 def step_rk3(states, cfg, scheme):
     arr = blocks_array(states)
     lib().ora_step_rk3(C.c_int(len(states)), arr, C.byref(cfg), C.c_int(scheme))
+
+
+# ------------------------------------------------------ grid pre-processing
+def gridtransformation(ext, x, y, z):
+    """sub_init_gridtransformation.f90:27-874 (3-D).  Outputs are NaN at the
+    edge and corner halo nodes, which the reference leaves to uninitialised
+    temporaries (the nodes with two or three coordinates outside the
+    physical range)."""
+    nk, nj, ni = ext.shape
+    dxidx = np.full((nk, nj, ni, 9), np.nan)
+    inv_j = np.full((nk, nj, ni), np.nan)
+    al, be, ga = (np.full((nk, nj, ni, 3), np.nan) for _ in range(3))
+    x, y, z = (np.ascontiguousarray(a, dtype=np.float64) for a in (x, y, z))
+    lib().ora_gridtransformation(C.byref(ext), ptr(x), ptr(y), ptr(z), ptr(dxidx), ptr(inv_j), ptr(al),
+                                 ptr(be), ptr(ga))
+    return dxidx, inv_j, al, be, ga
+
+
+def spacing(ext, x, y, z):
+    """sub_init_spacing.f90:16-110 -> spci, spcj, spck, spacing, vol (NaN
+    outside the physical range)."""
+    nk, nj, ni = ext.shape
+    out = [np.full((nk, nj, ni), np.nan) for _ in range(5)]
+    x, y, z = (np.ascontiguousarray(a, dtype=np.float64) for a in (x, y, z))
+    lib().ora_spacing(C.byref(ext), ptr(x), ptr(y), ptr(z), *[ptr(a) for a in out])
+    return out
+
+
+def walldist(ext, x, y, z, wall_xyz):
+    """sub_init_walldist.f90:30-85: distance of the physical nodes to the
+    nearest wall point (wall_xyz: (3, nw)); NaN on halos."""
+    nk, nj, ni = ext.shape
+    dist = np.full((nk, nj, ni), np.nan)
+    w = np.ascontiguousarray(wall_xyz, dtype=np.float64).reshape(3, -1)
+    x, y, z = (np.ascontiguousarray(a, dtype=np.float64) for a in (x, y, z))
+    lib().ora_walldist(C.byref(ext), ptr(x), ptr(y), ptr(z), C.c_long(w.shape[1]), ptr(w[0]), ptr(w[1]),
+                       ptr(w[2]), ptr(dist))
+    return dist
+
+
+def wall_points(exts, subfaces, xyz):
+    """Face-node coordinates of every wall subface (blk_t = -20) of every
+    block, in block and subface order (sub_init_readbcinfo.f90:294-400);
+    xyz[m] = (x, y, z) of block m.  Returns (3, nw)."""
+    pts = []
+    for e, subs, (x, y, z) in zip(exts, subfaces, xyz):
+        for s in subs:
+            if s[8] != -20:
+                continue
+            f = s[1]
+            i0, i1, j0, j1, k0, k1 = s[2:8]
+            if f == 1:
+                i0 = i1 = e.is0
+            elif f == 2:
+                i0 = i1 = e.ie0
+            elif f == 3:
+                j0 = j1 = e.js0
+            elif f == 4:
+                j0 = j1 = e.je0
+            elif f == 5:
+                k0 = k1 = e.ks0
+            else:
+                k0 = k1 = e.ke0
+            sl = (slice(k0 - e.ks, k1 - e.ks + 1), slice(j0 - e.js, j1 - e.js + 1), slice(i0 - e.is_, i1 - e.is_ + 1))
+            pts.append(np.stack([x[sl].ravel(), y[sl].ravel(), z[sl].ravel()]))
+    return np.concatenate(pts, axis=1) if pts else np.zeros((3, 0))

@@ -232,6 +232,10 @@ subroutine refh_xfer(mm, which, arr, n, io) bind(C, name="refh_xfer")
     case (22); call mv3(blk(mm)%x)
     case (23); call mv3(blk(mm)%y)
     case (24); call mv3(blk(mm)%z)
+    case (25); call mv3(blk(mm)%spci)
+    case (26); call mv3(blk(mm)%spcj)
+    case (27); call mv3(blk(mm)%spck)
+    case (28); call mv3(blk(mm)%vol)
     end select
 contains
     subroutine mv(a)
@@ -277,6 +281,75 @@ subroutine refh_average() bind(C, name="refh_average")
     implicit none
     call average1
     call average2
+end subroutine
+
+!! init_walldist after the face coordinates of every subface are stored as
+!! sub_init_readbcinfo.f90:294-400 stores them (writes result/dist_*.dat in
+!! the current directory)
+subroutine refh_walldist() bind(C, name="refh_walldist")
+    use blk_var
+    use glbindex_var
+    use flag_var
+    use index_var
+    use mpi_var
+    implicit none
+    integer :: mm, s, i, j, k, a, b, fc, na, nb
+    do mm = 1, blk_loop
+        do s = 1, blk0(mm)%num_subface
+            fc = blk0(mm)%bc0(s)%face
+            if (allocated(blk0(mm)%bc0(s)%x)) then
+                deallocate(blk0(mm)%bc0(s)%x, blk0(mm)%bc0(s)%y, blk0(mm)%bc0(s)%z)
+            end if
+            if (fc == 1 .or. fc == 2) then
+                na = blk0(mm)%bc0(s)%je - blk0(mm)%bc0(s)%js + 1
+                nb = blk0(mm)%bc0(s)%ke - blk0(mm)%bc0(s)%ks + 1
+            else if (fc == 3 .or. fc == 4) then
+                na = blk0(mm)%bc0(s)%ie - blk0(mm)%bc0(s)%is + 1
+                nb = blk0(mm)%bc0(s)%ke - blk0(mm)%bc0(s)%ks + 1
+            else
+                na = blk0(mm)%bc0(s)%ie - blk0(mm)%bc0(s)%is + 1
+                nb = blk0(mm)%bc0(s)%je - blk0(mm)%bc0(s)%js + 1
+            end if
+            allocate(blk0(mm)%bc0(s)%x(na, nb), blk0(mm)%bc0(s)%y(na, nb), blk0(mm)%bc0(s)%z(na, nb))
+            do k = blk0(mm)%bc0(s)%ks, blk0(mm)%bc0(s)%ke
+                do j = blk0(mm)%bc0(s)%js, blk0(mm)%bc0(s)%je
+                    do i = blk0(mm)%bc0(s)%is, blk0(mm)%bc0(s)%ie
+                        if (fc == 1 .or. fc == 2) then
+                            a = j - blk0(mm)%bc0(s)%js + 1; b = k - blk0(mm)%bc0(s)%ks + 1
+                        else if (fc == 3 .or. fc == 4) then
+                            a = i - blk0(mm)%bc0(s)%is + 1; b = k - blk0(mm)%bc0(s)%ks + 1
+                        else
+                            a = i - blk0(mm)%bc0(s)%is + 1; b = j - blk0(mm)%bc0(s)%js + 1
+                        end if
+                        !! the face plane itself (is0/ie0, js0/je0, ks0/ke0)
+                        blk0(mm)%bc0(s)%x(a, b) = blk(mm)%x(face_i(i), face_j(j), face_k(k))
+                        blk0(mm)%bc0(s)%y(a, b) = blk(mm)%y(face_i(i), face_j(j), face_k(k))
+                        blk0(mm)%bc0(s)%z(a, b) = blk(mm)%z(face_i(i), face_j(j), face_k(k))
+                    end do
+                end do
+            end do
+        end do
+    end do
+    call init_walldist
+contains
+    integer function face_i(i)
+        integer :: i
+        face_i = i
+        if (fc == 1) face_i = blk(mm)%is0
+        if (fc == 2) face_i = blk(mm)%ie0
+    end function
+    integer function face_j(j)
+        integer :: j
+        face_j = j
+        if (fc == 3) face_j = blk(mm)%js0
+        if (fc == 4) face_j = blk(mm)%je0
+    end function
+    integer function face_k(k)
+        integer :: k
+        face_k = k
+        if (fc == 5) face_k = blk(mm)%ks0
+        if (fc == 6) face_k = blk(mm)%ke0
+    end function
 end subroutine
 
 subroutine refh_update_coordinate() bind(C, name="refh_update_coordinate")

@@ -242,6 +242,21 @@ def init_spacing(ext, x, y, z):
     return spacing, vol
 
 
+def init_spacing_all(ext, x, y, z):
+    """spci, spcj, spck, spacing, vol of sub_init_spacing.f90:16-110."""
+    nk, nj, ni = ext.shape
+    out = [np.zeros((nk, nj, ni)) for _ in range(5)]
+    e = ext.as_list()
+    lib().init_spacing_(*[ptr(a) for a in out], ptr(x), ptr(y), ptr(z), *[_i(v) for v in e[:12]])
+    return out
+
+
+def walldist():
+    """init_walldist over the blocks given to begin() (coordinates put as
+    x, y, z); writes result/dist_*.dat in the current directory."""
+    lib().refh_walldist()
+
+
 def set_flag(name, value):
     """Set an integer flag of module flag_var directly."""
     C.c_int.in_dll(lib(), "_QMflag_varE" + name).value = int(value)

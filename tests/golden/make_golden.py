@@ -20,6 +20,11 @@ Cases (all small, a few hundred nodes per block):
   steps_s{solver}_t{turb}_a{tadv}  two 5^3 blocks with every physical
                condition and an interface: three whole time steps
                (main.f90:343-613)
+  grid         a warped 4-block grid with interface halos and wall faces:
+               gridtransformation, init_spacing, init_walldist (outputs
+               NaN where the reference leaves them undefined)
+
+    python tests/golden/make_golden.py [case ...]   # only the named cases
 """
 import json
 import os
@@ -208,6 +213,48 @@ def case_steps(solver, turb, tadv, nsteps=3):
 
 
 STEP_CASES = [(1, 0, 2), (0, 0, 2), (0, 2, 2), (0, 0, 3), (1, 0, 3)]
+GRID_PHYS = (-20, -40, -20, -10, -30, -20)
+
+
+def case_grid():
+    dims, subs, ranges, xyz = cases.warped_grid(10, 9, 7, cuts=((5,), (5,), ()), seed=3, phys=GRID_PHYS)
+    R.setup(solver=0, turb=2, itime=0)
+    exts = R.begin(dims, subs)
+    out = dict(dims=np.array(dims, dtype=np.int32), subfaces=subs_array(subs),
+               nsub=np.array([len(s) for s in subs], dtype=np.int32))
+    for m, e in enumerate(exts):
+        x, y, z = xyz[m]
+        nk, nj, ni = e.shape
+        K, J, I = np.meshgrid(np.arange(e.ks, e.ke + 1), np.arange(e.js, e.je + 1), np.arange(e.is_, e.ie + 1),
+                              indexing="ij")
+        out_i = (I < e.is0) | (I > e.ie0)
+        out_j = (J < e.js0) | (J > e.je0)
+        out_k = (K < e.ks0) | (K > e.ke0)
+        undefined = (out_i.astype(int) + out_j + out_k) > 1
+        halo = out_i | out_j | out_k
+        out["ext%d" % m] = np.array(e.as_list(), dtype=np.int32)
+        out["x%d" % m], out["y%d" % m], out["z%d" % m] = x, y, z
+        for name, a in zip(("dxidx", "inv_j", "aalpha", "bbeta", "ggamma"), R.gridtransformation(e, x, y, z)):
+            a = a.copy()
+            a[undefined] = np.nan
+            out["%s%d" % (name, m)] = a
+        for name, a in zip(("spci", "spcj", "spck", "spacing", "vol"), R.init_spacing_all(e, x, y, z)):
+            a = a.copy()
+            a[halo] = np.nan
+            out["%s%d" % (name, m)] = a
+        R.put(m + 1, "x", x)
+        R.put(m + 1, "y", y)
+        R.put(m + 1, "z", z)
+    R.walldist()
+    for m, e in enumerate(exts):
+        d = R.get(m + 1, "dist", e)
+        nk, nj, ni = e.shape
+        K, J, I = np.meshgrid(np.arange(e.ks, e.ke + 1), np.arange(e.js, e.je + 1), np.arange(e.is_, e.ie + 1),
+                              indexing="ij")
+        halo = (I < e.is0) | (I > e.ie0) | (J < e.js0) | (J > e.je0) | (K < e.ks0) | (K > e.ke0)
+        d[halo] = np.nan
+        out["dist%d" % m] = d
+    return out
 
 
 def main():
@@ -218,18 +265,25 @@ def main():
     os.chdir(work)  # the reference's step writes result/resi_history.dat
     os.makedirs("result", exist_ok=True)
     open("result/resi_history.dat", "w").close()
+    makers = {"kernels": case_kernels, "grid": case_grid}
+    for des in (0, 1, 2):
+        makers["sa_des%d" % des] = (lambda d=des: case_sa(d))
+    for ori in (1, 2, 3, 4):
+        makers["halo_ori%d" % ori] = (lambda o=ori: case_halo(o))
+    for s, t, a in STEP_CASES:
+        makers["steps_s%d_t%d_a%d" % (s, t, a)] = (lambda s=s, t=t, a=a: case_steps(s, t, a))
+    names = sys.argv[1:] or list(makers)
     made = {}
     try:
-        made["kernels"] = case_kernels()
-        for des in (0, 1, 2):
-            made["sa_des%d" % des] = case_sa(des)
-        for ori in (1, 2, 3, 4):
-            made["halo_ori%d" % ori] = case_halo(ori)
-        for s, t, a in STEP_CASES:
-            made["steps_s%d_t%d_a%d" % (s, t, a)] = case_steps(s, t, a)
+        for name in names:
+            made[name] = makers[name]()
     finally:
         os.chdir(cwd)
     manifest = {}
+    mpath = os.path.join(HERE, "manifest.json")
+    if sys.argv[1:] and os.path.exists(mpath):
+        with open(mpath) as fh:
+            manifest = json.load(fh)
     for name, arrs in made.items():
         path = os.path.join(HERE, name + ".npz")
         np.savez_compressed(path, **arrs)
@@ -239,8 +293,8 @@ def main():
                            "(oracle/Makefile target ref), driven by oracle/ref_harness.f90")
     with open(os.path.join(HERE, "manifest.json"), "w") as fh:
         json.dump(manifest, fh, indent=1, sort_keys=True)
-    print("wrote %d fixtures, %d bytes" % (len(made), sum(v["bytes"] for k, v in manifest.items()
-                                                          if not k.startswith("_"))))
+    print("wrote %d fixtures; %d bytes in all" % (len(made), sum(v["bytes"] for k, v in manifest.items()
+                                                                 if not k.startswith("_"))))
 
 
 if __name__ == "__main__":

@@ -26,7 +26,7 @@ def test_manifest_lists_every_fixture():
     with open(os.path.join(GOLD, "manifest.json")) as fh:
         man = json.load(fh)
     names = [k for k in man if not k.startswith("_")]
-    assert len(names) == 13
+    assert len(names) == 14
     for k in names:
         assert os.path.exists(os.path.join(GOLD, man[k]["file"]))
 
@@ -124,3 +124,31 @@ def test_steps_match_reference(name):
     for m in range(len(dims)):
         for k in STEP_OUTPUTS:
             assert np.array_equal(states[m][k], g["out_%s%d" % (k, m)]), (m, k)
+
+
+def same_where_defined(a, ref):
+    """Bitwise equality wherever the reference defines the output (the
+    fixture holds NaN elsewhere)."""
+    m = ~np.isnan(ref)
+    assert m.any()
+    return np.array_equal(a[m], ref[m])
+
+
+def test_grid_preprocessing_matches_reference():
+    """gridtransformation, init_spacing and init_walldist on the warped
+    4-block grid with interface halos and wall faces."""
+    g = load("grid")
+    exts = [ext_of(g["ext%d" % m]) for m in range(len(g["dims"]))]
+    subs, off = [], 0
+    for cnt in g["nsub"]:
+        subs.append(g["subfaces"][off:off + cnt].tolist())
+        off += cnt
+    xyz = [(g["x%d" % m], g["y%d" % m], g["z%d" % m]) for m in range(len(exts))]
+    wall = O.wall_points(exts, subs, xyz)
+    for m, e in enumerate(exts):
+        x, y, z = xyz[m]
+        for name, a in zip(("dxidx", "inv_j", "aalpha", "bbeta", "ggamma"), O.gridtransformation(e, x, y, z)):
+            assert same_where_defined(a, g["%s%d" % (name, m)]), (m, name)
+        for name, a in zip(("spci", "spcj", "spck", "spacing", "vol"), O.spacing(e, x, y, z)):
+            assert same_where_defined(a, g["%s%d" % (name, m)]), (m, name)
+        assert same_where_defined(O.walldist(e, x, y, z, wall), g["dist%d" % m]), m

@@ -201,3 +201,55 @@ def test_metrics_uniform_grid_match_closed_form():
     dx2, ij2, al2, be2, ga2 = cases.uniform_metrics(rng, (h, h, h))
     for a, b in ((dx, dx2), (ij, ij2), (al, al2), (be, be2), (ga, ga2)):
         assert np.allclose(a, b, rtol=1e-13, atol=1e-13)
+
+
+def grid_mask(e):
+    """Nodes with at most one coordinate outside the physical range."""
+    nk, nj, ni = e.shape
+    K, J, I = np.meshgrid(np.arange(e.ks, e.ke + 1), np.arange(e.js, e.je + 1), np.arange(e.is_, e.ie + 1),
+                          indexing="ij")
+    h = ((I < e.is0) | (I > e.ie0)).astype(int) + ((J < e.js0) | (J > e.je0)) + ((K < e.ks0) | (K > e.ke0))
+    return h <= 1
+
+
+def phys_mask(e):
+    m = np.zeros(e.shape, dtype=bool)
+    m[e.ks0 - e.ks:e.ke0 - e.ks + 1, e.js0 - e.js:e.je0 - e.js + 1, e.is0 - e.is_:e.ie0 - e.is_ + 1] = True
+    return m
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_grid_preprocessing_bitwise(seed, tmp_path):
+    """gridtransformation, init_spacing and init_walldist on a warped
+    multi-block grid with interface halos and wall faces: the oracle equals
+    the reference bit for bit where the reference defines its outputs."""
+    os.chdir(tmp_path)
+    os.makedirs("result", exist_ok=True)
+    dims, subs, ranges, xyz = cases.warped_grid(seed=seed, phys=(-20, -40, -20, -10, -30, -20))
+    R.setup(solver=0, turb=2, itime=0)
+    exts = R.begin(dims, subs)
+    for m, e in enumerate(exts):
+        assert list(e.as_list()) == list(ranges[m])
+        x, y, z = xyz[m]
+        a = R.gridtransformation(e, x, y, z)
+        b = O.gridtransformation(e, x, y, z)
+        mk = grid_mask(e)
+        for ra, ob in zip(a, b):
+            assert np.array_equal(ra[mk], ob[mk])
+        assert np.all(np.isfinite(b[1][mk])) and np.all(b[1][mk] > 0)
+        a = R.init_spacing_all(e, x, y, z)
+        b = O.spacing(e, x, y, z)
+        pm = phys_mask(e)
+        for ra, ob in zip(a, b):
+            assert np.array_equal(ra[pm], ob[pm])
+        R.put(m + 1, "x", x)
+        R.put(m + 1, "y", y)
+        R.put(m + 1, "z", z)
+    R.walldist()
+    wall = O.wall_points(exts, subs, xyz)
+    assert wall.shape[1] > 0
+    for m, e in enumerate(exts):
+        x, y, z = xyz[m]
+        d = O.walldist(e, x, y, z, wall)
+        pm = phys_mask(e)
+        assert np.array_equal(R.get(m + 1, "dist", e)[pm], d[pm])
