@@ -98,6 +98,11 @@ struct acfd_ctx {
     double *arena = nullptr;  // every block's fields
     long arena_n = 0;
     double *geo = nullptr;    // acfd_grid_metrics: coordinates and edge lengths of every block
+    // acfd_forces: wall cells (reference order), per-block offsets into geo, work
+    WallCell *d_wcells = nullptr;
+    int n_wcells = -1;
+    long *d_geo_off = nullptr;
+    double *d_fwork = nullptr;
     long totalnodes = 0;
     bool topo = false;
     // device scalars
@@ -116,6 +121,7 @@ struct acfd_ctx {
     // batch_blocks: block table and the per-kind grids of the batched launches
     BlockRec *d_blocks = nullptr;
     BatchGrid bg_getpri, bg_inv[4], bg_visc, bg_dt, bg_prep;
+    bool batch_ready = false;
     double *d_stage = nullptr;     // upload/download staging
     long stage_n = 0;
     // pointer tables for the multi-block copy kernels
@@ -402,12 +408,9 @@ void free_grid(BatchGrid &bg)
 
 long field_off(const acfd_ctx *ctx, const double *p) { return p ? (long)(p - ctx->arena) : -1; }
 
-// Block table and grids of the batched stage launches (batch_blocks).
-int build_batch(acfd_ctx *ctx)
+std::vector<BlockRec> block_records(const acfd_ctx *ctx)
 {
-    if (!ctx->cfg.batch_blocks) return 0;
     std::vector<BlockRec> rec;
-    std::vector<dim3> g_pri, g_zero, g_inv[3], g_visc, g_prep;
     for (auto &b : ctx->blk) {
         const Fields &f = b.f;
         BlockRec r;
@@ -423,23 +426,58 @@ int build_batch(acfd_ctx *ctx)
         o.w_flux = field_off(ctx, f.w_flux); o.w_dqs = field_off(ctx, f.w_dqs); o.w_dq = field_off(ctx, f.w_dq);
         o.w_diag = field_off(ctx, f.w_diag); o.w_r = field_off(ctx, f.w_r); o.w_sa = field_off(ctx, f.w_sa);
         rec.push_back(r);
+    }
+    return rec;
+}
+
+int ensure_block_table(acfd_ctx *ctx)
+{
+    if (ctx->d_blocks) return 0;
+    return dev_upload(&ctx->d_blocks, block_records(ctx));
+}
+
+// Block table and grids of the batched stage launches (batch_blocks).
+int build_batch(acfd_ctx *ctx)
+{
+    if (!ctx->cfg.batch_blocks) return 0;
+    std::vector<dim3> g_pri, g_zero, g_inv[3], g_visc, g_prep;
+    for (auto &b : ctx->blk) {
         g_pri.push_back(grid_1d(b.e.n));
         g_zero.push_back(grid_1d(5 * b.e.n));
         for (int a = 0; a < 3; a++) g_inv[a].push_back(inviscid_grid(b.e, a));
         g_visc.push_back(viscous_grid(b.e));
         g_prep.push_back(lu_prep_grid(b.e));
     }
-    int rc = dev_upload(&ctx->d_blocks, rec);
+    int rc = ensure_block_table(ctx);
     if (!rc) rc = make_grid(ctx->bg_getpri, g_pri);
     if (!rc) rc = make_grid(ctx->bg_inv[0], g_zero);
     for (int a = 0; a < 3 && !rc; a++) rc = make_grid(ctx->bg_inv[a + 1], g_inv[a]);
     if (!rc) rc = make_grid(ctx->bg_visc, g_visc);
     if (!rc) rc = make_grid(ctx->bg_dt, g_pri);
     if (!rc) rc = make_grid(ctx->bg_prep, g_prep);
+    ctx->batch_ready = rc == 0;
     return rc;
 }
 
-bool batched(const acfd_ctx *ctx) { return ctx->cfg.batch_blocks && ctx->d_blocks; }
+bool batched(const acfd_ctx *ctx) { return ctx->cfg.batch_blocks && ctx->batch_ready; }
+
+// Coordinates and edge lengths of every block (x, y, z, spci, spcj, spck).
+int ensure_geo(acfd_ctx *ctx)
+{
+    if (ctx->geo) return 0;
+    long tot = 0;
+    for (auto &b : ctx->blk) tot += 6 * b.e.n;
+    HIP_TRY(hipMalloc((void **)&ctx->geo, tot * sizeof(double)));
+    // on the context's stream: a null-stream memset is not ordered with the
+    // (non-blocking) context stream that then uploads the coordinates
+    HIP_TRY(hipMemsetAsync(ctx->geo, 0, tot * sizeof(double), ctx->stream));
+    long off = 0;
+    for (auto &b : ctx->blk) {
+        b.geo = ctx->geo + off;
+        off += 6 * b.e.n;
+    }
+    return 0;
+}
 
 // Workspace of the tile sweeps, sized for the largest block and every tile
 // shape: the sweep-layout operand array (ordinary device memory), the
@@ -479,14 +517,14 @@ int alloc_lu_ws(acfd_ctx *ctx)
     if (sl_n <= 0) return 0;
     HIP_TRY(hipMalloc((void **)&ctx->lu.sl, (size_t)sl_n * sizeof(double)));
     HIP_TRY(hipMalloc((void **)&ctx->lu.err, sizeof(int)));
-    HIP_TRY(hipMemset(ctx->lu.err, 0, sizeof(int)));
+    HIP_TRY(hipMemsetAsync(ctx->lu.err, 0, sizeof(int), ctx->stream));
     ctx->lu.ex = (double *)uc_acquire((size_t)ex_n * sizeof(double));
     if (!ctx->lu.ex) return fail(ACFD_EDEVICE, "uncached device memory for the tile exchange is unavailable");
     ctx->lu_ex_n = ex_n;
     KRN_TRY(launch_fill_sentinel(ctx->stream, ctx->lu.ex, ex_n));
     if (batch) {
         HIP_TRY(hipMalloc((void **)&ctx->lu_ctr, 2 * sizeof(int)));
-        HIP_TRY(hipMemset(ctx->lu_ctr, 0, 2 * sizeof(int)));
+        HIP_TRY(hipMemsetAsync(ctx->lu_ctr, 0, 2 * sizeof(int), ctx->stream));
         const bool sa = ctx->c.turb == 2 && ctx->c.solver == 0;
         for (int sh = 0; sh < lusgs_tile_shapes(); sh++) {
             if (lusgs_batch_make(nb, es.data(), fs.data(), ctx->arena, sh, ctx->lu.sl, ctx->lu.ex, &ctx->lub[sh]))
@@ -1158,8 +1196,9 @@ int acfd_create(const acfd_config *cfg, acfd_ctx **out)
     if (hipMalloc((void **)&ctx->d_res, 8 * sizeof(double)) != hipSuccess) return bad("alloc");
     double cfl = cfg->cfl;
     if (hipMemcpy(ctx->d_cfl, &cfl, sizeof(double), hipMemcpyHostToDevice) != hipSuccess) return bad("copy");
-    if (hipMemset(ctx->d_nneg, 0, sizeof(unsigned int)) != hipSuccess) return bad("memset");
-    if (hipMemset(ctx->d_res, 0, 8 * sizeof(double)) != hipSuccess) return bad("memset");
+    if (hipMemsetAsync(ctx->d_nneg, 0, sizeof(unsigned int), ctx->stream) != hipSuccess) return bad("memset");
+    if (hipMemsetAsync(ctx->d_res, 0, 8 * sizeof(double), ctx->stream) != hipSuccess) return bad("memset");
+    if (hipStreamSynchronize(ctx->stream) != hipSuccess) return bad("sync");
     unsigned long long st[2] = {0ull, ~0ull};
     if (hipMemcpy(ctx->d_stats, st, sizeof(st), hipMemcpyHostToDevice) != hipSuccess) return bad("copy");
     *out = ctx;
@@ -1173,6 +1212,8 @@ int acfd_destroy(acfd_ctx *ctx)
     if (ctx->graph_exec) (void)hipGraphExecDestroy(ctx->graph_exec);
     if (ctx->arena) (void)hipFree(ctx->arena);
     if (ctx->geo) (void)hipFree(ctx->geo);
+    for (void *p : {(void *)ctx->d_wcells, (void *)ctx->d_geo_off, (void *)ctx->d_fwork})
+        if (p) (void)hipFree(p);
     if (ctx->lu.ex) uc_release(ctx->lu.ex);
     if (ctx->lu.sl) (void)hipFree(ctx->lu.sl);
     if (ctx->lu.err) (void)hipFree(ctx->lu.err);
@@ -1378,6 +1419,10 @@ int acfd_upload(acfd_ctx *ctx, int m, int field, const double *host)
 {
     if (!ctx || !ctx->topo || m < 1 || m > (int)ctx->blk.size() || !host || field < 1 || field > kLastField)
         return fail(ACFD_EINVAL, "acfd_upload: bad arguments");
+    if (field >= ACFD_X) {
+        int rc = ensure_geo(ctx);
+        if (rc) return rc;
+    }
     Block &b = ctx->blk[m - 1];
     double *dst = field_ptr(b, field);
     if (!dst) return fail(ACFD_EINVAL, "acfd_upload: field not allocated for this configuration");
@@ -1557,15 +1602,9 @@ int acfd_grid_metrics(acfd_ctx *ctx, double *const *xyz)
     const int nb = (int)ctx->blk.size();
     for (int m = 0; m < nb; m++)
         if (!xyz[m]) return fail(ACFD_EINVAL, "acfd_grid_metrics: missing coordinates of a block");
-    if (!ctx->geo) {
-        long tot = 0;
-        for (auto &b : ctx->blk) tot += 6 * b.e.n;
-        HIP_TRY(hipMalloc((void **)&ctx->geo, tot * sizeof(double)));
-        long off = 0;
-        for (auto &b : ctx->blk) {
-            b.geo = ctx->geo + off;
-            off += 6 * b.e.n;
-        }
+    {
+        int rc = ensure_geo(ctx);
+        if (rc) return rc;
     }
     const bool sa = ctx->c.turb == 2 && ctx->c.solver == 0;
     for (int m = 0; m < nb; m++) {
@@ -1601,10 +1640,10 @@ int acfd_grid_metrics(acfd_ctx *ctx, double *const *xyz)
         }
         const long nw = (long)wx.size();
         double *dw = nullptr;
+        std::vector<double> w(wx);  // kept until the stream is synchronised below
+        w.insert(w.end(), wy.begin(), wy.end());
+        w.insert(w.end(), wz.begin(), wz.end());
         if (nw > 0) {
-            std::vector<double> w(wx);
-            w.insert(w.end(), wy.begin(), wy.end());
-            w.insert(w.end(), wz.begin(), wz.end());
             HIP_TRY(hipMalloc((void **)&dw, w.size() * sizeof(double)));
             HIP_TRY(hipMemcpyAsync(dw, w.data(), w.size() * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
         }
@@ -1620,6 +1659,47 @@ int acfd_grid_metrics(acfd_ctx *ctx, double *const *xyz)
         if (es != hipSuccess) return fail(ACFD_EDEVICE, std::string("acfd_grid_metrics: ") + hipGetErrorString(es));
     }
     HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+int acfd_forces(acfd_ctx *ctx, double sref, double *out)
+{
+    if (!ctx || !ctx->topo || !out) return fail(ACFD_EINVAL, "acfd_forces: bad arguments");
+    if (!ctx->geo) return fail(ACFD_ESTATE, "acfd_forces: no edge lengths (acfd_grid_metrics or upload ACFD_SPCI..K)");
+    if (ctx->n_wcells < 0) {
+        // wall cells in the reference's accumulation order: blocks, subfaces,
+        // outer then inner tangential index
+        std::vector<WallCell> cells;
+        for (int m = 0; m < (int)ctx->blk.size(); m++)
+            for (const acfd_subface &f : ctx->blk[m].sub) {
+                if (f.blk_t != -20) continue;
+                const int d = (f.face - 1) / 2;
+                const int r[3][2] = {{f.is, f.ie}, {f.js, f.je}, {f.ks, f.ke}};
+                const int t1 = d == 0 ? 1 : 0, t2 = d == 2 ? 1 : 2;
+                const int wall = (f.face & 1) ? r[d][0] : r[d][1];
+                for (int v = r[t2][0]; v <= r[t2][1] - 1; v++)
+                    for (int u = r[t1][0]; u <= r[t1][1] - 1; u++) cells.push_back(WallCell{m, f.face, wall, u, v, 0});
+            }
+        std::vector<long> goff;
+        for (auto &b : ctx->blk) goff.push_back((long)(b.geo - ctx->geo));
+        int rc = ensure_block_table(ctx);
+        if (!rc && !cells.empty()) rc = dev_upload(&ctx->d_wcells, cells);
+        if (!rc) rc = dev_upload(&ctx->d_geo_off, goff);
+        if (rc) return rc;
+        HIP_TRY(hipMalloc((void **)&ctx->d_fwork, (6 * cells.size() + 8) * sizeof(double)));
+        ctx->n_wcells = (int)cells.size();
+    }
+    double *out6 = ctx->d_fwork + 6L * ctx->n_wcells;
+    KRN_TRY(launch_wall_forces(ctx->stream, ctx->d_blocks, ctx->d_wcells, ctx->n_wcells, ctx->arena, ctx->geo,
+                               ctx->d_geo_off, ctx->c.p0, ctx->c.re, ctx->d_fwork, out6));
+    double f[6];
+    HIP_TRY(hipMemcpyAsync(f, out6, sizeof(f), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    // lift and drag (sub_output_forceintegration.f90:455-470), unit freestream
+    const double div = 1.0 / (0.5 * ctx->c.d0 * ctx->c.u0 * ctx->c.u0 * sref);
+    for (int c = 0; c < 6; c++) out[c] = f[c];
+    out[6] = ((f[1] + f[4]) * std::cos(ctx->c.aoa) - (f[0] + f[3]) * std::sin(ctx->c.aoa)) * div;
+    out[7] = ((f[1] + f[4]) * std::sin(ctx->c.aoa) + (f[0] + f[3]) * std::cos(ctx->c.aoa)) * div;
     return 0;
 }
 
@@ -1660,9 +1740,9 @@ int sensor_ready(acfd_ctx *ctx)
     HIP_TRY(hipMalloc((void **)&ctx->s_loss, sizeof(float)));
     HIP_TRY(hipMalloc((void **)&ctx->s_net, sensor_net_bytes()));
     HIP_TRY(hipMalloc((void **)&ctx->s_stats, 2 * sizeof(unsigned long long)));
-    HIP_TRY(hipMemset(ctx->s_stats, 0, 2 * sizeof(unsigned long long)));
-    HIP_TRY(hipMemset(ctx->s_m, 0, pb));
-    HIP_TRY(hipMemset(ctx->s_v, 0, pb));
+    HIP_TRY(hipMemsetAsync(ctx->s_stats, 0, 2 * sizeof(unsigned long long), ctx->stream));
+    HIP_TRY(hipMemsetAsync(ctx->s_m, 0, pb, ctx->stream));
+    HIP_TRY(hipMemsetAsync(ctx->s_v, 0, pb, ctx->stream));
     // He initialisation from a fixed generator (64-bit LCG + Box-Muller)
     std::vector<float> w(kSensorParams, 0.f);
     unsigned long long st = 0x9E3779B97F4A7C15ull;
@@ -1707,8 +1787,8 @@ int acfd_sensor_weights(acfd_ctx *ctx, float *w, int set)
     const size_t pb = kSensorParams * sizeof(float);
     if (set) {
         HIP_TRY(hipMemcpy(ctx->s_w, w, pb, hipMemcpyHostToDevice));
-        HIP_TRY(hipMemset(ctx->s_m, 0, pb));
-        HIP_TRY(hipMemset(ctx->s_v, 0, pb));
+        HIP_TRY(hipMemsetAsync(ctx->s_m, 0, pb, ctx->stream));
+        HIP_TRY(hipMemsetAsync(ctx->s_v, 0, pb, ctx->stream));
         ctx->s_adam_t = 0;
         KRN_TRY(launch_sensor_pack(ctx->stream, ctx->s_w, ctx->s_net));
         HIP_TRY(hipStreamSynchronize(ctx->stream));
@@ -1827,7 +1907,8 @@ int acfd_sensor_enable(acfd_ctx *ctx, int on, float threshold, int audit_interva
     ctx->s_off.assign(ctx->blk.size(), 0);
     ctx->s_audits = 0;
     ctx->s_audit_max = 0.0;
-    HIP_TRY(hipMemset(ctx->s_stats, 0, 2 * sizeof(unsigned long long)));
+    HIP_TRY(hipMemsetAsync(ctx->s_stats, 0, 2 * sizeof(unsigned long long), ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
     return 0;
 }
 

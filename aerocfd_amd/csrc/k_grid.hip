@@ -287,3 +287,95 @@ int launch_walldist(hipStream_t s, const Ext &e, const double *x, const double *
 }
 
 }  // namespace acfd
+
+namespace acfd {
+
+// ------------------------------------------------------------ wall forces
+// clcd_output (sub_output_forceintegration.f90:182-509, 3-D).  One thread
+// per wall cell evaluates the cell's pressure and viscous force exactly as
+// the reference does (velocity from the first interior layer, pressure and
+// viscosity on the wall, each averaged over the cell's four nodes; the
+// face-normal vector averaged from the unit metric of the four interior
+// nodes with the reference's mix of components: the second pair of nodes
+// takes the next component).  Six threads then add the cells up in the
+// reference's order, so the sums are the reference's to the last bit.
+__global__ void __launch_bounds__(256) k_wall_forces(const BlockRec *__restrict__ blks, const WallCell *__restrict__ cells,
+                                                     int ncell, double *arena, double *geo,
+                                                     const long *__restrict__ geo_off, double pinf, double re,
+                                                     double *__restrict__ f)
+{
+    const int x = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (x >= ncell) return;
+    const WallCell cl = cells[x];
+    const BlockRec &b = blks[cl.m];
+    const Ext e = b.e;
+    const long N = e.n;
+    const double *pri = arena + b.o.pri, *dxidx = arena + b.o.dxidx, *amu = arena + b.o.amu;
+    const double *spc = geo + geo_off[cl.m] + 3 * N;  // spci, spcj, spck
+    const int d = (cl.face - 1) / 2, c0 = 3 * d;
+    const double sg = (cl.face & 1) ? -1.0 : 1.0;
+    const int t1 = d == 0 ? 1 : 0, t2 = d == 2 ? 1 : 2;
+    const int in = (cl.face & 1) ? cl.wall + 1 : cl.wall - 1;
+    long W[4], I[4];
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+        int p[3];
+        p[d] = cl.wall;
+        p[t1] = cl.u + (c & 1);
+        p[t2] = cl.v + (c >> 1);
+        W[c] = e.ix(p[0], p[1], p[2]);
+        p[d] = in;
+        I[c] = e.ix(p[0], p[1], p[2]);
+    }
+    const double uc = (pri[N + I[0]] + pri[N + I[1]] + pri[N + I[2]] + pri[N + I[3]]) * 0.25;
+    const double vc = (pri[2 * N + I[0]] + pri[2 * N + I[1]] + pri[2 * N + I[2]] + pri[2 * N + I[3]]) * 0.25;
+    const double wc = (pri[3 * N + I[0]] + pri[3 * N + I[1]] + pri[3 * N + I[2]] + pri[3 * N + I[3]]) * 0.25;
+    const double pc = (pri[4 * N + W[0]] + pri[4 * N + W[1]] + pri[4 * N + W[2]] + pri[4 * N + W[3]]) * 0.25;
+    double t[4], M[4][3];
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+#pragma unroll
+        for (int k = 0; k < 3; k++) M[c][k] = dxidx[(c0 + k) * N + I[c]];
+        t[c] = 1.0 / sqrt(M[c][0] * M[c][0] + M[c][1] * M[c][1] + M[c][2] * M[c][2]);
+    }
+    const double nx = sg * (M[0][0] * t[0] + M[1][0] * t[1] + M[2][1] * t[2] + M[3][1] * t[3]) * 0.25;
+    const double ny = sg * (M[0][2] * t[0] + M[1][2] * t[1] + M[2][0] * t[2] + M[3][0] * t[3]) * 0.25;
+    const double nz = sg * (M[0][1] * t[0] + M[1][1] * t[1] + M[2][2] * t[2] + M[3][2] * t[3]) * 0.25;
+    const double miuc = (amu[2 * N + W[0]] + amu[2 * N + W[1]] + amu[2 * N + W[2]] + amu[2 * N + W[3]]) * 0.25;
+    const long w0 = W[0];
+    const double si = spc[w0], sj = spc[N + w0], sk = spc[2 * N + w0];
+    const double area = d == 0 ? sj * sk : d == 1 ? si * sk : si * sj;
+    const double dn = d == 0 ? si : d == 1 ? sj : sk;
+    // force_compute (:481-509)
+    const double un = uc * nx + vc * ny + wc * nz;
+    double *o = f + 6L * x;
+    o[0] = (pc - pinf) * area * nx;
+    o[1] = (pc - pinf) * area * ny;
+    o[2] = (pc - pinf) * area * nz;
+    o[3] = miuc * (uc - un * nx) / dn * area / re;
+    o[4] = miuc * (vc - un * ny) / dn * area / re;
+    o[5] = miuc * (wc - un * nz) / dn * area / re;
+}
+
+// out[c] = sum over cells of f[6 x + c], in cell order
+__global__ void k_wall_sum(const double *__restrict__ f, int ncell, double *__restrict__ out)
+{
+    const int c = (int)threadIdx.x;
+    if (c >= 6) return;
+    double s = 0.0;
+    for (int x = 0; x < ncell; x++) s = s + f[6L * x + c];
+    out[c] = s;
+}
+
+int launch_wall_forces(hipStream_t s, const BlockRec *blks, const WallCell *cells, int ncell, double *arena,
+                       double *geo, const long *geo_off, double pinf, double re, double *work, double *out6)
+{
+    if (ncell > 0)
+        hipLaunchKernelGGL(k_wall_forces, dim3((unsigned)((ncell + 255) / 256)), dim3(256), 0, s, blks, cells, ncell,
+                           arena, geo, geo_off, pinf, re, work);
+    hipLaunchKernelGGL(k_wall_sum, dim3(1), dim3(64), 0, s, work, ncell, out6);
+    ACFD_CHECK_LAUNCH();
+    return 0;
+}
+
+}  // namespace acfd

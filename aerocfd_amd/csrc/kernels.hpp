@@ -70,6 +70,13 @@ int launch_grid_spacing(hipStream_t s, const Ext &e, const double *x, const doub
                         double *spacing);
 int launch_walldist(hipStream_t s, const Ext &e, const double *x, const double *y, const double *z,
                     const double *wall, long nw, double *dist);
+// one cell of a wall subface: block index, face, wall-plane index and the
+// lower corner along the face's two tangential axes (in axis order)
+struct WallCell {
+    int m, face, wall, u, v, pad;
+};
+int launch_wall_forces(hipStream_t s, const BlockRec *blks, const WallCell *cells, int ncell, double *arena,
+                       double *geo, const long *geo_off, double pinf, double re, double *work, double *out6);
 
 // k_lusgs.hip
 // workspace of the tile sweeps: sweep-layout operands (normal memory),

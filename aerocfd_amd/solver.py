@@ -250,6 +250,13 @@ class Solver:
             ptrs[m] = _dp(a)
         _lib.check(self.lib.acfd_grid_metrics(self.h, ptrs))
 
+    def forces(self, sref=1.0):
+        """Wall forces of the resident state (clcd_output): dict with the
+        pressure force fp, viscous force fv (3 each), cl and cd."""
+        out = np.zeros(8)
+        _lib.check(self.lib.acfd_forces(self.h, C.c_double(sref), _dp(out)))
+        return {"fp": out[:3], "fv": out[3:6], "cl": out[6], "cd": out[7]}
+
     def exchange_coordinates(self, xyz, shifts=None):
         """Fill the coordinate halos of every block (UpdateBufferCoordinate).
         xyz: list per block of array (3, nk, nj, ni), updated in place;

@@ -200,6 +200,16 @@ int acfd_exchange_coordinates(acfd_ctx *ctx, double **xyz, const double *shift);
  * read by the path) the metrics are zero. */
 int acfd_grid_metrics(acfd_ctx *ctx, double *const *xyz);
 
+/* Wall forces and lift/drag of the resident state, replacing clcd_output
+ * (sub_output_forceintegration.f90:8-509, 3-D) without downloading it:
+ * every cell of every subface with blk_t = -20, evaluated as the reference
+ * does -- including its mix of metric components in the averaged face
+ * normal -- and summed in the reference's order.  Needs the edge lengths
+ * (acfd_grid_metrics, or ACFD_SPCI..ACFD_SPCK uploaded).  sref: the
+ * reference area of Control.in.  out[8] = pressure force x, y, z, viscous
+ * force x, y, z, cl, cd. */
+int acfd_forces(acfd_ctx *ctx, double sref, double *out);
+
 /* --------------------------------------------------- learned WCNS sensor */
 /* No counterpart in the reference.  A network 12 -> 32 -> 32 -> 1 reads the
  * smoothness indicators and jumps of density and pressure at the six nodes

@@ -168,6 +168,11 @@ void ora_gridtransformation(const ora_ext *e, const double *x, const double *y, 
 /* sub_init_spacing.f90:16-110 (physical range); vol may be NULL */
 void ora_spacing(const ora_ext *e, const double *x, const double *y, const double *z, double *spci, double *spcj,
                  double *spck, double *spacing, double *vol);
+/* wall forces of one block (sub_output_forceintegration.f90:182-453, 3-D),
+ * accumulated into fp[3], fv[3]; afp/afv accumulate the magnitudes */
+void ora_forces_block(const ora_ext *e, int nsub, const ora_subface *sub, const double *pri_v, const double *dxidx,
+                      const double *amu, const double *spci, const double *spcj, const double *spck, double pinf,
+                      double re, double *fp, double *fv, double *afp, double *afv);
 /* sub_init_walldist.f90:9-85 against nw wall points (physical range) */
 void ora_walldist(const ora_ext *e, const double *x, const double *y, const double *z, long nw, const double *xw,
                   const double *yw, const double *zw, double *dist);

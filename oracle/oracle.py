@@ -354,3 +354,25 @@ def wall_points(exts, subfaces, xyz):
             sl = (slice(k0 - e.ks, k1 - e.ks + 1), slice(j0 - e.js, j1 - e.js + 1), slice(i0 - e.is_, i1 - e.is_ + 1))
             pts.append(np.stack([x[sl].ravel(), y[sl].ravel(), z[sl].ravel()]))
     return np.concatenate(pts, axis=1) if pts else np.zeros((3, 0))
+
+
+def forces(exts, subfaces, fields, pinf, re, aoa_deg=0.0, sref=1.0):
+    """clcd_output (sub_output_forceintegration.f90:8-478, 3-D): fields[m]
+    holds pri_v, dxidx, amu, spci, spcj, spck of block m.  Returns fp, fv,
+    the term magnitudes afp, afv, and cl, cd (div = 1/(0.5 sref) with the
+    unit freestream)."""
+    fp, fv, afp, afv = (np.zeros(3) for _ in range(4))
+    for e, subs, f in zip(exts, subfaces, fields):
+        arr = (Subface * max(1, len(subs)))()
+        for i, s in enumerate(subs):
+            arr[i] = Subface(*[int(v) for v in s])
+        a = {k: np.ascontiguousarray(f[k], dtype=np.float64) for k in ("pri_v", "dxidx", "amu", "spci", "spcj",
+                                                                        "spck")}
+        lib().ora_forces_block(C.byref(e), C.c_int(len(subs)), C.cast(arr, C.POINTER(Subface)), ptr(a["pri_v"]),
+                               ptr(a["dxidx"]), ptr(a["amu"]), ptr(a["spci"]), ptr(a["spcj"]), ptr(a["spck"]),
+                               C.c_double(pinf), C.c_double(re), ptr(fp), ptr(fv), ptr(afp), ptr(afv))
+    aoa = np.deg2rad(aoa_deg)
+    div = 1.0 / (0.5 * 1.0 * 1.0 * 1.0 * sref)
+    cl = ((fp[1] + fv[1]) * np.cos(aoa) - (fp[0] + fv[0]) * np.sin(aoa)) * div
+    cd = ((fp[1] + fv[1]) * np.sin(aoa) + (fp[0] + fv[0]) * np.cos(aoa)) * div
+    return fp, fv, afp, afv, cl, cd

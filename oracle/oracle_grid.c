@@ -1,16 +1,22 @@
 /*
- * Grid pre-processing -- TEST INFRASTRUCTURE ONLY (see acfd_oracle.h).
+ * Grid pre-processing and wall forces -- TEST INFRASTRUCTURE ONLY (see
+ * acfd_oracle.h).
  *
  *   ora_gridtransformation  sub_init_gridtransformation.f90:27-874 (3-D branch)
  *   ora_spacing             sub_init_spacing.f90:16-110
  *   ora_walldist            sub_init_walldist.f90:9-85 (the distance part)
+ *   ora_forces_block        sub_output_forceintegration.f90:182-509 (3-D)
  *
  * Pinning: tests/test_oracle_vs_ref.py requires bitwise equality with the
  * reference's own routines (compiled from its sources into oracle/_ref) on
- * curvilinear multi-block grids with halos; tests/test_golden.py holds the
- * oracle to the committed fixture of the same cases; closed forms
- * (tests/test_oracle_grid.py): exact metrics of affine grids, metric
- * identities (dxidx . dx/dxi = I), the wall distance of a plane wall.
+ * curvilinear multi-block grids with halos and wall faces, and the lift and
+ * drag coefficients the reference's clcd_output writes;
+ * tests/test_golden.py holds the oracle to the committed fixture of the
+ * same cases; closed forms (tests/test_oracle_grid.py): exact metrics of
+ * affine grids, edge lengths of uniform grids, the wall distance of a plane
+ * wall and by brute force.  ora_forces_block has no closed form to meet:
+ * the reference's face-normal average mixes metric components (see the
+ * function), so it is pinned by the reference's own coefficients only.
  */
 #include <math.h>
 #include <stdlib.h>
@@ -229,4 +235,84 @@ void ora_walldist(const ora_ext *e, const double *x, const double *y, const doub
                 }
                 dist[n] = w;
             }
+}
+
+/* force_compute (sub_output_forceintegration.f90:481-509) */
+static void force_compute(double *fp, double *fv, double uc, double vc, double wc, double pc, double nx, double ny,
+                          double nz, double s, double dn, double pinf, double miuc, double re)
+{
+    fp[0] = (pc - pinf) * s * nx;
+    fp[1] = (pc - pinf) * s * ny;
+    fp[2] = (pc - pinf) * s * nz;
+    const double un = uc * nx + vc * ny + wc * nz;
+    fv[0] = miuc * (uc - un * nx) / dn * s / re;
+    fv[1] = miuc * (vc - un * ny) / dn * s / re;
+    fv[2] = miuc * (wc - un * nz) / dn * s / re;
+}
+
+/* The wall forces of one block (clcd_output, sub_output_forceintegration.f90:
+ * 182-453, 3-D): every cell of every wall subface (blk_t = -20), the
+ * velocity from the first interior layer and the pressure and viscosity on
+ * the wall, averaged over the cell's four nodes, against the face-normal
+ * vector averaged from the unit metric of the four interior nodes -- with
+ * the reference's own mix of components in that average (the second pair
+ * of nodes takes the next component), reproduced as is.  Accumulates into
+ * fp[3], fv[3] in the reference's order (subfaces, then the outer and inner
+ * cell loops), and the magnitudes of the terms into afp[3], afv[3]. */
+void ora_forces_block(const ora_ext *e, int nsub, const ora_subface *sub, const double *pri_v, const double *dxidx,
+                      const double *amu, const double *spci, const double *spcj, const double *spck, double pinf,
+                      double re, double *fp, double *fv, double *afp, double *afv)
+{
+    for (int s = 0; s < nsub; s++) {
+        const ora_subface *b = sub + s;
+        if (b->blk_t != -20) continue;
+        const int face = b->face;
+        const int d = (face - 1) / 2;          /* normal axis */
+        const int c0 = 3 * d;                  /* its metric row */
+        const double sg = (face & 1) ? -1.0 : 1.0;
+        /* tangential axes t1 < t2 and their ranges */
+        const int t1 = d == 0 ? 1 : 0, t2 = d == 2 ? 1 : 2;
+        const int r[3][2] = {{b->is, b->ie}, {b->js, b->je}, {b->ks, b->ke}};
+        const int wall = (face & 1) ? r[d][0] : r[d][1];
+        const int in = (face & 1) ? wall + 1 : wall - 1;
+        for (int v = r[t2][0]; v <= r[t2][1] - 1; v++)
+            for (int u = r[t1][0]; u <= r[t1][1] - 1; u++) {
+                long W[4], I[4];  /* corners 00, 10, 01, 11 on the wall and interior layers */
+                for (int c = 0; c < 4; c++) {
+                    int p[3];
+                    p[d] = wall;
+                    p[t1] = u + (c & 1);
+                    p[t2] = v + (c >> 1);
+                    W[c] = ora_ix(e, p[0], p[1], p[2]);
+                    p[d] = in;
+                    I[c] = ora_ix(e, p[0], p[1], p[2]);
+                }
+                const double uc = (pri_v[7 * I[0] + 1] + pri_v[7 * I[1] + 1] + pri_v[7 * I[2] + 1] + pri_v[7 * I[3] + 1]) * 0.25;
+                const double vc = (pri_v[7 * I[0] + 2] + pri_v[7 * I[1] + 2] + pri_v[7 * I[2] + 2] + pri_v[7 * I[3] + 2]) * 0.25;
+                const double wc = (pri_v[7 * I[0] + 3] + pri_v[7 * I[1] + 3] + pri_v[7 * I[2] + 3] + pri_v[7 * I[3] + 3]) * 0.25;
+                const double pc = (pri_v[7 * W[0] + 4] + pri_v[7 * W[1] + 4] + pri_v[7 * W[2] + 4] + pri_v[7 * W[3] + 4]) * 0.25;
+                double t[4];
+                for (int c = 0; c < 4; c++) {
+                    const double *m = dxidx + 9 * I[c] + c0;
+                    t[c] = 1.0 / sqrt(m[0] * m[0] + m[1] * m[1] + m[2] * m[2]);
+                }
+#define M(c, k) dxidx[9 * I[c] + c0 + (k)]
+                const double nx = sg * (M(0, 0) * t[0] + M(1, 0) * t[1] + M(2, 1) * t[2] + M(3, 1) * t[3]) * 0.25;
+                const double ny = sg * (M(0, 2) * t[0] + M(1, 2) * t[1] + M(2, 0) * t[2] + M(3, 0) * t[3]) * 0.25;
+                const double nz = sg * (M(0, 1) * t[0] + M(1, 1) * t[1] + M(2, 2) * t[2] + M(3, 2) * t[3]) * 0.25;
+#undef M
+                const double miuc = (amu[3 * W[0] + 2] + amu[3 * W[1] + 2] + amu[3 * W[2] + 2] + amu[3 * W[3] + 2]) * 0.25;
+                const long w0 = W[0];
+                const double area = d == 0 ? spcj[w0] * spck[w0] : d == 1 ? spci[w0] * spck[w0] : spci[w0] * spcj[w0];
+                const double dn = d == 0 ? spci[w0] : d == 1 ? spcj[w0] : spck[w0];
+                double f1[3], f2[3];
+                force_compute(f1, f2, uc, vc, wc, pc, nx, ny, nz, area, dn, pinf, miuc, re);
+                for (int x = 0; x < 3; x++) {
+                    fp[x] = fp[x] + f1[x];
+                    fv[x] = fv[x] + f2[x];
+                    afp[x] += fabs(f1[x]);
+                    afv[x] += fabs(f2[x]);
+                }
+            }
+    }
 }

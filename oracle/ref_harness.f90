@@ -352,6 +352,18 @@ contains
     end function
 end subroutine
 
+!! clcd_output with the reference area sref_ (it appends time, cl, cd to
+!! result/clcd_history.dat in the current directory, which must exist)
+subroutine refh_clcd(sref_) bind(C, name="refh_clcd")
+    use iso_c_binding
+    use ns_const
+    implicit none
+    real(c_double), value :: sref_
+    sref = sref_
+    timestep = 1
+    call clcd_output
+end subroutine
+
 subroutine refh_update_coordinate() bind(C, name="refh_update_coordinate")
     implicit none
     call UpdateBufferCoordinate

@@ -24,7 +24,7 @@ _DP = C.POINTER(C.c_double)
 XFER = {"q": 1, "pri_v": 2, "rhsi": 3, "rhsv": 4, "rhs": 5, "dudx": 6, "tao": 7, "vor": 8,
         "amu": 9, "dt": 10, "dxidx": 11, "inv_j": 12, "aalpha": 13, "bbeta": 14, "ggamma": 15,
         "dist": 16, "spacing": 17, "nut": 18, "sa_rhs": 19, "q0": 20, "length": 21, "x": 22,
-        "y": 23, "z": 24}
+        "y": 23, "z": 24, "spci": 25, "spcj": 26, "spck": 27, "vol": 28}
 NCOMP = {"q": 5, "pri_v": 7, "rhsi": 5, "rhsv": 5, "rhs": 5, "dudx": 9, "tao": 6, "amu": 3,
          "dxidx": 9, "aalpha": 3, "bbeta": 3, "ggamma": 3, "q0": 10}
 
@@ -260,3 +260,13 @@ def walldist():
 def set_flag(name, value):
     """Set an integer flag of module flag_var directly."""
     C.c_int.in_dll(lib(), "_QMflag_varE" + name).value = int(value)
+
+
+def clcd(sref):
+    """clcd_output (sub_output_forceintegration.f90:8-478) over the blocks
+    given to begin(); returns (time, cl, cd) as the reference writes them to
+    result/clcd_history.dat (list-directed, every digit of a double)."""
+    lib().refh_clcd(C.c_double(sref))
+    with open("result/clcd_history.dat") as fh:
+        last = fh.read().split()
+    return tuple(float(v.replace("D", "E")) for v in last[-3:])

@@ -23,6 +23,8 @@ Cases (all small, a few hundred nodes per block):
   grid         a warped 4-block grid with interface halos and wall faces:
                gridtransformation, init_spacing, init_walldist (outputs
                NaN where the reference leaves them undefined)
+  forces       the same kind of grid with walls on four sides and a random
+               state: clcd_output's lift and drag
 
     python tests/golden/make_golden.py [case ...]   # only the named cases
 """
@@ -257,6 +259,37 @@ def case_grid():
     return out
 
 
+FORCE_PHYS = (-20, -40, -20, -20, -30, -20)
+
+
+def case_forces():
+    dims, subs, ranges, xyz = cases.warped_grid(10, 9, 7, cuts=((5,), (5,), ()), seed=6, phys=FORCE_PHYS)
+    R.setup(solver=0, turb=2, itime=0, ma=0.3, re=2e5, aoa_deg=0.0)
+    exts = R.begin(dims, subs)
+    sc = R.scalars()
+    rng = np.random.default_rng(40)
+    out = dict(dims=np.array(dims, dtype=np.int32), subfaces=subs_array(subs),
+               nsub=np.array([len(s) for s in subs], dtype=np.int32),
+               params=np.array([sc["p0"], sc["re"], 0.0, 2.0]))  # pinf, re, aoa (deg), sref
+    for m, e in enumerate(exts):
+        x, y, z = xyz[m]
+        dx = R.gridtransformation(e, x, y, z)[0]
+        spci, spcj, spck = R.init_spacing_all(e, x, y, z)[:3]
+        q, _, _ = rand_state(e, rng, ma=0.3, amp=0.05)
+        pri = O.getpri(e, q, sc["gamma"], sc["cv"])[0]
+        amu = np.ones(e.shape + (3,))
+        amu[..., 2] += 0.1 * rng.random(e.shape)
+        f = dict(pri_v=pri, dxidx=dx, amu=amu, spci=spci, spcj=spcj, spck=spck)
+        out["ext%d" % m] = np.array(e.as_list(), dtype=np.int32)
+        for k, v in f.items():
+            R.put(m + 1, k, v)
+            out["%s%d" % (k, m)] = v
+    open("result/clcd_history.dat", "w").close()
+    t, cl, cd = R.clcd(2.0)
+    out["clcd"] = np.array([cl, cd])
+    return out
+
+
 def main():
     if not R.available():
         sys.exit("make_golden.py: the compiled reference (oracle/_ref) is not built; run make -C oracle ref")
@@ -265,7 +298,7 @@ def main():
     os.chdir(work)  # the reference's step writes result/resi_history.dat
     os.makedirs("result", exist_ok=True)
     open("result/resi_history.dat", "w").close()
-    makers = {"kernels": case_kernels, "grid": case_grid}
+    makers = {"kernels": case_kernels, "grid": case_grid, "forces": case_forces}
     for des in (0, 1, 2):
         makers["sa_des%d" % des] = (lambda d=des: case_sa(d))
     for ori in (1, 2, 3, 4):

@@ -26,7 +26,7 @@ def test_manifest_lists_every_fixture():
     with open(os.path.join(GOLD, "manifest.json")) as fh:
         man = json.load(fh)
     names = [k for k in man if not k.startswith("_")]
-    assert len(names) == 14
+    assert len(names) == 15
     for k in names:
         assert os.path.exists(os.path.join(GOLD, man[k]["file"]))
 
@@ -152,3 +152,17 @@ def test_grid_preprocessing_matches_reference():
         for name, a in zip(("spci", "spcj", "spck", "spacing", "vol"), O.spacing(e, x, y, z)):
             assert same_where_defined(a, g["%s%d" % (name, m)]), (m, name)
         assert same_where_defined(O.walldist(e, x, y, z, wall), g["dist%d" % m]), m
+
+
+def test_forces_match_reference():
+    g = load("forces")
+    exts = [ext_of(g["ext%d" % m]) for m in range(len(g["dims"]))]
+    subs, off = [], 0
+    for cnt in g["nsub"]:
+        subs.append(g["subfaces"][off:off + cnt].tolist())
+        off += cnt
+    fields = [{k: g["%s%d" % (k, m)] for k in ("pri_v", "dxidx", "amu", "spci", "spcj", "spck")}
+              for m in range(len(exts))]
+    pinf, re, aoa, sref = g["params"]
+    cl, cd = O.forces(exts, subs, fields, pinf, re, aoa, sref)[4:]
+    assert cl == g["clcd"][0] and cd == g["clcd"][1]

@@ -253,3 +253,35 @@ def test_grid_preprocessing_bitwise(seed, tmp_path):
         d = O.walldist(e, x, y, z, wall)
         pm = phys_mask(e)
         assert np.array_equal(R.get(m + 1, "dist", e)[pm], d[pm])
+
+
+@pytest.mark.parametrize("seed", [0, 4])
+def test_forces_match_reference(seed, tmp_path):
+    """clcd_output on a warped multi-block grid with wall faces on several
+    sides: the oracle's lift and drag equal the coefficients the reference
+    writes (every digit of the double)."""
+    os.chdir(tmp_path)
+    os.makedirs("result", exist_ok=True)
+    open("result/clcd_history.dat", "w").close()
+    dims, subs, ranges, xyz = cases.warped_grid(seed=seed, phys=(-20, -40, -20, -20, -30, -20))
+    R.setup(solver=0, turb=2, itime=0, ma=0.3, re=2e5, aoa_deg=0.0)
+    exts = R.begin(dims, subs)
+    sc = R.scalars()
+    rng = np.random.default_rng(seed)
+    fields = []
+    for m, e in enumerate(exts):
+        x, y, z = xyz[m]
+        dx, ij, al, be, ga = R.gridtransformation(e, x, y, z)
+        spci, spcj, spck, spacing, vol = R.init_spacing_all(e, x, y, z)
+        q, _, _ = rand_state(e, rng, ma=0.3, amp=0.05)
+        pri = O.getpri(e, q, sc["gamma"], sc["cv"])[0]
+        amu = np.ones(e.shape + (3,))
+        amu[..., 2] += 0.1 * rng.random(e.shape)
+        f = dict(pri_v=pri, dxidx=dx, amu=amu, spci=spci, spcj=spcj, spck=spck)
+        for k, v in f.items():
+            R.put(m + 1, k, v)
+        fields.append(f)
+    t, cl, cd = R.clcd(2.0)
+    fp, fv, afp, afv, ocl, ocd = O.forces(exts, subs, fields, sc["p0"], sc["re"], 0.0, 2.0)
+    assert afp.sum() > 0 and afv.sum() > 0
+    assert ocl == cl and ocd == cd, (ocl, cl, ocd, cd)
