@@ -352,6 +352,33 @@ contains
     end function
 end subroutine
 
+!! init_readcontrolfile (reads input/Control.in in the current directory)
+!! and the values it set, in the order of the file: integers in iv, reals
+!! in rv, the three file names in sv (180 characters each)
+subroutine refh_read_control(iv, rv, sv) bind(C, name="refh_read_control")
+    use iso_c_binding
+    use flag_var
+    use ns_const
+    use output_var
+    implicit none
+    integer(c_int) :: iv(22)
+    real(c_double) :: rv(17)
+    character(kind=c_char) :: sv(540)
+    integer :: c
+    call init_readcontrolfile
+    iv = [iflag_dimension, iflag_solver, iflag_time, iflag_splittingtype, iflag_inviscid, iflag_filter, &
+          iflag_timeadvance, iflag_turbulence, iflag_des, writeinterval, ntstep, iflag_init, nfile, &
+          restart_timestep, nobs, iflag_acoustic, Start_Acsoutic, End_Acsoutic, iflag_avg, Start_Avg, End_Avg, &
+          Avg_interval]
+    rv = [cfl, dt, ma, re, tinf, aoa, scale_x, scale_y, scale_z, lref, sref, fltr, ob_dist, acoustic_length, &
+          0.d0, 0.d0, 0.d0]
+    do c = 1, 180
+        sv(c) = grid(c:c)
+        sv(180 + c) = bcinfo(c:c)
+        sv(360 + c) = restartfile(c:c)
+    end do
+end subroutine
+
 !! clcd_output with the reference area sref_ (it appends time, cl, cd to
 !! result/clcd_history.dat in the current directory, which must exist)
 subroutine refh_clcd(sref_) bind(C, name="refh_clcd")

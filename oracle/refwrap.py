@@ -270,3 +270,26 @@ def clcd(sref):
     with open("result/clcd_history.dat") as fh:
         last = fh.read().split()
     return tuple(float(v.replace("D", "E")) for v in last[-3:])
+
+
+CONTROL_INTS = ("iflag_dimension", "iflag_solver", "iflag_time", "iflag_splittingtype", "iflag_inviscid",
+                "iflag_filter", "iflag_timeadvance", "iflag_turbulence", "iflag_des", "writeinterval", "ntstep",
+                "iflag_init", "nfile", "restart_timestep", "nobs", "iflag_acoustic", "Start_Acsoutic",
+                "End_Acsoutic", "iflag_avg", "Start_Avg", "End_Avg", "Avg_interval")
+CONTROL_REALS = ("cfl", "dt", "ma", "re", "tinf", "aoa", "scale_x", "scale_y", "scale_z", "lref", "sref", "fltr",
+                 "ob_dist", "acoustic_length")
+
+
+def read_control():
+    """init_readcontrolfile on input/Control.in of the current directory;
+    the values it set, by name."""
+    iv = np.zeros(22, dtype=np.int32)
+    rv = np.zeros(17)
+    sv = C.create_string_buffer(540)
+    lib().refh_read_control(iv.ctypes.data_as(C.POINTER(C.c_int)), ptr(rv), sv)
+    out = {k: int(v) for k, v in zip(CONTROL_INTS, iv)}
+    out.update({k: float(v) for k, v in zip(CONTROL_REALS, rv)})
+    raw = sv.raw.decode("latin-1")
+    for n, k in enumerate(("grid", "bcinfo", "restartfile")):
+        out[k] = raw[180 * n:180 * (n + 1)].rstrip()
+    return out
