@@ -329,7 +329,10 @@ __global__ void k_residual_final(const double *__restrict__ partial, int count, 
 {
     int v = threadIdx.x;
     if (v >= 6) return;
+    // in index order (the sums are reproducible); unrolled so that the
+    // loads are issued ahead of the dependent additions
     double s = 0.0;
+#pragma unroll 16
     for (int x = 0; x < count; x++) s += partial[6 * x + v];
     out[v] = s;
 }

@@ -363,6 +363,7 @@ __global__ void k_wall_sum(const double *__restrict__ f, int ncell, double *__re
     const int c = (int)threadIdx.x;
     if (c >= 6) return;
     double s = 0.0;
+#pragma unroll 16
     for (int x = 0; x < ncell; x++) s = s + f[6L * x + c];
     out[c] = s;
 }

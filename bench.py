@@ -68,8 +68,8 @@ MEASURED_TRAFFIC_256 = {
 
 
 CPU_SAMPLES = {
-    "tgv": ("cases.tgv_box(96)", 8, "TGV 96^3 periodic block"),
-    "flat_plate": ("cases.flat_plate(nblk=(4, 3, 1), lo=(24, 20, 12), hi=(48, 32, 16))", 60,
+    "tgv": ("cases.tgv_box(96)", 24, "TGV 96^3 periodic block"),
+    "flat_plate": ("cases.flat_plate(nblk=(4, 3, 1), lo=(24, 20, 12), hi=(48, 32, 16))", 150,
                    "flat plate, the same 12-block layout at reduced node counts"),
 }
 
