@@ -560,15 +560,22 @@ __device__ inline void lu_node(const double *op, const double *ta, const double 
 // the same tiles mirrored (lane (tj,tk) schedules as (TJ-1-tj, TK-1-tk)) and
 // is dispatched in reverse order.  Either way every wait is on a tile that
 // was dispatched earlier, so it is running or done.
+// With q (backward sweep only) the update q += dq is fused: each node's dq
+// is added to its q operand and stored into the natural layout (e: the
+// block's ranges), in place of writing dq to the sweep layout.
 template <int DIR, int TJ, int TK>
 __device__ inline void lu_tiles_body(const SlGeom &g, int tile, double *__restrict__ sl, double gamma, double *ex,
-                                     int *err, double (*pj)[5][TJ * TK], double (*pk)[5][TJ * TK])
+                                     int *err, double (*pj)[5][TJ * TK], double (*pk)[5][TJ * TK],
+                                     const Ext *e = nullptr, double *__restrict__ q = nullptr)
 {
     constexpr int TT = TJ * TK;
     const int bj = tile % g.nbj, bk = tile / g.nbj;
     const int l = threadIdx.x, tj = l % TJ, tk = l / TJ;
     const int lj = bj * TJ + tj, lk = bk * TK + tk;
     const bool active = lj < g.Lj && lk < g.Lk;
+    const bool fuse_add = DIR < 0 && q != nullptr;
+    const long qN = fuse_add ? e->n : 0;
+    const long nat0 = fuse_add && active ? e->ix(e->is1, e->js1 + lj, e->ks1 + lk) : 0;
     // schedule coordinates (mirrored for the backward sweep)
     const int sj = DIR > 0 ? tj : TJ - 1 - tj, sk = DIR > 0 ? tk : TK - 1 - tk;
     // the j / k predecessor: outside the computed range (0), in the
@@ -645,9 +652,16 @@ __device__ inline void lu_tiles_body(const SlGeom &g, int tile, double *__restri
                 for (int x = 0; x < 5; x++) tc[x] = pk[p ^ 1][x][lpk];
             }
             lu_node<DIR>(op, ta, tb, tc, gamma, res, pa, pb, pc);
+            if (fuse_add) {
+                const long n = nat0 + (g.Li - 1 - pos);
+#pragma unroll
+                for (int x = 0; x < 5; x++) q[x * qN + n] = op[x] + res[x];
+            } else {
+#pragma unroll
+                for (int x = 0; x < 5; x++) sl[g.slot(tile, row(s), kSlOut + x, l)] = res[x];
+            }
 #pragma unroll
             for (int x = 0; x < 5; x++) {
-                sl[g.slot(tile, row(s), kSlOut + x, l)] = res[x];
                 ta[x] = pa[x];
                 pj[p][x][l] = pb[x];
                 pk[p][x][l] = pc[x];
@@ -680,11 +694,11 @@ __device__ inline void lu_tiles_body(const SlGeom &g, int tile, double *__restri
 
 template <int DIR, int TJ, int TK>
 __global__ void __launch_bounds__(TJ *TK) k_lu_tiles(SlGeom g, double *__restrict__ sl, double gamma,
-                                                      double *ex, int *err)
+                                                      double *ex, int *err, Ext e, double *q)
 {
     __shared__ double pj[2][5][TJ * TK], pk[2][5][TJ * TK];
     const int tile = DIR > 0 ? (int)blockIdx.x : g.ntiles - 1 - (int)blockIdx.x;
-    lu_tiles_body<DIR, TJ, TK>(g, tile, sl, gamma, ex, err, pj, pk);
+    lu_tiles_body<DIR, TJ, TK>(g, tile, sl, gamma, ex, err, pj, pk, &e, q);
 }
 
 // ------------------------------------------------------------ batched tiles
@@ -744,7 +758,8 @@ template <int DIR, int TJ, int TK>
 __global__ void __launch_bounds__(TJ *TK) k_lu_tiles_batch(const LuBatchEnt *__restrict__ ents,
                                                             const int *__restrict__ tile_ent, int total,
                                                             int resident, double *slb, double *exb,
-                                                            double gamma, int *err, int *ctr)
+                                                            double gamma, int *err, int *ctr, double *arena,
+                                                            int fuse_add)
 {
     __shared__ double pj[2][5][TJ * TK], pk[2][5][TJ * TK];
     __shared__ int ticket;
@@ -757,8 +772,9 @@ __global__ void __launch_bounds__(TJ *TK) k_lu_tiles_batch(const LuBatchEnt *__r
     const LuBatchEnt &b = ents[tile_ent[flat]];
     const SlGeom g = b.g;
     const int local = flat - b.tile0;
+    const Ext e = b.e;
     lu_tiles_body<DIR, TJ, TK>(g, DIR > 0 ? local : g.ntiles - 1 - local, slb + b.sl, gamma, exb + b.ex, err, pj,
-                               pk);
+                               pk, &e, fuse_add ? arena + b.q : nullptr);
     if (!resident && threadIdx.x == 0 &&
         __hip_atomic_fetch_add(ctr + 1, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == total - 1) {
         __hip_atomic_store(ctr, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -976,6 +992,24 @@ void lusgs_ws_sizes(const Ext &e, long *sl_n, long *ex_n)
     *ex_n = b;
 }
 
+// Whether the backward sweep adds dq to q itself (and the add pass is
+// skipped).  Its stores then go to the natural layout, one line per lane:
+// measured cheaper than the separate pass on blocks of up to a few million
+// nodes, dearer on larger ones (256^3: +1.0 ms in the sweep against 0.64 ms
+// saved), so it is chosen by the largest block's computed nodes;
+// ACFD_LU_FUSED_ADD=0/1 forces it.
+constexpr long kFuseAddMaxNodes = 1L << 22;
+static bool fused_add(long max_nodes)
+{
+    const char *v = std::getenv("ACFD_LU_FUSED_ADD");
+    if (v) return std::atoi(v) != 0;
+    return max_nodes < kFuseAddMaxNodes;
+}
+static long computed_nodes(const Ext &e)
+{
+    return (long)(e.ie1 - e.is1 + 1) * (e.je1 - e.js1 + 1) * (e.ke1 - e.ks1 + 1);
+}
+
 static long round_up32(long n) { return (n + 31) / 32 * 32; }
 static void batch_residency(LuBatch *b);
 
@@ -1037,6 +1071,9 @@ int lusgs_batch_make(int nblk, const Ext *e, const Fields *f, double *arena, int
     out->total = (int)te.size();
     if (ents.empty()) return 0;
     batch_residency(out);
+    long maxn = 0;
+    for (const LuBatchEnt &b : ents) maxn = std::max(maxn, computed_nodes(b.e));
+    out->fuse_add = fused_add(maxn) ? 1 : 0;
     if (hipMalloc(&out->ents, ents.size() * sizeof(LuBatchEnt)) != hipSuccess) return 1;
     if (hipMalloc((void **)&out->tile_ent, te.size() * sizeof(int)) != hipSuccess) return 1;
     if (hipMemcpy(out->ents, ents.data(), ents.size() * sizeof(LuBatchEnt), hipMemcpyHostToDevice) != hipSuccess ||
@@ -1072,7 +1109,7 @@ static void launch_tiles_batch(hipStream_t s, const LuBatch &b, double gamma, in
 {
     hipLaunchKernelGGL((k_lu_tiles_batch<DIR, TJ, TK>), dim3((unsigned)b.total), dim3(TJ * TK), 0, s,
                        (const LuBatchEnt *)b.ents, b.tile_ent, b.total, b.resident[DIR > 0 ? 0 : 1], b.sl, b.ex,
-                       gamma, err, ctr);
+                       gamma, err, ctr, b.arena, DIR < 0 && b.fuse_add ? 1 : 0);
 }
 
 static void batch_residency(LuBatch *b)
@@ -1123,6 +1160,7 @@ int launch_lusgs_batch(hipStream_t s, const LuBatch &b, int phase, double gamma,
     const int tt = kTileShapes[b.shape][0] * kTileShapes[b.shape][1];
     if (phase == 2) launch_tiles_batch_shape<1>(s, b, gamma, err, ctr);
     else if (phase == 3) launch_tiles_batch_shape<-1>(s, b, gamma, err, ctr);
+    else if (phase == 4 && b.fuse_add) return 0;
     else {
         const int which = phase == 1 ? 0 : 1;
         if (tt == 256) launch_layout_batch<256>(s, which, b);
@@ -1252,20 +1290,21 @@ int launch_fill_sentinel(hipStream_t s, double *p, long n)
 }
 
 template <int DIR, int TJ, int TK>
-static void launch_tiles(hipStream_t s, const SlGeom &g, const LuWs &ws, double gamma)
+static void launch_tiles(hipStream_t s, const SlGeom &g, const LuWs &ws, double gamma, const Ext &e, double *q)
 {
     hipLaunchKernelGGL((k_lu_tiles<DIR, TJ, TK>), dim3((unsigned)g.ntiles), dim3(TJ * TK), 0, s, g, ws.sl, gamma,
-                       ws.ex, ws.err);
+                       ws.ex, ws.err, e, q);
 }
 
 template <int DIR>
-static void launch_tiles_shape(hipStream_t s, int shape, const SlGeom &g, const LuWs &ws, double gamma)
+static void launch_tiles_shape(hipStream_t s, int shape, const SlGeom &g, const LuWs &ws, double gamma,
+                               const Ext &e, double *q)
 {
     switch (shape) {
-    case 0: launch_tiles<DIR, 16, 16>(s, g, ws, gamma); break;
-    case 1: launch_tiles<DIR, 16, 8>(s, g, ws, gamma); break;
-    case 2: launch_tiles<DIR, 8, 8>(s, g, ws, gamma); break;
-    default: launch_tiles<DIR, 16, 4>(s, g, ws, gamma); break;
+    case 0: launch_tiles<DIR, 16, 16>(s, g, ws, gamma, e, q); break;
+    case 1: launch_tiles<DIR, 16, 8>(s, g, ws, gamma, e, q); break;
+    case 2: launch_tiles<DIR, 8, 8>(s, g, ws, gamma, e, q); break;
+    default: launch_tiles<DIR, 16, 4>(s, g, ws, gamma, e, q); break;
     }
 }
 
@@ -1273,7 +1312,8 @@ static void launch_tiles_shape(hipStream_t s, int shape, const SlGeom &g, const 
 //   0 rhs, spectral radii, diag (hyperplanes also clear dq* and dq),
 //   1 tiles: gather the operands into the sweep layout,
 //   2 forward sweep, 3 backward sweep,
-//   4 q += dq (tiles: from the sweep layout).
+//   4 q += dq (tiles: from the sweep layout, or fused into the backward
+//     sweep, which then stores q + dq itself).
 // variant 0 = hyperplanes, 1 = tiles of the given shape index.
 int launch_lusgs_phase(hipStream_t s, const Ext &e, const Fields &f, const Consts &c, const double *cfl,
                        int variant, const LuWs *ws, int phase, int shape)
@@ -1291,10 +1331,10 @@ int launch_lusgs_phase(hipStream_t s, const Ext &e, const Fields &f, const Const
         if (phase == 1)
             launch_layout_tt(s, 0, e, g, f, *ws);
         else if (phase == 2)
-            launch_tiles_shape<1>(s, shape, g, *ws, c.gamma);
+            launch_tiles_shape<1>(s, shape, g, *ws, c.gamma, e, nullptr);
         else if (phase == 3)
-            launch_tiles_shape<-1>(s, shape, g, *ws, c.gamma);
-        else
+            launch_tiles_shape<-1>(s, shape, g, *ws, c.gamma, e, fused_add(computed_nodes(e)) ? f.q : nullptr);
+        else if (!fused_add(computed_nodes(e)))
             launch_layout_tt(s, 1, e, g, f, *ws);
     } else if (phase == 4) {
         Box b{e.is1, e.ie1, e.js1, e.je1, e.ks1, e.ke1};

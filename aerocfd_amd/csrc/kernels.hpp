@@ -102,6 +102,7 @@ struct LuBatch {
     double *arena = nullptr, *sl = nullptr, *ex = nullptr;  // bases of the table's offsets
     int nent = 0, total = 0, shape = 0;
     int resident[2] = {0, 0};  // forward / backward sweep: every workgroup fits on the device at once
+    int fuse_add = 0;          // the backward sweep stores q + dq (no add pass)
 };
 void lusgs_batch_ws(int nblk, const Ext *e, int shape, long *sl_n, long *ex_n);
 // f[m]'s fields must lie in the allocation starting at arena

@@ -47,6 +47,10 @@ BYTES_PER_NODE = {
     "lusgs_add": 40 + 40 + 40,                      # q, dq -> q
     "lusgs_other": 32 + 72 + 8 + 80 + 40 + 32,      # u,v,w,c, dxidx, dt, rhsi+rhsv -> rhs, r_abc, diag
 }
+# the tile sweep backward pass adds dq to q itself on blocks under 2^22
+# computed nodes: q in and q out instead of dq out (the add pass is skipped)
+FUSED_ADD_MAX_NODES = 1 << 22
+FUSED_BACKWARD_BYTES = 24 * 8 + 40 + 40
 HYPERPLANE_BYTES = {  # the hyperplane sweeps read the operands in place
     "lusgs_forward": 40 + 72 + 8 + 24 + 8 + 40 + 40,
     "lusgs_backward": 40 + 72 + 8 + 24 + 8 + 40 + 40,
@@ -239,6 +243,11 @@ def main():
     table = dict(BYTES_PER_NODE)
     if args.lusgs == 0:
         table.update(HYPERPLANE_BYTES)
+    else:
+        env = os.environ.get("ACFD_LU_FUSED_ADD")
+        comp_max = max((r[13] - r[12] + 1) * (r[15] - r[14] + 1) * (r[17] - r[16] + 1) for r in rr)
+        if (int(env) != 0) if env is not None else comp_max < FUSED_ADD_MAX_NODES:
+            table["lusgs_backward"] = FUSED_BACKWARD_BYTES
     cand = {k: v for k, v in tm.items() if k in table and v[1] > 0}
     dom = max(cand, key=lambda k: cand[k][0])
     avg_ms = cand[dom][0] / cand[dom][1]
