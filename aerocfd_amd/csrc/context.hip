@@ -116,6 +116,15 @@ struct acfd_ctx {
     LuBatch lub[4];          // lusgs_kernel 2: every block in one launch, per tile shape
     LuBatch sab[4];          // the same for the SA model's scalar sweeps
     int lub_shape = -1;      // chosen by the autotuner on first use
+    // what the sweep layout holds of the grid operands: the block of the last
+    // gather (-1: the batch, -2: nothing valid) and its tiling
+    int sl_block = -2, sl_shape = -1;
+    bool capturing = false;
+    // the captured step gathers without the grid operands: it replays only
+    // onto the contents it was captured on (sl_block, sl_shape at its start)
+    // and leaves those of its end
+    bool graph_geo_cached = false;
+    int graph_sl_in[2] = {-2, -1}, graph_sl_out[2] = {-2, -1};
     int *lu_ctr = nullptr;   // ticket counters of the batched sweeps
     double *d_cfl_blk = nullptr;  // cfl after each block's find_dt
     // batch_blocks: block table and the per-kind grids of the batched launches
@@ -691,6 +700,37 @@ constexpr int kTimeLuForward = 12, kTimeLuBackward = 13, kTimeLuGather = 14, kTi
 // and forward sweep once per tile shape (they recompute dq* in the sweep
 // layout from unchanged inputs, so repeating them changes nothing) and keep
 // the fastest shape.
+bool geo_cache_on()
+{
+    static const bool on = [] {
+        const char *v = std::getenv("ACFD_SL_GEO_CACHE");
+        return !(v && std::atoi(v) == 0);
+    }();
+    return on;
+}
+
+// Whether the sweep layout already holds the grid operands (dxidx, inv_j)
+// of `block` (-1: the batch) in tiling `shape`, so that the gather about to
+// be launched may leave them in place; records that it holds them after.
+// The sweeps write only the result fields, and the grid operands change
+// only through sl_invalidate's callers.  ACFD_SL_GEO_CACHE=0 gathers every
+// field every time.
+int sl_geo_cached(acfd_ctx *ctx, int block, int shape)
+{
+    const bool same = ctx->sl_block == block && ctx->sl_shape == shape && geo_cache_on();
+    ctx->sl_block = block;
+    ctx->sl_shape = shape;
+    if (same && ctx->capturing) ctx->graph_geo_cached = true;
+    return same ? 1 : 0;
+}
+
+// the metrics of some block were written: the next gathers move every field
+void sl_invalidate(acfd_ctx *ctx)
+{
+    ctx->sl_block = -2;
+    ctx->sl_shape = -1;
+}
+
 int tune_tiles(acfd_ctx *ctx, Block &b)
 {
     if (const char *fix = std::getenv("ACFD_TILE_SHAPE")) {
@@ -708,7 +748,8 @@ int tune_tiles(acfd_ctx *ctx, Block &b)
     for (int s = 0; s < lusgs_tile_shapes(); s++) {
         if (lusgs_tiles(b.e, s) <= 0) continue;
         (void)hipEventRecord(e0, ctx->stream);
-        int rc = launch_lusgs_phase(ctx->stream, b.e, b.f, ctx->c, ctx->d_cfl, 1, &ctx->lu, 1, s);
+        int rc = launch_lusgs_phase(ctx->stream, b.e, b.f, ctx->c, ctx->d_cfl, 1, &ctx->lu, 1, s,
+                                    sl_geo_cached(ctx, (int)(&b - ctx->blk.data()), s));
         if (!rc) rc = launch_lusgs_phase(ctx->stream, b.e, b.f, ctx->c, ctx->d_cfl, 1, &ctx->lu, 2, s);
         (void)hipEventRecord(e1, ctx->stream);
         if (rc) return fail(ACFD_EDEVICE, "tile sweep launch failed during tuning");
@@ -740,8 +781,9 @@ int st_implicit(acfd_ctx *ctx, int m, const double *cfl)
         }
         StageTimer ts(ctx, ph == 2 ? kTimeLuForward : ph == 3 ? kTimeLuBackward
                            : ph == 1 ? kTimeLuGather : ph == 4 ? kTimeLuAdd : kTimeLuOther);
-        KRN_TRY(launch_lusgs_phase(ctx->stream, b.e, b.f, ctx->c, cfl, variant, &ctx->lu, ph,
-                                   b.tile_shape < 0 ? 0 : b.tile_shape));
+        const int shape = b.tile_shape < 0 ? 0 : b.tile_shape;
+        KRN_TRY(launch_lusgs_phase(ctx->stream, b.e, b.f, ctx->c, cfl, variant, &ctx->lu, ph, shape,
+                                   ph == 1 && variant == 1 ? sl_geo_cached(ctx, m, shape) : 0));
     }
     return 0;
 }
@@ -766,7 +808,8 @@ int tune_batch(acfd_ctx *ctx)
         const LuBatch &lb = ctx->lub[s];
         if (lb.total <= 0) continue;
         (void)hipEventRecord(e0, ctx->stream);
-        int rc = launch_lusgs_batch(ctx->stream, lb, 1, ctx->c.gamma, ctx->lu.err, ctx->lu_ctr);
+        int rc = launch_lusgs_batch(ctx->stream, lb, 1, ctx->c.gamma, ctx->lu.err, ctx->lu_ctr,
+                                    sl_geo_cached(ctx, -1, s));
         if (!rc) rc = launch_lusgs_batch(ctx->stream, lb, 2, ctx->c.gamma, ctx->lu.err, ctx->lu_ctr);
         (void)hipEventRecord(e1, ctx->stream);
         if (rc) return fail(ACFD_EDEVICE, "batched tile sweep launch failed during tuning");
@@ -822,7 +865,8 @@ int st_implicit_all(acfd_ctx *ctx, const double *cfl_per_block)
     const LuBatch &lb = ctx->lub[ctx->lub_shape];
     for (int ph = 1; ph < kLusgsPhases; ph++) {
         StageTimer ts(ctx, ph == 2 ? kTimeLuForward : ph == 3 ? kTimeLuBackward : ph == 1 ? kTimeLuGather : kTimeLuAdd);
-        KRN_TRY(launch_lusgs_batch(ctx->stream, lb, ph, ctx->c.gamma, ctx->lu.err, ctx->lu_ctr));
+        KRN_TRY(launch_lusgs_batch(ctx->stream, lb, ph, ctx->c.gamma, ctx->lu.err, ctx->lu_ctr,
+                                   ph == 1 ? sl_geo_cached(ctx, -1, ctx->lub_shape) : 0));
     }
     return 0;
 }
@@ -1114,7 +1158,14 @@ int capture_step(acfd_ctx *ctx)
     const bool timing = ctx->timing;
     ctx->timing = false;  // no event records inside the graph
     HIP_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    ctx->capturing = true;
+    ctx->graph_geo_cached = false;
+    ctx->graph_sl_in[0] = ctx->sl_block;
+    ctx->graph_sl_in[1] = ctx->sl_shape;
     const int rc = one_step(ctx);
+    ctx->capturing = false;
+    ctx->graph_sl_out[0] = ctx->sl_block;
+    ctx->graph_sl_out[1] = ctx->sl_shape;
     hipGraph_t g = nullptr;
     const hipError_t ec = hipStreamEndCapture(ctx->stream, &g);
     ctx->timing = timing;
@@ -1136,6 +1187,7 @@ void drop_graph(acfd_ctx *ctx)
 {
     if (ctx->graph_exec) (void)hipGraphExecDestroy(ctx->graph_exec);
     ctx->graph_exec = nullptr;
+    ctx->graph_geo_cached = false;
 }
 
 }  // namespace
@@ -1426,6 +1478,7 @@ int acfd_upload(acfd_ctx *ctx, int m, int field, const double *host)
     Block &b = ctx->blk[m - 1];
     double *dst = field_ptr(b, field);
     if (!dst) return fail(ACFD_EINVAL, "acfd_upload: field not allocated for this configuration");
+    if (field == ACFD_DXIDX || field == ACFD_INV_J) sl_invalidate(ctx);
     int nc = kNcomp[field];
     long tot = b.e.n * nc;
     int rc = ensure_stage(ctx, tot);
@@ -1515,6 +1568,9 @@ int acfd_step(acfd_ctx *ctx, int nsteps)
     // audited sensor steps read results on the host, and steps under the
     // per-stage timers record events between launches: both stay eager
     const bool graph = ctx->cfg.use_graph && !ctx->timing && !(ctx->s_on && ctx->s_audit > 0);
+    if (ctx->graph_exec && ctx->graph_geo_cached &&
+        (ctx->sl_block != ctx->graph_sl_in[0] || ctx->sl_shape != ctx->graph_sl_in[1]))
+        drop_graph(ctx);  // the sweep layout holds other grid operands than at capture
     if (graph && !ctx->graph_exec && nsteps > 0) {
         int rc = one_step(ctx);
         if (rc) return rc;
@@ -1527,6 +1583,8 @@ int acfd_step(acfd_ctx *ctx, int nsteps)
     for (; s < nsteps; s++) {
         if (graph && ctx->graph_exec) {
             HIP_TRY(hipGraphLaunch(ctx->graph_exec, ctx->stream));
+            ctx->sl_block = ctx->graph_sl_out[0];
+            ctx->sl_shape = ctx->graph_sl_out[1];
             continue;
         }
         int rc = one_step(ctx);
@@ -1607,6 +1665,7 @@ int acfd_grid_metrics(acfd_ctx *ctx, double *const *xyz)
         if (rc) return rc;
     }
     const bool sa = ctx->c.turb == 2 && ctx->c.solver == 0;
+    sl_invalidate(ctx);
     for (int m = 0; m < nb; m++) {
         Block &b = ctx->blk[m];
         const long n = b.e.n;

@@ -268,10 +268,15 @@ __global__ void __launch_bounds__(256) k_add5(Ext e, Box b, double *__restrict__
 }
 
 // ------------------------------------------------------------ tile sweeps
-// Sweep-layout fields: q 0-4, dxidx 5-13, inv_j 14, rr 15-17, diag 18,
+// Sweep-layout fields: dxidx 0-8, inv_j 9, q 10-14, rr 15-17, diag 18,
 // rhs 19-23 (overwritten by dq* in the forward and dq in the backward sweep).
+// The grid's operands come first, as whole field pairs: while the sweep
+// layout holds the same block and tiling and the metrics are unchanged, a
+// gather can leave those pairs in place and move only the other 14 fields.
 constexpr int kSlF = 24;
+constexpr int kSlQ = 10;
 constexpr int kSlOut = 19;
+constexpr int kSlGeoPairs = 5;
 // exchange sentinel: a negative signalling NaN with a payload; arithmetic
 // only ever produces quiet NaNs
 constexpr unsigned long long kSentinel = 0xFFF5A5A5A5A5A5A5ull;
@@ -391,18 +396,19 @@ __device__ inline const double *gather_src(int f, long N, const double *q, const
                                            const double *inv_j, const double *rr, const double *diag,
                                            const double *rhs)
 {
-    return f < 5 ? q + f * N : f < 14 ? dxidx + (f - 5) * N : f == 14 ? inv_j
+    return f < 9 ? dxidx + f * N : f == 9 ? inv_j : f < 15 ? q + (f - 10) * N
          : f < 18 ? rr + (f - 15) * N : f == 18 ? diag : rhs + (f - 19) * N;
 }
 
-// Gather the 24 operands of every node into the sweep layout, a field pair
-// at a time (each lane then stores whole 16-byte pairs).  TT is a template
+// Gather the operands of every node into the sweep layout, a field pair at
+// a time (each lane then stores whole 16-byte pairs), from pair fp0 on (0:
+// all 24 fields, kSlGeoPairs: all but the grid's).  TT is a template
 // parameter so that a thread's CT loads are issued back to back.
 template <int TT>
 __device__ inline void sl_gather_tile(const Ext &e, const SlGeom &g, int tile, const double *__restrict__ q,
                                       const double *__restrict__ dxidx, const double *__restrict__ inv_j,
                                       const double *__restrict__ rr, const double *__restrict__ diag,
-                                      const double *__restrict__ rhs, double *__restrict__ sl,
+                                      const double *__restrict__ rhs, double *__restrict__ sl, int fp0,
                                       double *buf /* [2][CT][TT] LDS, swizzled */)
 {
     constexpr int CT = sl_ct_of(TT), lstep = TT / CT;
@@ -422,7 +428,7 @@ __device__ inline void sl_gather_tile(const Ext &e, const SlGeom &g, int tile, c
             nat[m] = (t0 + ii_n < g.nsteps && li >= 0 && li < g.Li && sl_line_ok(g, tile, line))
                          ? (int)sl_nat(e, g, tile, line, li) : -1;
         }
-        for (int fp = 0; fp < kSlF / 2; fp++) {
+        for (int fp = fp0; fp < kSlF / 2; fp++) {
 #pragma unroll
             for (int h = 0; h < 2; h++) {
                 const double *src = gather_src(2 * fp + h, N, q, dxidx, inv_j, rr, diag, rhs);
@@ -456,10 +462,10 @@ __global__ void __launch_bounds__(TT) k_sl_gather(Ext e, SlGeom g, const double 
                                                   const double *__restrict__ inv_j,
                                                   const double *__restrict__ rr,
                                                   const double *__restrict__ diag,
-                                                  const double *__restrict__ rhs, double *__restrict__ sl)
+                                                  const double *__restrict__ rhs, double *__restrict__ sl, int fp0)
 {
     extern __shared__ double buf[];
-    sl_gather_tile<TT>(e, g, (int)blockIdx.x, q, dxidx, inv_j, rr, diag, rhs, sl, buf);
+    sl_gather_tile<TT>(e, g, (int)blockIdx.x, q, dxidx, inv_j, rr, diag, rhs, sl, fp0, buf);
 }
 
 // q += dq from the sweep layout (:568-574), same walk as the gather, one
@@ -518,22 +524,22 @@ __global__ void __launch_bounds__(TT) k_sl_add(Ext e, SlGeom g, const double *__
 
 template <int TT>
 static void launch_layout(hipStream_t s, int which, const Ext &e, const SlGeom &g, const Fields &f,
-                          const LuWs &ws)
+                          const LuWs &ws, int fp0)
 {
     const size_t lds = 2 * (size_t)kSlElems * sizeof(double);
     if (which == 0)
         hipLaunchKernelGGL(k_sl_gather<TT>, dim3((unsigned)g.ntiles), dim3(TT), lds, s, e, g, f.q, f.dxidx,
-                           f.inv_j, f.w_r, f.w_diag, f.rhs, ws.sl);
+                           f.inv_j, f.w_r, f.w_diag, f.rhs, ws.sl, fp0);
     else
         hipLaunchKernelGGL(k_sl_add<TT>, dim3((unsigned)g.ntiles), dim3(TT), lds, s, e, g, ws.sl, f.q);
 }
 
 static void launch_layout_tt(hipStream_t s, int which, const Ext &e, const SlGeom &g, const Fields &f,
-                             const LuWs &ws)
+                             const LuWs &ws, int fp0 = 0)
 {
-    if (g.TT == 256) launch_layout<256>(s, which, e, g, f, ws);
-    else if (g.TT == 128) launch_layout<128>(s, which, e, g, f, ws);
-    else launch_layout<64>(s, which, e, g, f, ws);
+    if (g.TT == 256) launch_layout<256>(s, which, e, g, f, ws, fp0);
+    else if (g.TT == 128) launch_layout<128>(s, which, e, g, f, ws, fp0);
+    else launch_layout<64>(s, which, e, g, f, ws, fp0);
 }
 
 // Fill exchange slots with the sentinel.
@@ -548,11 +554,11 @@ template <int DIR>
 __device__ inline void lu_node(const double *op, const double *ta, const double *tb, const double *tc,
                                double gamma, double *res, double *pa, double *pb, double *pc)
 {
-    if (DIR > 0) lu_fwd_combine(op + kSlOut, ta, tb, tc, op[18], op[14], res);
-    else lu_bwd_combine(op + kSlOut, ta, tb, tc, op[18], op[14], res);
-    jac_mv_r<DIR>(op, op[5], op[6], op[7], op[14], op[15], gamma, res, pa);
-    jac_mv_r<DIR>(op, op[8], op[9], op[10], op[14], op[16], gamma, res, pb);
-    jac_mv_r<DIR>(op, op[11], op[12], op[13], op[14], op[17], gamma, res, pc);
+    if (DIR > 0) lu_fwd_combine(op + kSlOut, ta, tb, tc, op[18], op[9], res);
+    else lu_bwd_combine(op + kSlOut, ta, tb, tc, op[18], op[9], res);
+    jac_mv_r<DIR>(op + kSlQ, op[0], op[1], op[2], op[9], op[15], gamma, res, pa);
+    jac_mv_r<DIR>(op + kSlQ, op[3], op[4], op[5], op[9], op[16], gamma, res, pb);
+    jac_mv_r<DIR>(op + kSlQ, op[6], op[7], op[8], op[9], op[17], gamma, res, pc);
 }
 
 // Persistent tile sweep over the sweep layout.  The forward sweep numbers
@@ -655,7 +661,7 @@ __device__ inline void lu_tiles_body(const SlGeom &g, int tile, double *__restri
             if (fuse_add) {
                 const long n = nat0 + (g.Li - 1 - pos);
 #pragma unroll
-                for (int x = 0; x < 5; x++) q[x * qN + n] = op[x] + res[x];
+                for (int x = 0; x < 5; x++) q[x * qN + n] = op[kSlQ + x] + res[x];
             } else {
 #pragma unroll
                 for (int x = 0; x < 5; x++) sl[g.slot(tile, row(s), kSlOut + x, l)] = res[x];
@@ -722,7 +728,7 @@ struct LuBatchEnt {
 template <int TT>
 __global__ void __launch_bounds__(TT) k_sl_gather_batch(const LuBatchEnt *__restrict__ ents,
                                                         const int *__restrict__ tile_ent, double *arena,
-                                                        double *slb)
+                                                        double *slb, int fp0)
 {
     extern __shared__ double buf[];
     const int flat = (int)blockIdx.x;
@@ -730,7 +736,7 @@ __global__ void __launch_bounds__(TT) k_sl_gather_batch(const LuBatchEnt *__rest
     const Ext e = b.e;
     const SlGeom g = b.g;
     sl_gather_tile<TT>(e, g, flat - b.tile0, arena + b.q, arena + b.dxidx, arena + b.inv_j, arena + b.rr,
-                       arena + b.diag, arena + b.rhs, slb + b.sl, buf);
+                       arena + b.diag, arena + b.rhs, slb + b.sl, fp0, buf);
 }
 
 template <int TT>
@@ -1140,21 +1146,23 @@ static void launch_tiles_batch_shape(hipStream_t s, const LuBatch &b, double gam
 }
 
 template <int TT>
-static void launch_layout_batch(hipStream_t s, int which, const LuBatch &b)
+static void launch_layout_batch(hipStream_t s, int which, const LuBatch &b, int fp0)
 {
     const size_t lds = 2 * (size_t)kSlElems * sizeof(double);
     const LuBatchEnt *ents = (const LuBatchEnt *)b.ents;
     if (which == 0)
         hipLaunchKernelGGL(k_sl_gather_batch<TT>, dim3((unsigned)b.total), dim3(TT), lds, s, ents, b.tile_ent,
-                           b.arena, b.sl);
+                           b.arena, b.sl, fp0);
     else
         hipLaunchKernelGGL(k_sl_add_batch<TT>, dim3((unsigned)b.total), dim3(TT), lds, s, ents, b.tile_ent,
                            b.arena, (const double *)b.sl);
 }
 
 // Phases 1-4 of the LU-SGS update for every block of the batch at once
-// (phase 0, the per-node preparation, stays per block).
-int launch_lusgs_batch(hipStream_t s, const LuBatch &b, int phase, double gamma, int *err, int *ctr)
+// (phase 0, the per-node preparation, stays per block).  geo_cached: the
+// sweep layout already holds this batch's grid operands (phase 1 then
+// gathers the other fields only).
+int launch_lusgs_batch(hipStream_t s, const LuBatch &b, int phase, double gamma, int *err, int *ctr, int geo_cached)
 {
     if (b.total <= 0 || phase < 1 || phase > 4) return 0;
     const int tt = kTileShapes[b.shape][0] * kTileShapes[b.shape][1];
@@ -1162,10 +1170,10 @@ int launch_lusgs_batch(hipStream_t s, const LuBatch &b, int phase, double gamma,
     else if (phase == 3) launch_tiles_batch_shape<-1>(s, b, gamma, err, ctr);
     else if (phase == 4 && b.fuse_add) return 0;
     else {
-        const int which = phase == 1 ? 0 : 1;
-        if (tt == 256) launch_layout_batch<256>(s, which, b);
-        else if (tt == 128) launch_layout_batch<128>(s, which, b);
-        else launch_layout_batch<64>(s, which, b);
+        const int which = phase == 1 ? 0 : 1, fp0 = geo_cached ? kSlGeoPairs : 0;
+        if (tt == 256) launch_layout_batch<256>(s, which, b, fp0);
+        else if (tt == 128) launch_layout_batch<128>(s, which, b, fp0);
+        else launch_layout_batch<64>(s, which, b, fp0);
     }
     ACFD_CHECK_LAUNCH();
     return 0;
@@ -1310,13 +1318,15 @@ static void launch_tiles_shape(hipStream_t s, int shape, const SlGeom &g, const 
 
 // Phases of one LU-SGS update:
 //   0 rhs, spectral radii, diag (hyperplanes also clear dq* and dq),
-//   1 tiles: gather the operands into the sweep layout,
+//   1 tiles: gather the operands into the sweep layout (with geo_cached,
+//     which says that it already holds this block's grid operands in this
+//     tiling, all but those),
 //   2 forward sweep, 3 backward sweep,
 //   4 q += dq (tiles: from the sweep layout, or fused into the backward
 //     sweep, which then stores q + dq itself).
 // variant 0 = hyperplanes, 1 = tiles of the given shape index.
 int launch_lusgs_phase(hipStream_t s, const Ext &e, const Fields &f, const Consts &c, const double *cfl,
-                       int variant, const LuWs *ws, int phase, int shape)
+                       int variant, const LuWs *ws, int phase, int shape, int geo_cached)
 {
     int Li = e.ie1 - e.is1 + 1, Lj = e.je1 - e.js1 + 1, Lk = e.ke1 - e.ks1 + 1;
     const bool tiles = variant == 1 && ws && ws->sl && ws->ex;
@@ -1329,7 +1339,7 @@ int launch_lusgs_phase(hipStream_t s, const Ext &e, const Fields &f, const Const
     } else if (tiles) {
         const SlGeom g = sl_geom(e, shape);
         if (phase == 1)
-            launch_layout_tt(s, 0, e, g, f, *ws);
+            launch_layout_tt(s, 0, e, g, f, *ws, geo_cached ? kSlGeoPairs : 0);
         else if (phase == 2)
             launch_tiles_shape<1>(s, shape, g, *ws, c.gamma, e, nullptr);
         else if (phase == 3)

@@ -90,7 +90,7 @@ constexpr int kLusgsPhases = 5;
 int launch_lusgs(hipStream_t s, const Ext &e, const Fields &f, const Consts &c, const double *cfl,
                  int variant, const LuWs *ws, int shape);
 int launch_lusgs_phase(hipStream_t s, const Ext &e, const Fields &f, const Consts &c, const double *cfl,
-                       int variant, const LuWs *ws, int phase, int shape);
+                       int variant, const LuWs *ws, int phase, int shape, int geo_cached = 0);
 int lusgs_tile_shapes();
 long lusgs_tiles(const Ext &e, int shape);
 // doubles of the sweep layout and of the exchange slots (max over shapes)
@@ -109,7 +109,8 @@ void lusgs_batch_ws(int nblk, const Ext *e, int shape, long *sl_n, long *ex_n);
 int lusgs_batch_make(int nblk, const Ext *e, const Fields *f, double *arena, int shape, double *sl, double *ex,
                      LuBatch *out);
 void lusgs_batch_free(LuBatch *b);
-int launch_lusgs_batch(hipStream_t s, const LuBatch &b, int phase, double gamma, int *err, int *ctr);
+int launch_lusgs_batch(hipStream_t s, const LuBatch &b, int phase, double gamma, int *err, int *ctr,
+                       int geo_cached = 0);
 int launch_fill_sentinel(hipStream_t s, double *p, long n);
 // tile sweeps of the SA model's scalar LU-SGS (cu = a, b, c, diag)
 int launch_sa_sweeps(hipStream_t s, const Ext &e, int shape, const double *cu, const double *rhs, double *dns,

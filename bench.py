@@ -43,7 +43,9 @@ BYTES_PER_NODE = {
     # tile sweeps over the sweep layout: 24 operands in, 5 results out
     "lusgs_forward": 24 * 8 + 40,                   # q, dxidx, inv_j, r_abc, diag, rhs -> dq*
     "lusgs_backward": 24 * 8 + 40,                  # q, dxidx, inv_j, r_abc, diag, dq* -> dq
-    "lusgs_gather": 2 * 24 * 8,                     # 24 operands in (natural) and out (sweep layout)
+    # 14 operands in (natural) and out (sweep layout): the 10 grid operands
+    # stay in the sweep layout from one update of a block to the next
+    "lusgs_gather": 2 * 14 * 8,
     "lusgs_add": 40 + 40 + 40,                      # q, dq -> q
     "lusgs_other": 32 + 72 + 8 + 80 + 40 + 32,      # u,v,w,c, dxidx, dt, rhsi+rhsv -> rhs, r_abc, diag
 }
@@ -65,7 +67,6 @@ HYPERPLANE_BYTES = {  # the hyperplane sweeps read the operands in place
 MEASURED_TRAFFIC_256 = {
     "lusgs_forward": 5.409e9,
     "lusgs_backward": 5.407e9,
-    "lusgs_gather": 8.165e9,
     "lusgs_add": 3.132e9,
     "lusgs_other": 5.578e9,
 }
@@ -248,6 +249,8 @@ def main():
         comp_max = max((r[13] - r[12] + 1) * (r[15] - r[14] + 1) * (r[17] - r[16] + 1) for r in rr)
         if (int(env) != 0) if env is not None else comp_max < FUSED_ADD_MAX_NODES:
             table["lusgs_backward"] = FUSED_BACKWARD_BYTES
+        if os.environ.get("ACFD_SL_GEO_CACHE", "1") == "0":
+            table["lusgs_gather"] = 2 * 24 * 8
     cand = {k: v for k, v in tm.items() if k in table and v[1] > 0}
     dom = max(cand, key=lambda k: cand[k][0])
     avg_ms = cand[dom][0] / cand[dom][1]
