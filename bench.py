@@ -66,9 +66,9 @@ HYPERPLANE_BYTES = {  # the hyperplane sweeps read the operands in place
 # 256^3, FP64, tile sweeps), from two rocprofv3 --pmc runs (FETCH_SIZE,
 # WRITE_SIZE) scaled by a 1 GiB copy; profiles/tgv256_fp64_hbm_traffic.md.
 MEASURED_TRAFFIC_256 = {
-    "lusgs_forward": 5.409e9,
-    "lusgs_backward": 5.407e9,
-    "lusgs_add": 3.132e9,
+    "lusgs_forward": 5.421e9,
+    "lusgs_backward": 5.423e9,
+    "lusgs_add": 3.131e9,
     "lusgs_other": 5.578e9,
 }
 
