@@ -125,6 +125,12 @@ struct acfd_ctx {
     // and leaves those of its end
     bool graph_geo_cached = false;
     int graph_sl_in[2] = {-2, -1}, graph_sl_out[2] = {-2, -1};
+    // pri_v holds getpri of the current q of every block: set by the closing
+    // getpri of a step, cleared by every call that may write q or pri_v.
+    // The next step's residual then starts without its getpri; a captured
+    // step that did so replays only while it holds.
+    bool pri_current = false;
+    bool graph_needs_pri = false;
     int *lu_ctr = nullptr;   // ticket counters of the batched sweeps
     double *d_cfl_blk = nullptr;  // cfl after each block's find_dt
     // batch_blocks: block table and the per-kind grids of the batched launches
@@ -959,9 +965,9 @@ int refresh(acfd_ctx *ctx)
     return rc;
 }
 
-int residual_stage(acfd_ctx *ctx, int m, int store_aux)
+int residual_stage(acfd_ctx *ctx, int m, int store_aux, bool pri_ok = false)
 {
-    int rc = st_getpri(ctx, m);
+    int rc = pri_ok ? 0 : st_getpri(ctx, m);
     if (!rc) rc = st_inviscid(ctx, m);
     if (!rc && ctx->c.solver == 0) rc = st_viscous(ctx, m, store_aux);
     if (!rc) rc = st_find_dt(ctx, m);
@@ -1032,9 +1038,9 @@ int st_find_dt_all(acfd_ctx *ctx)
     return 0;
 }
 
-int residual_stage_all(acfd_ctx *ctx, int store_aux)
+int residual_stage_all(acfd_ctx *ctx, int store_aux, bool pri_ok = false)
 {
-    int rc = st_getpri_all(ctx);
+    int rc = pri_ok ? 0 : st_getpri_all(ctx);
     if (!rc) rc = st_inviscid_all(ctx);
     if (!rc && ctx->c.solver == 0) rc = st_viscous_all(ctx, store_aux);
     if (!rc) rc = st_find_dt_all(ctx);
@@ -1072,8 +1078,15 @@ int one_step(acfd_ctx *ctx)
     const double *cfl = ctx->c.itime == 0 ? nullptr : ctx->d_cfl_blk;
     auto cfl_of = [&](int m) { return cfl ? cfl + m : ctx->d_cfl; };
     const bool stagewise = ctx->cfg.batch_blocks || ctx->cfg.lusgs_kernel == 2;
+    // pri_v of every block is getpri of its q when the last step's closing
+    // getpri is the latest write (a block's q changes only in its own update,
+    // after its residual)
+    const bool pri_ok = ctx->pri_current;
+    ctx->pri_current = false;
+    if (pri_ok && ctx->capturing && (ctx->cfg.iflag_timeadvance == 2 || ctx->cfg.iflag_timeadvance == 4))
+        ctx->graph_needs_pri = true;
     if (ctx->cfg.iflag_timeadvance == 2 && stagewise) {
-        rc = residual_stage_all(ctx, aux);
+        rc = residual_stage_all(ctx, aux, pri_ok);
         if (!rc) rc = st_implicit_all(ctx, cfl ? cfl : nullptr);
         if (!rc && sa) rc = st_sa_all(ctx, cfl_of);
         if (!rc) rc = refresh(ctx);
@@ -1091,7 +1104,7 @@ int one_step(acfd_ctx *ctx)
         if (!rc) rc = st_getpri_all(ctx);
     } else if (ctx->cfg.iflag_timeadvance == 2) {
         for (int m = 0; m < nb && !rc; m++) {
-            rc = residual_stage(ctx, m, aux);
+            rc = residual_stage(ctx, m, aux, pri_ok);
             if (!rc) rc = st_implicit(ctx, m, cfl_of(m));
             if (!rc && sa) rc = st_sa(ctx, m, cfl_of(m));
         }
@@ -1119,7 +1132,7 @@ int one_step(acfd_ctx *ctx)
         for (int stage = 0; stage < 3 && !rc; stage++) {
             if (stage > 0) rc = refresh(ctx);
             if (stagewise) {
-                if (!rc) rc = st_getpri_all(ctx);
+                if (!rc && !(stage == 0 && pri_ok)) rc = st_getpri_all(ctx);
                 if (!rc) rc = st_inviscid_all(ctx);
                 if (!rc && ctx->c.solver == 0) rc = st_viscous_all(ctx, 0);
                 if (!rc && stage == 0) rc = st_find_dt_all(ctx);
@@ -1131,7 +1144,7 @@ int one_step(acfd_ctx *ctx)
                 continue;
             }
             for (int m = 0; m < nb && !rc; m++) {
-                rc = st_getpri(ctx, m);
+                rc = stage == 0 && pri_ok ? 0 : st_getpri(ctx, m);
                 if (!rc) rc = st_inviscid(ctx, m);
                 if (!rc && ctx->c.solver == 0) rc = st_viscous(ctx, m, 0);
                 if (!rc && stage == 0) rc = st_find_dt(ctx, m);
@@ -1148,6 +1161,7 @@ int one_step(acfd_ctx *ctx)
         return fail(ACFD_EINVAL, "iflag_timeadvance must be 2, 3 or 4");
     }
     if (!rc) rc = launch_residual_all(ctx) ? ACFD_EDEVICE : 0;
+    if (!rc) ctx->pri_current = true;
     return rc;
 }
 
@@ -1160,6 +1174,7 @@ int capture_step(acfd_ctx *ctx)
     HIP_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
     ctx->capturing = true;
     ctx->graph_geo_cached = false;
+    ctx->graph_needs_pri = false;
     ctx->graph_sl_in[0] = ctx->sl_block;
     ctx->graph_sl_in[1] = ctx->sl_shape;
     const int rc = one_step(ctx);
@@ -1188,6 +1203,7 @@ void drop_graph(acfd_ctx *ctx)
     if (ctx->graph_exec) (void)hipGraphExecDestroy(ctx->graph_exec);
     ctx->graph_exec = nullptr;
     ctx->graph_geo_cached = false;
+    ctx->graph_needs_pri = false;
 }
 
 }  // namespace
@@ -1471,6 +1487,7 @@ int acfd_upload(acfd_ctx *ctx, int m, int field, const double *host)
 {
     if (!ctx || !ctx->topo || m < 1 || m > (int)ctx->blk.size() || !host || field < 1 || field > kLastField)
         return fail(ACFD_EINVAL, "acfd_upload: bad arguments");
+    ctx->pri_current = false;
     if (field >= ACFD_X) {
         int rc = ensure_geo(ctx);
         if (rc) return rc;
@@ -1509,6 +1526,7 @@ int acfd_download(acfd_ctx *ctx, int m, int field, double *host)
 int acfd_run_stage(acfd_ctx *ctx, int stage)
 {
     if (!ctx || !ctx->topo) return fail(ACFD_ESTATE, "acfd_run_stage: no topology");
+    ctx->pri_current = false;
     const int nb = (int)ctx->blk.size();
     int rc = 0;
     switch (stage) {
@@ -1553,6 +1571,7 @@ int acfd_run_stage(acfd_ctx *ctx, int stage)
 int acfd_prepare(acfd_ctx *ctx)
 {
     if (!ctx || !ctx->topo) return fail(ACFD_ESTATE, "acfd_prepare: no topology");
+    ctx->pri_current = false;
     int rc = refresh(ctx);
     if (rc) return rc;
     HIP_TRY(hipStreamSynchronize(ctx->stream));
@@ -1571,6 +1590,8 @@ int acfd_step(acfd_ctx *ctx, int nsteps)
     if (ctx->graph_exec && ctx->graph_geo_cached &&
         (ctx->sl_block != ctx->graph_sl_in[0] || ctx->sl_shape != ctx->graph_sl_in[1]))
         drop_graph(ctx);  // the sweep layout holds other grid operands than at capture
+    if (ctx->graph_exec && ctx->graph_needs_pri && !ctx->pri_current)
+        drop_graph(ctx);  // q or pri_v was written since the last step
     if (graph && !ctx->graph_exec && nsteps > 0) {
         int rc = one_step(ctx);
         if (rc) return rc;
@@ -1585,6 +1606,7 @@ int acfd_step(acfd_ctx *ctx, int nsteps)
             HIP_TRY(hipGraphLaunch(ctx->graph_exec, ctx->stream));
             ctx->sl_block = ctx->graph_sl_out[0];
             ctx->sl_shape = ctx->graph_sl_out[1];
+            ctx->pri_current = true;
             continue;
         }
         int rc = one_step(ctx);
