@@ -344,19 +344,17 @@ __device__ inline double ndiff(const double *a, long n, long st, int x, int x0, 
     return (a[n + st] - a[n - st]) / 2.0;
 }
 
-// Stage 1 (sub_viscous.f90:107-293) on nodes is0..ie0: velocity and
-// temperature gradients, Sutherland viscosity, stress, heat flux and the
-// contravariant node fluxes fv/gv/hv (components 2..5) into w (12 fields).
+// Stage 1 (sub_viscous.f90:107-293) at node (i,j,k) of is0..ie0: velocity
+// and temperature gradients, Sutherland viscosity, stress, heat flux and
+// the contravariant node fluxes fv/gv/hv (components 2..5) into w[12]
+// (fv 0-3, gv 4-7, hv 8-11); am1 / am3 are the laminar / total viscosity,
+// du the velocity gradient, ta the stress.
 template <typename R>
-__device__ inline void viscous_node_body(const Ext &e, int wx, int wy, int wz, const Consts &c,
-                                         const double *__restrict__ pri, double *__restrict__ amu,
+__device__ inline void viscous_node_eval(const Ext &e, int i, int j, int k, const Consts &c,
+                                         const double *__restrict__ pri, const double *__restrict__ amu,
                                          const double *__restrict__ dxidx, const double *__restrict__ inv_j,
-                                         double *__restrict__ w, double *__restrict__ vor, double *__restrict__ dudx,
-                                         double *__restrict__ tao, int store_aux)
+                                         double *w, R &am1, R &am3, R *du, R *ta)
 {
-    const Box b{e.is0, e.ie0, e.js0, e.je0, e.ks0, e.ke0};
-    int i, j, k;
-    if (!box_node_at(b, wx, wy, wz, i, j, k)) return;
     const long N = e.n;
     long n = e.ix(i, j, k);
     const long sj = e.ni, sk = (long)e.ni * e.nj;
@@ -374,16 +372,13 @@ __device__ inline void viscous_node_body(const Ext &e, int wx, int wy, int wz, c
 #pragma unroll
     for (int x = 0; x < 9; x++) m[x] = (R)dxidx[x * N + n];
     R u0 = (R)pri[N + n], v0 = (R)pri[2 * N + n], w0 = (R)pri[3 * N + n], t0 = (R)pri[5 * N + n];
-    R am1 = (R)((1.0 + 124.0 / c.tinf) / ((double)t0 + 124.0 / c.tinf) * pow((double)t0, 1.5));
+    am1 = (R)((1.0 + 124.0 / c.tinf) / ((double)t0 + 124.0 / c.tinf) * pow((double)t0, 1.5));
     R am2 = (R)amu[N + n];
-    R am3 = am1 + am2;
-    amu[n] = (double)am1;
-    amu[2 * N + n] = (double)am3;
+    am3 = am1 + am2;
     R ki = (R)(c.gamma * c.cv) * (am1 / (R)c.prl + am2 / (R)c.prt);
     R tx = g[0][0] * m[0] + g[0][1] * m[3] + g[0][2] * m[6];
     R ty = g[0][0] * m[1] + g[0][1] * m[4] + g[0][2] * m[7];
     R tz = g[0][0] * m[2] + g[0][1] * m[5] + g[0][2] * m[8];
-    R du[9];
 #pragma unroll
     for (int f = 0; f < 3; f++) {
         du[3 * f + 0] = g[f + 1][0] * m[0] + g[f + 1][1] * m[3] + g[f + 1][2] * m[6];
@@ -391,7 +386,6 @@ __device__ inline void viscous_node_body(const Ext &e, int wx, int wy, int wz, c
         du[3 * f + 2] = g[f + 1][0] * m[2] + g[f + 1][1] * m[5] + g[f + 1][2] * m[8];
     }
     const R c43 = (R)(4.0 / 3.0), c23 = (R)(2.0 / 3.0);
-    R ta[6];
     ta[0] = am3 * (c43 * du[0] - c23 * (du[4] + du[8]));
     ta[1] = am3 * (du[1] + du[3]);
     ta[2] = am3 * (du[6] + du[2]);
@@ -405,11 +399,22 @@ __device__ inline void viscous_node_body(const Ext &e, int wx, int wy, int wz, c
 #pragma unroll
     for (int d = 0; d < 3; d++) {
         const R *r = m + 3 * d;
-        w[(4 * d + 0) * N + n] = (double)((r[0] * ta[0] + r[1] * ta[1] + r[2] * ta[2]) * ij);
-        w[(4 * d + 1) * N + n] = (double)((r[0] * ta[1] + r[1] * ta[3] + r[2] * ta[4]) * ij);
-        w[(4 * d + 2) * N + n] = (double)((r[0] * ta[2] + r[1] * ta[4] + r[2] * ta[5]) * ij);
-        w[(4 * d + 3) * N + n] = (double)((r[0] * bx + r[1] * by + r[2] * bz) * ij);
+        w[4 * d + 0] = (double)((r[0] * ta[0] + r[1] * ta[1] + r[2] * ta[2]) * ij);
+        w[4 * d + 1] = (double)((r[0] * ta[1] + r[1] * ta[3] + r[2] * ta[4]) * ij);
+        w[4 * d + 2] = (double)((r[0] * ta[2] + r[1] * ta[4] + r[2] * ta[5]) * ij);
+        w[4 * d + 3] = (double)((r[0] * bx + r[1] * by + r[2] * bz) * ij);
     }
+}
+
+// the node's own outputs besides the fluxes: viscosities, and with
+// store_aux the vorticity magnitude (1) plus gradient and stress (2)
+template <typename R>
+__device__ inline void viscous_node_store(long N, long n, R am1, R am3, const R *du, const R *ta,
+                                          double *__restrict__ amu, double *__restrict__ vor,
+                                          double *__restrict__ dudx, double *__restrict__ tao, int store_aux)
+{
+    amu[n] = (double)am1;
+    amu[2 * N + n] = (double)am3;
     if (store_aux) {
         vor[n] = (double)sqrt((du[5] - du[7]) * (du[5] - du[7]) + (du[2] - du[6]) * (du[2] - du[6])
                               + (du[1] - du[3]) * (du[1] - du[3]));
@@ -420,6 +425,26 @@ __device__ inline void viscous_node_body(const Ext &e, int wx, int wy, int wz, c
             for (int x = 0; x < 6; x++) tao[x * N + n] = (double)ta[x];
         }
     }
+}
+
+template <typename R>
+__device__ inline void viscous_node_body(const Ext &e, int wx, int wy, int wz, const Consts &c,
+                                         const double *__restrict__ pri, double *__restrict__ amu,
+                                         const double *__restrict__ dxidx, const double *__restrict__ inv_j,
+                                         double *__restrict__ w, double *__restrict__ vor, double *__restrict__ dudx,
+                                         double *__restrict__ tao, int store_aux)
+{
+    const Box b{e.is0, e.ie0, e.js0, e.je0, e.ks0, e.ke0};
+    int i, j, k;
+    if (!box_node_at(b, wx, wy, wz, i, j, k)) return;
+    const long N = e.n;
+    const long n = e.ix(i, j, k);
+    double w12[12];
+    R am1, am3, du[9], ta[6];
+    viscous_node_eval<R>(e, i, j, k, c, pri, amu, dxidx, inv_j, w12, am1, am3, du, ta);
+    viscous_node_store<R>(N, n, am1, am3, du, ta, amu, vor, dudx, tao, store_aux);
+#pragma unroll
+    for (int x = 0; x < 12; x++) w[x * N + n] = w12[x];
 }
 
 template <typename R>
