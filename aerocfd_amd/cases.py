@@ -513,6 +513,74 @@ def flat_plate(seed=3, nblk=(4, 3, 1), lo=(32, 32, 32), hi=(160, 96, 64), length
     return case
 
 
+def channel_bump(seed=5, nblk=(5, 4, 2), lo=(160, 110, 90), hi=(220, 150, 120), length=(4.0, 1.0, 0.5),
+                 bump=0.05, first=2.0e-5, cfg=None):
+    """The combined workload: a channel whose lower wall carries a Gaussian
+    bump of height `bump` (chord 1, y_w = bump exp(-(x - x_c)^2 / 0.06),
+    x_c = length[0] / 2), cut into nblk[0] x nblk[1] x nblk[2] point-matched
+    blocks whose node counts per axis are drawn from a seeded generator in
+    [lo, hi] (40 blocks, ~100 M nodes by default).  x uniform, y tanh-
+    clustered from the wall (first spacing `first`) to the flat top at
+    y = length[1], z uniform.  Faces: inflow (i-), outflow (i+), no-slip
+    wall (j-), symmetry (j+, k).  Mach 0.6, Re = 1e6, SA with DES97, local
+    time step, first-order implicit, uniform start.  Only coordinates are
+    given (over the extended extents, halos from the same mapping): the
+    metrics, edge lengths and wall distance come from acfd_grid_metrics.
+    Fields: q, amu, nut, dt."""
+    from .solver import FlowConfig
+    rng = np.random.default_rng(seed)
+    sizes = [[int(rng.integers(lo[a], hi[a] + 1)) for _ in range(nblk[a])] for a in range(3)]
+    cuts, tot = [], []
+    for a in range(3):
+        c, pos = [], 1
+        for sz in sizes[a][:-1]:
+            pos += sz - 1
+            c.append(pos)
+        cuts.append(tuple(c))
+        tot.append(sum(sizes[a]) - (nblk[a] - 1))
+    NI, NJ, NK = tot
+    if cfg is None:
+        cfg = FlowConfig(ma=0.6, re=1.0e6, iflag_solver=0, iflag_turbulence=2, iflag_des=1, iflag_time=0,
+                         iflag_timeadvance=2, cfl=5.0, dt=1.0e-3, prl=0.72)
+    dims, subs, origins = split_box(NI, NJ, NK, cuts, phys=(-50, -40, -20, -30, -30, -30))
+    case = Case("channel_bump_%dblk_%dx%dx%d" % (len(dims), NI, NJ, NK), cfg, dims, subs)
+    case.coords = []
+    hx, hz = length[0] / (NI - 1), length[2] / (NK - 1)
+    _, ymap, _ = tanh_stretch(NJ, 1.0, first / length[1])
+    xc = 0.5 * length[0]
+    g = GAMMA
+    cv = 1.0 / g / (g - 1.0) / cfg.ma ** 2
+    aoa = math.radians(cfg.aoa_deg)
+    for m in range(len(dims)):
+        r = case.ranges[m]
+        org, _ = origins[m]
+        shp = case.shape(m)
+        gi = org[0] + np.arange(r[0], r[1] + 1) - 1
+        gj = org[1] + np.arange(r[2], r[3] + 1) - 1
+        gk = org[2] + np.arange(r[4], r[5] + 1) - 1
+        K, J, I = np.meshgrid(gk, gj, gi, indexing="ij")
+        x = (I - 1.0) * hx
+        yw = bump * np.exp(-(x - xc) ** 2 / 0.06)
+        y = yw + (length[1] - yw) * ymap((J - 1.0) / (NJ - 1))
+        z = (K - 1.0) * hz
+        case.coords.append((x, y, z))
+        q = np.zeros(shp + (5,))
+        q[..., 0] = 1.0
+        q[..., 1] = math.cos(aoa)
+        q[..., 2] = math.sin(aoa)
+        q[..., 4] = cv + 0.5
+        fl = case.fields[m]
+        fl["q"] = q
+        amu = np.zeros(shp + (3,))
+        amu[..., 0] = 1.0
+        amu[..., 1] = 0.009
+        amu[..., 2] = 1.009
+        fl["amu"] = amu
+        fl["nut"] = np.full(shp, NUT_INF)
+        fl["dt"] = np.zeros(shp)
+    return case
+
+
 def warped_grid(NI=14, NJ=12, NK=10, cuts=((7,), (6,), ()), phys=(-10, -40, -20, -10, -30, -30), amp=0.08,
                 seed=0):
     """A curvilinear multi-block grid: the box [0,1]^3 (NI x NJ x NK nodes)

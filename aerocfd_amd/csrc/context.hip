@@ -132,6 +132,13 @@ struct acfd_ctx {
     bool pri_current = false;
     bool graph_needs_pri = false;
     int *lu_ctr = nullptr;   // ticket counters of the batched sweeps
+    // SA's update runs beside the flow's LU-SGS update on a second stream
+    // (batched sweeps, SA): its own exchange slots and ticket counters
+    bool sa_concurrent = false;
+    hipStream_t stream2 = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    double *sa_ex = nullptr;
+    int *sa_ctr = nullptr;
     double *d_cfl_blk = nullptr;  // cfl after each block's find_dt
     // batch_blocks: block table and the per-kind grids of the batched launches
     BlockRec *d_blocks = nullptr;
@@ -541,11 +548,32 @@ int alloc_lu_ws(acfd_ctx *ctx)
         HIP_TRY(hipMalloc((void **)&ctx->lu_ctr, 2 * sizeof(int)));
         HIP_TRY(hipMemsetAsync(ctx->lu_ctr, 0, 2 * sizeof(int), ctx->stream));
         const bool sa = ctx->c.turb == 2 && ctx->c.solver == 0;
+        const char *cc = std::getenv("ACFD_SA_CONCURRENT");
+        ctx->sa_concurrent = sa && !(cc && std::atoi(cc) == 0);
+        if (ctx->sa_concurrent) {
+            // the SA sweeps need fewer slots than the flow sweeps (one value
+            // per slot against five)
+            ctx->sa_ex = (double *)uc_acquire((size_t)ex_n * sizeof(double));
+            if (!ctx->sa_ex) return fail(ACFD_EDEVICE, "uncached device memory for the tile exchange is unavailable");
+            KRN_TRY(launch_fill_sentinel(ctx->stream, ctx->sa_ex, ex_n));
+            HIP_TRY(hipMalloc((void **)&ctx->sa_ctr, 2 * sizeof(int)));
+            HIP_TRY(hipMemsetAsync(ctx->sa_ctr, 0, 2 * sizeof(int), ctx->stream));
+            HIP_TRY(hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking));
+            HIP_TRY(hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming));
+            HIP_TRY(hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming));
+        }
         for (int sh = 0; sh < lusgs_tile_shapes(); sh++) {
             if (lusgs_batch_make(nb, es.data(), fs.data(), ctx->arena, sh, ctx->lu.sl, ctx->lu.ex, &ctx->lub[sh]))
                 return fail(ACFD_EDEVICE, "batched LU-SGS tables: device allocation failed");
-            if (sa && sa_batch_make(nb, es.data(), fs.data(), ctx->arena, sh, ctx->lu.ex, &ctx->sab[sh]))
+            if (sa && sa_batch_make(nb, es.data(), fs.data(), ctx->arena, sh, ctx->sa_concurrent ? ctx->sa_ex : ctx->lu.ex,
+                                    &ctx->sab[sh]))
                 return fail(ACFD_EDEVICE, "batched SA sweep tables: device allocation failed");
+            if (ctx->sa_concurrent) {
+                // two persistent launches share the device: tiles by ticket
+                // (each waits only on tiles that are already running)
+                ctx->lub[sh].resident[0] = ctx->lub[sh].resident[1] = 0;
+                ctx->sab[sh].resident[0] = ctx->sab[sh].resident[1] = 0;
+            }
         }
     }
     HIP_TRY(hipStreamSynchronize(ctx->stream));
@@ -561,8 +589,10 @@ int tile_error(acfd_ctx *ctx)
     HIP_TRY(hipMemcpy(&err, ctx->lu.err, sizeof(int), hipMemcpyDeviceToHost));
     if (err) {
         (void)launch_fill_sentinel(ctx->stream, ctx->lu.ex, ctx->lu_ex_n);
+        if (ctx->sa_ex) (void)launch_fill_sentinel(ctx->stream, ctx->sa_ex, ctx->lu_ex_n);
         (void)hipMemsetAsync(ctx->lu.err, 0, sizeof(int), ctx->stream);
         if (ctx->lu_ctr) (void)hipMemsetAsync(ctx->lu_ctr, 0, 2 * sizeof(int), ctx->stream);
+        if (ctx->sa_ctr) (void)hipMemsetAsync(ctx->sa_ctr, 0, 2 * sizeof(int), ctx->stream);
         (void)hipStreamSynchronize(ctx->stream);
         return fail(ACFD_EDEVICE, "pipelined LU-SGS sweep timed out waiting for a neighbouring tile");
     }
@@ -884,7 +914,7 @@ int st_sa(acfd_ctx *ctx, int m, const double *cfl)
     if ((ctx->cfg.lusgs_kernel == 1 || ctx->cfg.lusgs_kernel == 2) && ctx->lu.ex) {
         // tile sweeps in the flow sweep's exchange slots (idle, sentinel-filled)
         const int shape = b.tile_shape >= 0 ? b.tile_shape : ctx->lub_shape >= 0 ? ctx->lub_shape : 0;
-        const SaSweep sw{shape, ctx->lu.ex, ctx->lu.err};
+        const SaSweep sw{shape, ctx->sa_ex ? ctx->sa_ex : ctx->lu.ex, ctx->lu.err};
         KRN_TRY(launch_sa(ctx->stream, b.e, b.f, ctx->c, cfl, &sw));
     } else {
         KRN_TRY(launch_sa(ctx->stream, b.e, b.f, ctx->c, cfl));
@@ -909,7 +939,7 @@ int st_sa_all(acfd_ctx *ctx, F cfl_of)
     StageTimer t(ctx, ACFD_STAGE_SA);
     for (int m = 0; m < nb; m++)
         KRN_TRY(launch_sa_pre(ctx->stream, ctx->blk[m].e, ctx->blk[m].f, ctx->c, cfl_of(m)));
-    KRN_TRY(launch_sa_batch(ctx->stream, ctx->sab[shape], ctx->lu.err, ctx->lu_ctr));
+    KRN_TRY(launch_sa_batch(ctx->stream, ctx->sab[shape], ctx->lu.err, ctx->sa_ctr ? ctx->sa_ctr : ctx->lu_ctr));
     for (int m = 0; m < nb; m++) KRN_TRY(launch_sa_post(ctx->stream, ctx->blk[m].e, ctx->blk[m].f));
     return 0;
 }
@@ -1087,8 +1117,22 @@ int one_step(acfd_ctx *ctx)
         ctx->graph_needs_pri = true;
     if (ctx->cfg.iflag_timeadvance == 2 && stagewise) {
         rc = residual_stage_all(ctx, aux, pri_ok);
-        if (!rc) rc = st_implicit_all(ctx, cfl ? cfl : nullptr);
-        if (!rc && sa) rc = st_sa_all(ctx, cfl_of);
+        if (!rc && sa && ctx->sa_concurrent && ctx->lub_shape >= 0) {
+            // sa_model reads nothing implicit1st writes (pri_v, amu, vor, dt
+            // and the cfl are from the residual) and implicit1st reads no
+            // SA field: the SA update runs on the second stream meanwhile
+            HIP_TRY(hipEventRecord(ctx->ev_fork, ctx->stream));
+            HIP_TRY(hipStreamWaitEvent(ctx->stream2, ctx->ev_fork, 0));
+            std::swap(ctx->stream, ctx->stream2);
+            rc = st_sa_all(ctx, cfl_of);
+            std::swap(ctx->stream, ctx->stream2);
+            if (!rc) rc = st_implicit_all(ctx, cfl ? cfl : nullptr);
+            HIP_TRY(hipEventRecord(ctx->ev_join, ctx->stream2));
+            HIP_TRY(hipStreamWaitEvent(ctx->stream, ctx->ev_join, 0));
+        } else {
+            if (!rc) rc = st_implicit_all(ctx, cfl ? cfl : nullptr);
+            if (!rc && sa) rc = st_sa_all(ctx, cfl_of);
+        }
         if (!rc) rc = refresh(ctx);
         if (!rc) rc = st_getpri_all(ctx);
     } else if (ctx->cfg.iflag_timeadvance == 3 && stagewise) {
@@ -1304,6 +1348,11 @@ int acfd_destroy(acfd_ctx *ctx)
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (auto &ev : ctx->ev_pool) (void)hipEventDestroy(ev);
+    if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
+    if (ctx->ev_join) (void)hipEventDestroy(ctx->ev_join);
+    if (ctx->sa_ctr) (void)hipFree(ctx->sa_ctr);
+    if (ctx->sa_ex) uc_release(ctx->sa_ex);
+    if (ctx->stream2) (void)hipStreamDestroy(ctx->stream2);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return 0;

@@ -1007,16 +1007,17 @@ void lusgs_ws_sizes(const Ext &e, long *sl_n, long *ex_n)
 
 // Whether the backward sweep adds dq to q itself (and the add pass is
 // skipped).  Its stores then go to the natural layout, one line per lane:
-// measured cheaper than the separate pass on blocks of up to a few million
-// nodes, dearer on larger ones (256^3: +1.0 ms in the sweep against 0.64 ms
-// saved), so it is chosen by the largest block's computed nodes;
-// ACFD_LU_FUSED_ADD=0/1 forces it.
+// measured cheaper than the separate pass on launches of up to a few
+// million computed nodes, dearer on larger ones (one 256^3 block: +1.0 ms
+// in the sweep against 0.64 ms saved; 40 blocks of 98 M nodes: +8 ms
+// against 2.4 ms), so it is chosen by the computed nodes of the whole
+// launch; ACFD_LU_FUSED_ADD=0/1 forces it.
 constexpr long kFuseAddMaxNodes = 1L << 22;
-static bool fused_add(long max_nodes)
+static bool fused_add(long launch_nodes)
 {
     const char *v = std::getenv("ACFD_LU_FUSED_ADD");
     if (v) return std::atoi(v) != 0;
-    return max_nodes < kFuseAddMaxNodes;
+    return launch_nodes < kFuseAddMaxNodes;
 }
 static long computed_nodes(const Ext &e)
 {
@@ -1084,9 +1085,9 @@ int lusgs_batch_make(int nblk, const Ext *e, const Fields *f, double *arena, int
     out->total = (int)te.size();
     if (ents.empty()) return 0;
     batch_residency(out);
-    long maxn = 0;
-    for (const LuBatchEnt &b : ents) maxn = std::max(maxn, computed_nodes(b.e));
-    out->fuse_add = fused_add(maxn) ? 1 : 0;
+    long total = 0;
+    for (const LuBatchEnt &b : ents) total += computed_nodes(b.e);
+    out->fuse_add = fused_add(total) ? 1 : 0;
     if (hipMalloc(&out->ents, ents.size() * sizeof(LuBatchEnt)) != hipSuccess) return 1;
     if (hipMalloc((void **)&out->tile_ent, te.size() * sizeof(int)) != hipSuccess) return 1;
     if (hipMemcpy(out->ents, ents.data(), ents.size() * sizeof(LuBatchEnt), hipMemcpyHostToDevice) != hipSuccess ||

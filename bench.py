@@ -49,10 +49,18 @@ BYTES_PER_NODE = {
     "lusgs_add": 40 + 40 + 40,                      # q, dq -> q
     "lusgs_other": 32 + 72 + 8 + 80 + 40 + 32,      # u,v,w,c, dxidx, dt, rhsi+rhsv -> rhs, r_abc, diag
 }
-# the tile sweep backward pass adds dq to q itself on blocks under 2^22
+# the tile sweep backward pass adds dq to q itself on launches under 2^22
 # computed nodes: q in and q out instead of dq out (the add pass is skipped)
 FUSED_ADD_MAX_NODES = 1 << 22
 FUSED_BACKWARD_BYTES = 24 * 8 + 40 + 40
+# scheme -> (inviscid_scheme, iflag_timeadvance, workload suffix)
+SCHEMES = {
+    "reference": (0, None, "weno3_roe_lusgs"),
+    "weno5_rk3": (1, 4, "sw_weno5_rk3"),
+    "wcns_rk3": (2, 4, "wcns_rk3"),
+}
+# stages timed as one call but launching several kernels
+LAUNCHES_PER_CALL = {"inviscid": 3, "viscous": 2}
 SINGLE_KERNEL = {"getpri", "lusgs_forward", "lusgs_backward", "lusgs_gather", "lusgs_add", "lusgs_other"}
 HYPERPLANE_BYTES = {  # the hyperplane sweeps read the operands in place
     "lusgs_forward": 40 + 72 + 8 + 24 + 8 + 40 + 40,
@@ -77,6 +85,8 @@ CPU_SAMPLES = {
     "tgv": ("cases.tgv_box(96)", 24, "TGV 96^3 periodic block"),
     "flat_plate": ("cases.flat_plate(nblk=(4, 3, 1), lo=(24, 20, 12), hi=(48, 32, 16))", 150,
                    "flat plate, the same 12-block layout at reduced node counts"),
+    "channel_bump": ("cases.channel_bump(lo=(16, 14, 10), hi=(22, 18, 14), first=2.0e-4)", 120,
+                     "channel with bump, the same 40-block layout at reduced node counts"),
 }
 
 
@@ -93,6 +103,17 @@ from oracle import refwrap as R, oracle as O
 from tests.helpers import ora_config, ora_states
 steps = %d
 case = %s
+if "dxidx" not in case.fields[0]:
+    # coordinates only: metrics, edge lengths and wall distance by the
+    # oracle's restatement of the reference's pre-processing (not timed)
+    exts0 = [O.Ext.from_list(r) for r in case.ranges]
+    wall = O.wall_points(exts0, case.subfaces, case.coords)
+    for m, (e, (x, y, z)) in enumerate(zip(exts0, case.coords)):
+        f = dict(zip(("dxidx", "inv_j", "aalpha", "bbeta", "ggamma"), O.gridtransformation(e, x, y, z)))
+        f["spacing"] = O.spacing(e, x, y, z)[3]
+        f["dist"] = O.walldist(e, x, y, z, wall)
+        for k, v in f.items():
+            case.fields[m][k] = np.nan_to_num(v, nan=0.0)
 kind = "reference" if R.available() else "port"
 if kind == "reference":
     os.chdir(tempfile.mkdtemp())
@@ -149,7 +170,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=256)
-    ap.add_argument("--workload", choices=["tgv", "flat_plate"], default="tgv")
+    ap.add_argument("--workload", choices=["tgv", "flat_plate", "channel_bump"], default="tgv")
     ap.add_argument("--precision", type=int, default=None,
                     help="0 FP64, 1 FP32 flux / FP64 update (default: 0 for tgv, 1 for flat_plate)")
     ap.add_argument("--lusgs", type=int, default=None, help="0 hyperplane launches, 1 pipelined tile sweep per block, 2 tile sweeps "
@@ -160,18 +181,24 @@ def main():
     ap.add_argument("--cuts", type=int, default=1, help="blocks per axis (the workload split into cuts^3 blocks)")
     ap.add_argument("--batch", type=int, default=None,
                     help="1 = one launch per stage over all blocks (default: 0 for tgv, 1 for flat_plate)")
+    ap.add_argument("--scheme", choices=list(SCHEMES), default="reference",
+                    help="reference: WENO3 + Roe with the first-order implicit LU-SGS update (the reference's "
+                         "scheme, the default); weno5_rk3 / wcns_rk3: Steger-Warming + WENO5 or WCNS with "
+                         "explicit RK3 (TGV only)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB (a known-size kernel that "
                          "calibrates HBM byte counters)")
     args = ap.parse_args()
-    fp = args.workload == "flat_plate"
+    fp = args.workload in ("flat_plate", "channel_bump")  # multi-block SA workloads
     if args.precision is None:
         args.precision = 1 if fp else 0
     if args.lusgs is None:
         args.lusgs = 2 if fp else 1
     if args.batch is None:
         args.batch = 1 if fp else 0
+    if args.workload == "channel_bump":
+        args.graph = True
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; every block of the workload is resident on it. "
               "Sharding blocks over several GPUs is described in DESIGN.md but not built, so --gpus %d "
@@ -189,11 +216,18 @@ def main():
         torch.cuda.synchronize()
         del src, dst
 
-    if fp:
+    if args.workload == "channel_bump":
+        case = cases.channel_bump()
+    elif fp:
         case = cases.flat_plate()
     else:
         cut = tuple(round(args.n * c / args.cuts) for c in range(1, args.cuts))
         case = cases.tgv_box(args.n, blocks=(cut, cut, cut))
+    if args.scheme != "reference":
+        if fp:
+            print("bench.py: --scheme applies to the TGV workload", file=sys.stderr)
+            sys.exit(2)
+        case.cfg.inviscid_scheme, case.cfg.iflag_timeadvance = SCHEMES[args.scheme][:2]
     case.cfg.precision = args.precision
     case.cfg.lusgs_kernel = args.lusgs
     case.cfg.use_graph = 1 if args.graph else 0
@@ -201,6 +235,9 @@ def main():
     sol = Solver(case.cfg, case.dims, case.subfaces)
     case.upload(sol)
     del case.fields
+    if args.workload == "channel_bump":
+        sol.grid_metrics(case.coords)  # metrics, edge lengths, wall distance on the device
+        del case.coords
     sol.run_stage("physical_bc")
     sol.run_stage("getpri")
     sol.prepare()
@@ -247,19 +284,28 @@ def main():
         table.update(HYPERPLANE_BYTES)
     else:
         env = os.environ.get("ACFD_LU_FUSED_ADD")
-        comp_max = max((r[13] - r[12] + 1) * (r[15] - r[14] + 1) * (r[17] - r[16] + 1) for r in rr)
-        if (int(env) != 0) if env is not None else comp_max < FUSED_ADD_MAX_NODES:
+        # computed nodes of one backward launch: every block when batched
+        launch = comp_all if args.lusgs == 2 else max(
+            (r[13] - r[12] + 1) * (r[15] - r[14] + 1) * (r[17] - r[16] + 1) for r in rr)
+        if (int(env) != 0) if env is not None else launch < FUSED_ADD_MAX_NODES:
             table["lusgs_backward"] = FUSED_BACKWARD_BYTES
         if os.environ.get("ACFD_SL_GEO_CACHE", "1") == "0":
             table["lusgs_gather"] = 2 * 24 * 8
     # timers that time one kernel per launch (the inviscid and viscous
     # stages launch several kernels each)
-    cand = {k: v for k, v in tm.items() if k in table and k in SINGLE_KERNEL and v[1] > 0}
+    cand = {k: v for k, v in tm.items() if k in table and k in SINGLE_KERNEL and k.startswith("lusgs") and v[1] > 0}
+    if cand:
+        cand = {k: v for k, v in tm.items() if k in table and k in SINGLE_KERNEL and v[1] > 0}
+    else:
+        # explicit RK3: no sweep; the stages themselves (one call = all
+        # kernels of the stage for one block)
+        cand = {k: v for k, v in tm.items() if k in table and v[1] > 0}
     dom = max(cand, key=lambda k: cand[k][0])
     avg_ms = cand[dom][0] / cand[dom][1]
     bytes_launch = table[dom] * (comp_all if dom in batched else comp_all / len(rr))
     achieved = bytes_launch / (avg_ms * 1e-3) / 1e9
-    default_cfg = not fp and args.n == 256 and args.lusgs == 1 and args.precision == 0 and args.cuts == 1
+    default_cfg = (not fp and args.n == 256 and args.lusgs == 1 and args.precision == 0 and args.cuts == 1
+                   and args.scheme == "reference")
     traffic = MEASURED_TRAFFIC_256.get(dom) if default_cfg else None
     out = {
         "metric": "Mcell-updates/s",
@@ -274,23 +320,28 @@ def main():
         "vs_baseline": None,
         "dtype": "f64" if args.precision == 0 else "f32-flux/f64-update",
         "data": "synthetic",
-        "config": {"workload": (case.name + "_turbulent_sa_weno3_roe_lusgs" if fp else
-                                "tgv_%d^3_periodic_laminar_ns_weno3_roe_lusgs" % args.n),
+        "config": {"workload": (case.name + ("_turbulent_sa_des97" if case.cfg.iflag_des == 1 else "_turbulent_sa")
+                                + "_weno3_roe_lusgs" if fp else
+                                "tgv_%d^3_periodic_laminar_ns_%s" % (args.n, SCHEMES[args.scheme][2])),
                    "nodes": nodes, "blocks": len(case.dims), "graph": bool(args.graph),
                    "batch_blocks": bool(args.batch),
                    "mach": case.cfg.ma, "re": case.cfg.re,
-                   "scheme": "WENO3+Roe, central viscous, %sLU-SGS" % ("SA, " if fp else ""),
+                   "scheme": ("WENO3+Roe, central viscous, %sLU-SGS" % ("SA, " if fp else "")
+                              if args.scheme == "reference" else
+                              "%s, central viscous, RK3" % ("Steger-Warming+WENO5" if args.scheme == "weno5_rk3"
+                                                            else "WCNS")),
                    "lusgs_kernel": {0: "hyperplanes", 1: "pipelined-tiles", 2: "batched-pipelined-tiles"}[args.lusgs]},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": PEAK_HBM_GBS,
                      "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4),
                      "traffic": traffic,
-                     "bytes_per_node": table[dom], "avg_launch_ms": round(avg_ms, 4)},
+                     "bytes_per_node": table[dom], "avg_launch_ms": round(avg_ms, 4),
+                     "launches_per_call": LAUNCHES_PER_CALL.get(dom, 1)},
         "stage_ms_per_step": {k: round(v[0] / tsteps, 4) for k, v in tm.items() if v[1] > 0},
         "residual": [float("%.6e" % x) for x in res[:5]],
     }
     if not args.no_cpu:
-        cb = cpu_baseline(args.workload)
-        out["cpu_baseline"] = cb
+        # the reference has only its own scheme; the variants report none
+        out["cpu_baseline"] = cpu_baseline(args.workload) if args.scheme == "reference" else None
     print(json.dumps(out), flush=True)
 
 

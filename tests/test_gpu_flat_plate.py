@@ -80,6 +80,30 @@ def test_flat_plate_batched_identical():
     say("  batched (graph and eager) identical to block-by-block over %d steps" % nsteps)
 
 
+def test_flat_plate_sa_concurrent_identical():
+    """The SA update on its own stream beside the flow update (the default
+    with batched sweeps), eager and replayed from a graph, against the two
+    updates one after the other."""
+    nsteps = 6
+    import os
+    old = os.environ.get("ACFD_SA_CONCURRENT")
+    try:
+        os.environ["ACFD_SA_CONCURRENT"] = "0"
+        ref = gpu_run(cases.flat_plate(**SMALL), nsteps, lusgs=2, batch=1)
+        os.environ["ACFD_SA_CONCURRENT"] = "1"
+        for graph in (0, 1):
+            got = gpu_run(cases.flat_plate(**SMALL), nsteps, lusgs=2, batch=1, graph=graph)
+            for a, b in zip(ref[0] + ref[1], got[0] + got[1]):
+                assert np.array_equal(a, b), (graph, np.abs(a - b).max())
+            assert np.array_equal(ref[2], got[2])
+    finally:
+        if old is None:
+            os.environ.pop("ACFD_SA_CONCURRENT", None)
+        else:
+            os.environ["ACFD_SA_CONCURRENT"] = old
+    say("  concurrent SA update (eager and graph) identical over %d steps" % nsteps)
+
+
 def test_flat_plate_sweep_variants_identical():
     """Hyperplane launches (lusgs_kernel 0) and tile sweeps, per block (1)
     and batched (2), for both the flow and the SA updates: same bits."""
