@@ -38,8 +38,8 @@
 //                  consumer spins on the value itself and writes the sentinel
 //                  back.  No counters, no fences, no waits for store
 //                  acknowledgements on the per-step path; operands and
-//                  exchange values are loaded two steps ahead so that in
-//                  steady state memory latency is hidden.
+//                  exchange values are loaded one step ahead, and a slot
+//                  found late is polled for all its late components at once.
 //
 // The flux Jacobian products use the closed form of A(q) v for the Euler
 // flux in generalised coordinates (one reciprocal of rho instead of the ~25
