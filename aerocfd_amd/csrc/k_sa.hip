@@ -287,10 +287,11 @@ __global__ void __launch_bounds__(256) k_add1(Ext e, Box b, double *__restrict__
 int launch_sa_pre(hipStream_t s, const Ext &e, const Fields &f, const Consts &c, const double *cfl)
 {
     int bs = 256;
-    // sa_rhs = 0 and zeroed temporaries (sub_tur_sa.f90:48, :276-291)
-    int rc = launch_fill(s, f.sa_rhs, e.n, 0.0);
-    if (!rc) rc = launch_fill(s, f.w_sa, 9 * e.n, 0.0);
-    if (rc) return rc;
+    // The reference zeroes sa_rhs and its temporaries every call
+    // (sub_tur_sa.f90:48, :276-291).  Here they live in the arena, which is
+    // zeroed when allocated, and only their computed range is ever written
+    // (w_sa[8] on the ring is reset by k_sa_prep below); sa_rhs outside the
+    // computed range is never read.  So they hold those zeros already.
     hipLaunchKernelGGL(k_sa_length, dim3((unsigned)((e.n + bs - 1) / bs)), dim3(bs), 0, s, e, c, f.dist,
                        f.spacing, f.pri, f.nut, f.amu, f.dudx, f.length);
     Box b{e.is1, e.ie1, e.js1, e.je1, e.ks1, e.ke1};

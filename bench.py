@@ -177,7 +177,8 @@ def main():
                          "of all blocks in one launch (default: 1 for tgv, 2 for flat_plate)")
     ap.add_argument("--graph", action="store_true",
                     help="replay one captured HIP graph per step in the timed region (per-kernel times then "
-                         "come from two eager steps after it)")
+                         "come from two eager steps after it); the default for the multi-block workloads")
+    ap.add_argument("--no-graph", action="store_true", help="eager launches for the multi-block workloads")
     ap.add_argument("--cuts", type=int, default=1, help="blocks per axis (the workload split into cuts^3 blocks)")
     ap.add_argument("--batch", type=int, default=None,
                     help="1 = one launch per stage over all blocks (default: 0 for tgv, 1 for flat_plate)")
@@ -197,7 +198,7 @@ def main():
         args.lusgs = 2 if fp else 1
     if args.batch is None:
         args.batch = 1 if fp else 0
-    if args.workload == "channel_bump":
+    if fp and not args.no_graph:
         args.graph = True
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; every block of the workload is resident on it. "
