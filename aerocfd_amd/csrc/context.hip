@@ -1117,10 +1117,12 @@ int one_step(acfd_ctx *ctx)
         ctx->graph_needs_pri = true;
     if (ctx->cfg.iflag_timeadvance == 2 && stagewise) {
         rc = residual_stage_all(ctx, aux, pri_ok);
-        if (!rc && sa && ctx->sa_concurrent && ctx->lub_shape >= 0) {
+        if (!rc && sa && ctx->sa_concurrent && ctx->lub_shape >= 0 && !ctx->timing) {
             // sa_model reads nothing implicit1st writes (pri_v, amu, vor, dt
             // and the cfl are from the residual) and implicit1st reads no
             // SA field: the SA update runs on the second stream meanwhile
+            // (not under the per-stage timers, whose stage times would then
+            // include each other's)
             HIP_TRY(hipEventRecord(ctx->ev_fork, ctx->stream));
             HIP_TRY(hipStreamWaitEvent(ctx->stream2, ctx->ev_fork, 0));
             std::swap(ctx->stream, ctx->stream2);

@@ -476,8 +476,9 @@ __global__ void __launch_bounds__(TT) k_sl_gather(Ext e, SlGeom g, const double 
     sl_gather_tile<TT>(e, g, (int)blockIdx.x, q, dxidx, inv_j, rr, diag, rhs, sl, fp0, buf);
 }
 
-// q += dq from the sweep layout (:568-574), same walk as the gather, one
-// component at a time.
+// q += dq (:568-574): the backward sweep left q + dq in the result slots of
+// the sweep layout; this pass moves it to q in the natural layout, same walk
+// as the gather, one component at a time.
 template <int TT>
 __device__ inline void sl_add_tile(const Ext &e, const SlGeom &g, int tile, const double *__restrict__ sl,
                                    double *__restrict__ q, double *buf /* [CT][TT] LDS, swizzled */)
@@ -509,13 +510,10 @@ __device__ inline void sl_add_tile(const Ext &e, const SlGeom &g, int tile, cons
                 for (int ii = 0; ii < CT; ii++) buf[ii * TT + (x ^ ii)] = d[ii];
             }
             __syncthreads();
-            double a[CT];
-#pragma unroll
-            for (int m = 0; m < CT; m++) a[m] = nat[m] >= 0 ? q[v * N + nat[m]] : 0.0;
 #pragma unroll
             for (int m = 0; m < CT; m++) {
                 const int line = m * lstep + x / CT;
-                if (nat[m] >= 0) q[v * N + nat[m]] = a[m] + buf[ii_n * TT + (line ^ ii_n)];
+                if (nat[m] >= 0) q[v * N + nat[m]] = buf[ii_n * TT + (line ^ ii_n)];
             }
             __syncthreads();
         }
@@ -671,8 +669,11 @@ __device__ inline void lu_tiles_body(const SlGeom &g, int tile, double *__restri
 #pragma unroll
                 for (int x = 0; x < 5; x++) q[x * qN + n] = op[kSlQ + x] + res[x];
             } else {
+                // the backward sweep leaves q + dq (its q operand plus the
+                // result, the update's one addition), the forward sweep dq*
 #pragma unroll
-                for (int x = 0; x < 5; x++) sl[g.slot(tile, row(s), kSlOut + x, l)] = res[x];
+                for (int x = 0; x < 5; x++)
+                    sl[g.slot(tile, row(s), kSlOut + x, l)] = DIR < 0 ? op[kSlQ + x] + res[x] : res[x];
             }
 #pragma unroll
             for (int x = 0; x < 5; x++) {

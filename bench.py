@@ -46,7 +46,7 @@ BYTES_PER_NODE = {
     # 14 operands in (natural) and out (sweep layout): the 10 grid operands
     # stay in the sweep layout from one update of a block to the next
     "lusgs_gather": 2 * 14 * 8,
-    "lusgs_add": 40 + 40 + 40,                      # q, dq -> q
+    "lusgs_add": 40 + 40,                           # q + dq (sweep layout) -> q
     "lusgs_other": 32 + 72 + 8 + 80 + 40 + 32,      # u,v,w,c, dxidx, dt, rhsi+rhsv -> rhs, r_abc, diag
 }
 # the tile sweep backward pass adds dq to q itself on launches under 2^22
@@ -76,7 +76,6 @@ HYPERPLANE_BYTES = {  # the hyperplane sweeps read the operands in place
 MEASURED_TRAFFIC_256 = {
     "lusgs_forward": 5.421e9,
     "lusgs_backward": 5.423e9,
-    "lusgs_add": 3.131e9,
     "lusgs_other": 5.578e9,
 }
 
