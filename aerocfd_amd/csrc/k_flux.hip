@@ -428,7 +428,7 @@ __device__ inline void viscous_node_store(long N, long n, R am1, R am3, const R 
 }
 
 template <typename R>
-__device__ inline void viscous_node_body(const Ext &e, int wx, int wy, int wz, const Consts &c,
+__device__ inline void viscous_node_body(const Ext &e, long wg, const Consts &c,
                                          const double *__restrict__ pri, double *__restrict__ amu,
                                          const double *__restrict__ dxidx, const double *__restrict__ inv_j,
                                          double *__restrict__ w, double *__restrict__ vor, double *__restrict__ dudx,
@@ -436,9 +436,9 @@ __device__ inline void viscous_node_body(const Ext &e, int wx, int wy, int wz, c
 {
     const Box b{e.is0, e.ie0, e.js0, e.je0, e.ks0, e.ke0};
     int i, j, k;
-    if (!box_node_at(b, wx, wy, wz, i, j, k)) return;
+    long n;
+    if (!lin_node_at(e, b, wg, i, j, k, n)) return;
     const long N = e.n;
-    const long n = e.ix(i, j, k);
     double w12[12];
     R am1, am3, du[9], ta[6];
     viscous_node_eval<R>(e, i, j, k, c, pri, amu, dxidx, inv_j, w12, am1, am3, du, ta);
@@ -454,19 +454,19 @@ __global__ void __launch_bounds__(256) k_viscous_node(Ext e, Consts c, const dou
                                                       double *__restrict__ vor, double *__restrict__ dudx,
                                                       double *__restrict__ tao, int store_aux)
 {
-    viscous_node_body<R>(e, (int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z, c, pri, amu, dxidx, inv_j, w, vor,
-                         dudx, tao, store_aux);
+    viscous_node_body<R>(e, xcd_slab(blockIdx.x, gridDim.x), c, pri, amu, dxidx, inv_j, w, vor, dudx, tao,
+                         store_aux);
 }
 
 // Stage 2 (sub_viscous.f90:296-365): rhsv = (d/dxi fv + d/deta gv + d/dzeta hv)/re
-__device__ inline void viscous_diff_body(const Ext &e, int bx, int by, int bz, const double *__restrict__ w,
+__device__ inline void viscous_diff_body(const Ext &e, long wg, const double *__restrict__ w,
                                          double *__restrict__ rhsv, double re)
 {
     const Box b{e.is0, e.ie0, e.js0, e.je0, e.ks0, e.ke0};
     int i, j, k;
-    if (!box_node_at(b, bx, by, bz, i, j, k)) return;
+    long n;
+    if (!lin_node_at(e, b, wg, i, j, k, n)) return;
     const long N = e.n;
-    long n = e.ix(i, j, k);
     const long sj = e.ni, sk = (long)e.ni * e.nj;
     rhsv[n] = (0.0 + 0.0 + 0.0) / re;
 #pragma unroll
@@ -481,7 +481,7 @@ __device__ inline void viscous_diff_body(const Ext &e, int bx, int by, int bz, c
 __global__ void __launch_bounds__(256) k_viscous_diff(Ext e, const double *__restrict__ w,
                                                       double *__restrict__ rhsv, double re)
 {
-    viscous_diff_body(e, (int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z, w, rhsv, re);
+    viscous_diff_body(e, xcd_slab(blockIdx.x, gridDim.x), w, rhsv, re);
 }
 
 template <typename R>
@@ -493,7 +493,7 @@ __global__ void __launch_bounds__(256) k_viscous_node_batch(const BlockRec *__re
     int bx, by, bz;
     const BlockRec &b = blks[batch_wg(map, kg, bx, by, bz)];
     const Ext e = b.e;
-    viscous_node_body<R>(e, bx, by, bz, c, arena + b.o.pri, arena + b.o.amu, arena + b.o.dxidx, arena + b.o.inv_j,
+    viscous_node_body<R>(e, bx, c, arena + b.o.pri, arena + b.o.amu, arena + b.o.dxidx, arena + b.o.inv_j,
                          arena + b.o.w_flux, arena + b.o.vor, arena + b.o.dudx, arena + b.o.tao, store_aux);
 }
 
@@ -504,11 +504,12 @@ __global__ void __launch_bounds__(256) k_viscous_diff_batch(const BlockRec *__re
     int bx, by, bz;
     const BlockRec &b = blks[batch_wg(map, kg, bx, by, bz)];
     const Ext e = b.e;
-    viscous_diff_body(e, bx, by, bz, arena + b.o.w_flux, arena + b.o.rhsv, re);
+    viscous_diff_body(e, bx, arena + b.o.w_flux, arena + b.o.rhsv, re);
 }
 
-// per-block grid of both viscous kernels: the physical range, 64 x 4 threads
-dim3 viscous_grid(const Ext &e) { return box_grid(Box{e.is0, e.ie0, e.js0, e.je0, e.ks0, e.ke0}); }
+// per-block grid of both viscous kernels: linear tiles of 256 nodes over the
+// block's storage (the physical range is masked inside), 64 x 4 threads
+dim3 viscous_grid(const Ext &e) { return lin_grid(e); }
 
 int launch_viscous(hipStream_t s, const Ext &e, const Fields &f, const Consts &c, int precision,
                    int store_aux)

@@ -153,7 +153,7 @@ __device__ inline void lu_backward(const LuArgs &a, long n, const double *vi, co
 // rhs = rhsi + rhsv on the computed range; spectral radii ra, rb, rc and
 // diag on the ring is1-1..ie1+1 (:72-156); with clear_dq, dq* and dq cleared
 // on the ring so that out-of-range neighbours read 0 (:105-106).
-__device__ inline void lu_prep_body(const Ext &e, int bx, int by, int bz, const double *__restrict__ pri,
+__device__ inline void lu_prep_body(const Ext &e, long wg, const double *__restrict__ pri,
                                     const double *__restrict__ dxidx, const double *__restrict__ dt,
                                     const double *__restrict__ rhsi, const double *__restrict__ rhsv,
                                     double *__restrict__ rhs, double *__restrict__ rr, double *__restrict__ diag,
@@ -162,9 +162,9 @@ __device__ inline void lu_prep_body(const Ext &e, int bx, int by, int bz, const 
 {
     const Box ring{e.is1 - 1, e.ie1 + 1, e.js1 - 1, e.je1 + 1, e.ks1 - 1, e.ke1 + 1};
     int i, j, k;
-    if (!box_node_at(ring, bx, by, bz, i, j, k)) return;
+    long n;
+    if (!lin_node_at(e, ring, wg, i, j, k, n)) return;
     const long N = e.n;
-    long n = e.ix(i, j, k);
     double phi = *cfl >= 1.0e3 ? 0.0 : 1.0;  // :98-102
     double u = pri[N + n], v = pri[2 * N + n], w = pri[3 * N + n], c = pri[6 * N + n];
     double r3[3];
@@ -196,8 +196,8 @@ __global__ void __launch_bounds__(256) k_lu_prep(Ext e, const double *__restrict
                                                  double *__restrict__ diag, double *__restrict__ dqs,
                                                  double *__restrict__ dq, const double *cfl, int clear_dq)
 {
-    lu_prep_body(e, (int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z, pri, dxidx, dt, rhsi, rhsv, rhs, rr, diag,
-                 dqs, dq, cfl, clear_dq);
+    lu_prep_body(e, xcd_slab(blockIdx.x, gridDim.x), pri, dxidx, dt, rhsi, rhsv, rhs, rr, diag, dqs, dq, cfl,
+                 clear_dq);
 }
 
 // every block; block m reads cfl[m * cfl_stride]
@@ -210,12 +210,14 @@ __global__ void __launch_bounds__(256) k_lu_prep_batch(const BlockRec *__restric
     const int m = batch_wg(map, kg, bx, by, bz);
     const BlockRec &b = blks[m];
     const Ext e = b.e;
-    lu_prep_body(e, bx, by, bz, arena + b.o.pri, arena + b.o.dxidx, arena + b.o.dt, arena + b.o.rhsi,
+    lu_prep_body(e, bx, arena + b.o.pri, arena + b.o.dxidx, arena + b.o.dt, arena + b.o.rhsi,
                  arena + b.o.rhsv, arena + b.o.rhs, arena + b.o.w_r, arena + b.o.w_diag, arena + b.o.w_dqs,
                  arena + b.o.w_dq, cfl + (long)m * cfl_stride, clear_dq);
 }
 
-dim3 lu_prep_grid(const Ext &e) { return box_grid(Box{e.is1 - 1, e.ie1 + 1, e.js1 - 1, e.je1 + 1, e.ks1 - 1, e.ke1 + 1}); }
+// linear tiles of 256 nodes over the block's storage (the ring of the
+// computed range is masked inside), 64 x 4 threads
+dim3 lu_prep_grid(const Ext &e) { return lin_grid(e); }
 
 int launch_lu_prep_batch(hipStream_t s, const BlockRec *blks, const BatchGrid &bg, double *arena, const double *cfl,
                          int cfl_stride, int clear_dq)
