@@ -3,6 +3,7 @@
 // the scalar LU-SGS update of nut.
 //
 // Reference: sub_tur_sa.f90:7-825, constants module_tursa.f90:4-11.
+#include "batch.hpp"
 #include "common.hpp"
 #include "kernels.hpp"
 
@@ -90,9 +91,9 @@ __global__ void __launch_bounds__(256) k_sa_a(Ext e, Box b, const double *__rest
                                               double *__restrict__ sa_rhs, double *__restrict__ w)
 {
     int i, j, k;
-    if (!box_node(b, i, j, k)) return;
+    long n;
+    if (!lin_node_at(e, b, xcd_slab(blockIdx.x, gridDim.x), i, j, k, n)) return;
     const long N = e.n;
-    long n = e.ix(i, j, k);
     const long st[3] = {1, (long)e.ni, (long)e.ni * e.nj};
     double u = pri[N + n], v = pri[2 * N + n], ww = pri[3 * N + n];
     double r[3];
@@ -129,9 +130,9 @@ __global__ void __launch_bounds__(256) k_sa_b(Ext e, Box b, Consts c, const doub
                                               double *__restrict__ sa_rhs, double *__restrict__ w)
 {
     int i, j, k;
-    if (!box_node(b, i, j, k)) return;
+    long n;
+    if (!lin_node_at(e, b, xcd_slab(blockIdx.x, gridDim.x), i, j, k, n)) return;
     const long N = e.n;
-    long n = e.ix(i, j, k);
     const long st[3] = {1, (long)e.ni, (long)e.ni * e.nj};
     const double re = c.re;
     double tempa = sa_d<true>(e, w + 3 * N, n, st[0], i, e.is0, e.ie0, e.is1, e.ie1);
@@ -220,9 +221,9 @@ __global__ void __launch_bounds__(256) k_sa_prep(Ext e, Box ring, Consts c, cons
                                                  double *__restrict__ dnut)
 {
     int i, j, k;
-    if (!box_node(ring, i, j, k)) return;
+    long n;
+    if (!lin_node_at(e, ring, xcd_slab(blockIdx.x, gridDim.x), i, j, k, n)) return;
     const long N = e.n;
-    long n = e.ix(i, j, k);
     double phi = (*cfl >= 1.0e3 && c.itime == 0) ? 0.0 : 1.0;
     double u = pri[N + n], v = pri[2 * N + n], ww = pri[3 * N + n];
     double m[9];
@@ -278,8 +279,8 @@ __global__ void __launch_bounds__(256) k_add1(Ext e, Box b, double *__restrict__
                                               const double *__restrict__ d)
 {
     int i, j, k;
-    if (!box_node(b, i, j, k)) return;
-    long n = e.ix(i, j, k);
+    long n;
+    if (!lin_node_at(e, b, xcd_slab(blockIdx.x, gridDim.x), i, j, k, n)) return;
     x[n] = x[n] + d[n];
 }
 
@@ -294,15 +295,17 @@ int launch_sa_pre(hipStream_t s, const Ext &e, const Fields &f, const Consts &c,
     // computed range is never read.  So they hold those zeros already.
     hipLaunchKernelGGL(k_sa_length, dim3((unsigned)((e.n + bs - 1) / bs)), dim3(bs), 0, s, e, c, f.dist,
                        f.spacing, f.pri, f.nut, f.amu, f.dudx, f.length);
+    // linear tiles of 256 nodes over the block's storage, the box masked
+    // inside (whole cache lines per wavefront)
     Box b{e.is1, e.ie1, e.js1, e.je1, e.ks1, e.ke1};
-    dim3 g = box_grid(b), t(64, 4, 1);
-    if (g.x == 0) return 0;
+    if (box_grid(b).x == 0) return 0;
+    const dim3 g = lin_grid(e), t(64, 4, 1);
     hipLaunchKernelGGL(k_sa_a, g, t, 0, s, e, b, f.pri, f.dxidx, f.nut, f.amu, f.aal, f.bbe, f.gga, f.sa_rhs,
                        f.w_sa);
     hipLaunchKernelGGL(k_sa_b, g, t, 0, s, e, b, c, f.pri, f.dxidx, f.inv_j, f.nut, f.amu, f.vor, f.length,
                        f.sa_rhs, f.w_sa);
     Box ring{e.is1 - 1, e.ie1 + 1, e.js1 - 1, e.je1 + 1, e.ks1 - 1, e.ke1 + 1};
-    hipLaunchKernelGGL(k_sa_prep, box_grid(ring), t, 0, s, e, ring, c, f.pri, f.dxidx, f.nut, f.dt, cfl, f.w_sa,
+    hipLaunchKernelGGL(k_sa_prep, g, t, 0, s, e, ring, c, f.pri, f.dxidx, f.nut, f.dt, cfl, f.w_sa,
                        f.w_dqs, f.w_dq);
     ACFD_CHECK_LAUNCH();
     return 0;
@@ -334,9 +337,8 @@ int launch_sa_planes(hipStream_t s, const Ext &e, const Fields &f)
 int launch_sa_post(hipStream_t s, const Ext &e, const Fields &f)
 {
     Box b{e.is1, e.ie1, e.js1, e.je1, e.ks1, e.ke1};
-    dim3 g = box_grid(b), t(64, 4, 1);
-    if (g.x == 0) return 0;
-    hipLaunchKernelGGL(k_add1, g, t, 0, s, e, b, f.nut, f.w_dq);
+    if (box_grid(b).x == 0) return 0;
+    hipLaunchKernelGGL(k_add1, lin_grid(e), dim3(64, 4, 1), 0, s, e, b, f.nut, f.w_dq);
     ACFD_CHECK_LAUNCH();
     return 0;
 }
