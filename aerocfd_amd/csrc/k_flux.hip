@@ -196,13 +196,24 @@ __device__ inline void inviscid_jk_body(const Ext &e, int bx, int by, int bz, co
                                         const double *__restrict__ dxidx, const double *__restrict__ inv_j,
                                         double *__restrict__ rhsi, double gamma)
 {
-    const int i = e.is1 + (int)(bx * blockDim.x + threadIdx.x);
-    const int o = (int)(by * blockDim.y + threadIdx.y);  // the other transverse index
     const int a0 = e.lo1(AXIS) + bz * kInvRun;
     const int a1 = min(a0 + kInvRun - 1, e.hi1(AXIS));
-    const int j = AXIS == 1 ? a0 : e.js1 + o;
-    const int k = AXIS == 2 ? a0 : e.ks1 + o;
-    if (i > e.ie1 || (AXIS == 1 ? k > e.ke1 : j > e.je1)) return;
+    int i, j, k;
+    if (AXIS == 1) {
+        i = e.is1 + (int)(bx * blockDim.x + threadIdx.x);
+        j = a0;
+        k = e.ks1 + (int)(by * blockDim.y + threadIdx.y);
+        if (i > e.ie1 || k > e.ke1) return;
+    } else {
+        // a k-plane is contiguous in storage: the 256 lanes of workgroup bx
+        // take 256 consecutive positions of it (whole cache lines whatever
+        // the row length); positions outside the computed range idle
+        const int p = bx * 256 + (int)(threadIdx.x + blockDim.x * threadIdx.y);
+        i = e.is + p % e.ni;
+        j = e.js + p / e.ni;
+        k = a0;
+        if (i < e.is1 || i > e.ie1 || j < e.js1 || j > e.je1) return;
+    }
     const long N = e.n, st = e.stride(AXIS);
     const int alo = e.lo(AXIS), ahi = e.hi(AXIS);
     const int c0 = 3 * AXIS;
@@ -279,7 +290,8 @@ dim3 inviscid_grid(const Ext &e, int axis)
     if (Li <= 0 || Lj <= 0 || Lk <= 0) return dim3(0, 0, 0);
     if (axis == 0) return dim3((Li + kInvI - 1) / kInvI, (Lj + 3) / 4, Lk);
     if (axis == 1) return dim3((Li + 63) / 64, (Lk + 3) / 4, (Lj + kInvRun - 1) / kInvRun);
-    return dim3((Li + 63) / 64, (Lj + 3) / 4, (Lk + kInvRun - 1) / kInvRun);
+    // k: linear positions over the whole (extended) i-j plane
+    return dim3((unsigned)(((long)e.ni * e.nj + 255) / 256), 1, (Lk + kInvRun - 1) / kInvRun);
 }
 
 template <typename R>
