@@ -78,7 +78,7 @@ MEASURED_TRAFFIC_256 = {
     "lusgs_backward": 5.423e9,
     "lusgs_gather": 4.763e9,   # a later gather (the grid operands in place)
     "lusgs_add": 2.085e9,
-    "lusgs_other": 5.578e9,
+    "lusgs_other": 4.537e9,
 }
 
 
