@@ -287,6 +287,7 @@ constexpr unsigned long long kSentinel = 0xFFF5A5A5A5A5A5A5ull;
 struct SlGeom {
     int TJ, TK, TT, nbj, nbk, ntiles, nsteps, Li, Lj, Lk;
     int lgTJ;  // TJ is a power of two
+    int refill;  // consumers leave exchange slots as they are; the region is refilled after each sweep
     long exj_tile, exk_tile;  // doubles per tile of the j / k exchange regions
     // fields are stored in pairs, so a lane's operands of one step are 12
     // 16-byte loads; double index of field f of lane l at row t
@@ -299,9 +300,23 @@ struct SlGeom {
 const int kTileShapes[][2] = {{16, 16}, {16, 8}, {8, 8}, {16, 4}};
 const int kNumTileShapes = 4;
 
+// Exchange slots are reset to the sentinel by one fill kernel after each
+// flow sweep rather than by their consumers, one store each: the per-step
+// stores were on the sweep's critical path.
+static int ex_refill()
+{
+    static int r = -1;
+    if (r < 0) {
+        const char *v = getenv("ACFD_EX_REFILL");
+        r = v && atoi(v) == 0 ? 0 : 1;
+    }
+    return r;
+}
+
 static SlGeom sl_geom(const Ext &e, int shape)
 {
     SlGeom g;
+    g.refill = ex_refill();
     g.TJ = kTileShapes[shape][0];
     g.TK = kTileShapes[shape][1];
     g.TT = g.TJ * g.TK;
@@ -339,8 +354,10 @@ __device__ inline void xstore(double *p, double v)
 // so that a slot that was late costs one round trip more, not one per
 // component.  The wait is bounded: running out sets *err and every later
 // wait of the launch gives up at once, so the sweep always ends (the host
-// then reports the failure).  The slot is reset for the next sweep.
-__device__ inline void ex_take(double *slot, const unsigned long long *pre, double *v, int *err, bool &failed)
+// then reports the failure).  With `reset` the slot is set back to the
+// sentinel for the next sweep (otherwise the host refills the region).
+__device__ inline void ex_take(double *slot, const unsigned long long *pre, double *v, int *err, bool &failed,
+                               bool reset)
 {
     unsigned long long cur[5];
 #pragma unroll
@@ -365,7 +382,7 @@ __device__ inline void ex_take(double *slot, const unsigned long long *pre, doub
             if (failed) b = 0;
         }
         v[x] = __longlong_as_double((long long)b);
-        xstore_bits(slot + x, kSentinel);
+        if (reset) xstore_bits(slot + x, kSentinel);
     }
 }
 
@@ -651,7 +668,7 @@ __device__ inline void lu_tiles_body(const SlGeom &g, int tile, double *__restri
 #pragma unroll
                 for (int x = 0; x < 5; x++) tb[x] = 0.0;
             } else if (j_in) {
-                ex_take(jin + (long)(pos + sk) * TK * 5, xj, tb, err, failed);
+                ex_take(jin + (long)(pos + sk) * TK * 5, xj, tb, err, failed, !g.refill);
             } else {
 #pragma unroll
                 for (int x = 0; x < 5; x++) tb[x] = pj[p ^ 1][x][lpj];
@@ -660,7 +677,7 @@ __device__ inline void lu_tiles_body(const SlGeom &g, int tile, double *__restri
 #pragma unroll
                 for (int x = 0; x < 5; x++) tc[x] = 0.0;
             } else if (k_in) {
-                ex_take(kin + (long)(pos + sj) * TJ * 5, xk, tc, err, failed);
+                ex_take(kin + (long)(pos + sj) * TJ * 5, xk, tc, err, failed, !g.refill);
             } else {
 #pragma unroll
                 for (int x = 0; x < 5; x++) tc[x] = pk[p ^ 1][x][lpk];
@@ -1086,6 +1103,8 @@ int lusgs_batch_make(int nblk, const Ext *e, const Fields *f, double *arena, int
     out->ex = ex;
     out->nent = (int)ents.size();
     out->total = (int)te.size();
+    out->ex_n = xo;
+    out->refill = ex_refill();
     if (ents.empty()) return 0;
     batch_residency(out);
     long total = 0;
@@ -1177,9 +1196,13 @@ int launch_lusgs_batch(hipStream_t s, const LuBatch &b, int phase, double gamma,
 {
     if (b.total <= 0 || phase < 1 || phase > 4) return 0;
     const int tt = kTileShapes[b.shape][0] * kTileShapes[b.shape][1];
-    if (phase == 2) launch_tiles_batch_shape<1>(s, b, gamma, err, ctr);
-    else if (phase == 3) launch_tiles_batch_shape<-1>(s, b, gamma, err, ctr);
-    else if (phase == 4 && b.fuse_add) return 0;
+    if (phase == 2 || phase == 3) {
+        if (phase == 2) launch_tiles_batch_shape<1>(s, b, gamma, err, ctr);
+        else launch_tiles_batch_shape<-1>(s, b, gamma, err, ctr);
+        if (b.refill)
+            hipLaunchKernelGGL(k_fill_sentinel, dim3((unsigned)((b.ex_n + 255) / 256)), dim3(256), 0, s, b.ex,
+                               b.ex_n);
+    } else if (phase == 4 && b.fuse_add) return 0;
     else {
         const int which = phase == 1 ? 0 : 1, fp0 = geo_cached ? kSlGeoPairs : 0;
         if (tt == 256) launch_layout_batch<256>(s, which, b, fp0);
@@ -1351,11 +1374,14 @@ int launch_lusgs_phase(hipStream_t s, const Ext &e, const Fields &f, const Const
         const SlGeom g = sl_geom(e, shape);
         if (phase == 1)
             launch_layout_tt(s, 0, e, g, f, *ws, geo_cached ? kSlGeoPairs : 0);
-        else if (phase == 2)
-            launch_tiles_shape<1>(s, shape, g, *ws, c.gamma, e, nullptr);
-        else if (phase == 3)
-            launch_tiles_shape<-1>(s, shape, g, *ws, c.gamma, e, fused_add(computed_nodes(e)) ? f.q : nullptr);
-        else if (!fused_add(computed_nodes(e)))
+        else if (phase == 2 || phase == 3) {
+            if (phase == 2) launch_tiles_shape<1>(s, shape, g, *ws, c.gamma, e, nullptr);
+            else launch_tiles_shape<-1>(s, shape, g, *ws, c.gamma, e, fused_add(computed_nodes(e)) ? f.q : nullptr);
+            if (g.refill) {
+                const long n = (long)g.ntiles * (g.exj_tile + g.exk_tile);
+                hipLaunchKernelGGL(k_fill_sentinel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, ws->ex, n);
+            }
+        } else if (!fused_add(computed_nodes(e)))
             launch_layout_tt(s, 1, e, g, f, *ws);
     } else if (phase == 4) {
         Box b{e.is1, e.ie1, e.js1, e.je1, e.ks1, e.ke1};

@@ -103,6 +103,8 @@ struct LuBatch {
     int nent = 0, total = 0, shape = 0;
     int resident[2] = {0, 0};  // forward / backward sweep: every workgroup fits on the device at once
     int fuse_add = 0;          // the backward sweep stores q + dq (no add pass)
+    int refill = 0;            // the exchange region is refilled with the sentinel after each sweep
+    long ex_n = 0;             // doubles of the exchange region the batch uses
 };
 void lusgs_batch_ws(int nblk, const Ext *e, int shape, long *sl_n, long *ex_n);
 // f[m]'s fields must lie in the allocation starting at arena
