@@ -278,6 +278,7 @@ __global__ void __launch_bounds__(256) k_add5(Ext e, Box b, double *__restrict__
 constexpr int kSlF = 24;
 constexpr int kSlQ = 10;
 constexpr int kSlOut = 19;
+static_assert(kSlOut % 2 == 1 && kSlOut + 5 == kSlF, "the add pass reads the results as pairs 9-11");
 constexpr int kSlGeoPairs = 5;
 // exchange sentinel: a negative signalling NaN with a payload; arithmetic
 // only ever produces quiet NaNs
@@ -497,10 +498,10 @@ __global__ void __launch_bounds__(TT) k_sl_gather(Ext e, SlGeom g, const double 
 
 // q += dq (:568-574): the backward sweep left q + dq in the result slots of
 // the sweep layout; this pass moves it to q in the natural layout, same walk
-// as the gather, one component at a time.
+// as the gather, one field pair at a time.
 template <int TT>
 __device__ inline void sl_add_tile(const Ext &e, const SlGeom &g, int tile, const double *__restrict__ sl,
-                                   double *__restrict__ q, double *buf /* [CT][TT] LDS, swizzled */)
+                                   double *__restrict__ q, double *buf /* [2][CT][TT] LDS, swizzled */)
 {
     constexpr int CT = sl_ct_of(TT), lstep = TT / CT;
     const int x = threadIdx.x;
@@ -517,22 +518,34 @@ __device__ inline void sl_add_tile(const Ext &e, const SlGeom &g, int tile, cons
             nat[m] = (t0 + ii_n < g.nsteps && li >= 0 && li < g.Li && sl_line_ok(g, tile, line))
                          ? (int)sl_nat(e, g, tile, line, li) : -1;
         }
-        for (int v = 0; v < 5; v++) {
+        // the results are fields 19-23: the upper half of pair 9 (whose
+        // lower half, diag, is not written) and pairs 10 and 11, read as
+        // whole 16-byte pairs
+        for (int fp = kSlOut / 2; fp < kSlF / 2; fp++) {
+            const int v0 = 2 * fp - kSlOut;  // component of the pair's lower field (-1: diag)
             if (me_ok) {
-                double d[CT];
+                const double2 *sl2 = reinterpret_cast<const double2 *>(sl);
+                double2 d[CT];
 #pragma unroll
                 for (int ii = 0; ii < CT; ii++) {
                     const int t = t0 + ii, li = t - tj - tk;
-                    d[ii] = (t < g.nsteps && li >= 0 && li < g.Li) ? sl[g.slot(tile, t, kSlOut + v, x)] : 0.0;
+                    d[ii] = (t < g.nsteps && li >= 0 && li < g.Li) ? sl2[g.slot(tile, t, 2 * fp, x) >> 1]
+                                                                   : make_double2(0.0, 0.0);
                 }
 #pragma unroll
-                for (int ii = 0; ii < CT; ii++) buf[ii * TT + (x ^ ii)] = d[ii];
+                for (int ii = 0; ii < CT; ii++) {
+                    buf[ii * TT + (x ^ ii)] = d[ii].x;
+                    buf[(CT + ii) * TT + (x ^ ii)] = d[ii].y;
+                }
             }
             __syncthreads();
 #pragma unroll
             for (int m = 0; m < CT; m++) {
                 const int line = m * lstep + x / CT;
-                if (nat[m] >= 0) q[v * N + nat[m]] = buf[ii_n * TT + (line ^ ii_n)];
+                if (nat[m] >= 0) {
+                    if (v0 >= 0) q[v0 * N + nat[m]] = buf[ii_n * TT + (line ^ ii_n)];
+                    q[(v0 + 1) * N + nat[m]] = buf[(CT + ii_n) * TT + (line ^ ii_n)];
+                }
             }
             __syncthreads();
         }

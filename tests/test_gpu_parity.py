@@ -6,6 +6,7 @@ after one evaluation and <= 1e-10 after many steps (the accuracy bar of the
 path); the GPU kernels use fused multiply-adds and a different division
 sequence, so agreement is to rounding, not bitwise.
 """
+import os
 import sys
 
 import numpy as np
@@ -356,23 +357,36 @@ def test_steps(kind, nsteps):
         sol.close()
 
 
-def test_lusgs_variants_identical():
+@pytest.mark.parametrize("variant,fused", [(1, None), (1, "0"), (2, "0")])
+def test_lusgs_variants_identical(variant, fused):
     """The pipelined tile sweep computes each node from the same operands as
-    the hyperplane sweep: results must agree to the last bit."""
-    say("test_lusgs_variants_identical tgv 40^3")
+    the hyperplane sweep: results must agree to the last bit.  fused "0"
+    moves q + dq to q in a separate pass (the form of blocks over 2^22
+    computed nodes) instead of in the backward sweep; variant 2 is the
+    batched sweep (one block cut in eight)."""
+    say("test_lusgs_variants_identical tgv 40^3 variant", variant, "fused", fused)
     outs = []
-    for v in (0, 1):
-        case = cases.tgv_box(40)
+    for v in (0, variant):
+        case = cases.tgv_box(40) if variant < 2 else cases.tgv_box(40, blocks=((20,), (16,), (24,)))
         case.cfg.lusgs_kernel = v
-        sol = Solver(case.cfg, case.dims, case.subfaces)
+        old = os.environ.get("ACFD_LU_FUSED_ADD")
+        if fused is not None and v > 0:
+            os.environ["ACFD_LU_FUSED_ADD"] = fused
         try:
-            case.upload(sol)
-            sol.run_stage("getpri")
-            sol.prepare()
-            sol.step(3)
-            outs.append(sol.download(1, "q"))
+            sol = Solver(case.cfg, case.dims, case.subfaces)
+            try:
+                case.upload(sol)
+                sol.run_stage("getpri")
+                sol.prepare()
+                sol.step(3)
+                outs.append(np.concatenate([sol.download(m + 1, "q").ravel() for m in range(len(case.dims))]))
+            finally:
+                sol.close()
         finally:
-            sol.close()
+            if old is None:
+                os.environ.pop("ACFD_LU_FUSED_ADD", None)
+            else:
+                os.environ["ACFD_LU_FUSED_ADD"] = old
     d = np.abs(outs[0] - outs[1])
     say("  max |q_hyperplane - q_tiles| = %.3e" % d.max())
     assert np.array_equal(outs[0], outs[1])
