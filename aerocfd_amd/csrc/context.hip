@@ -143,6 +143,9 @@ struct acfd_ctx {
     // batch_blocks: block table and the per-kind grids of the batched launches
     BlockRec *d_blocks = nullptr;
     BatchGrid bg_getpri, bg_inv[4], bg_visc, bg_dt, bg_prep;
+    // physical boundary conditions: round r holds the r-th subface of every block
+    std::vector<BcTask *> d_bc_round;
+    std::vector<BatchGrid> bg_bc;
     bool batch_ready = false;
     double *d_stage = nullptr;     // upload/download staging
     long stage_n = 0;
@@ -477,6 +480,30 @@ int build_batch(acfd_ctx *ctx)
     if (!rc) rc = make_grid(ctx->bg_visc, g_visc);
     if (!rc) rc = make_grid(ctx->bg_dt, g_pri);
     if (!rc) rc = make_grid(ctx->bg_prep, g_prep);
+    // physical boundary rounds; a block's subfaces keep the order of ctx->bc
+    for (size_t r = 0; r < ctx->bg_bc.size(); r++) {
+        if (ctx->d_bc_round[r]) (void)hipFree(ctx->d_bc_round[r]);
+        free_grid(ctx->bg_bc[r]);
+    }
+    ctx->d_bc_round.clear();
+    ctx->bg_bc.clear();
+    std::vector<std::vector<BcTask>> per(ctx->blk.size());
+    for (const BcTask &t : ctx->bc) per[t.block].push_back(t);
+    size_t rounds = 0;
+    for (auto &v : per) rounds = std::max(rounds, v.size());
+    for (size_t r = 0; r < rounds && !rc; r++) {
+        std::vector<BcTask> tasks;
+        std::vector<dim3> g;
+        for (auto &v : per)
+            if (r < v.size()) {
+                tasks.push_back(v[r]);
+                g.push_back(dim3((unsigned)((std::max(bc_nodes(v[r]), 0L) + 255) / 256)));
+            }
+        ctx->d_bc_round.push_back(nullptr);
+        ctx->bg_bc.push_back(BatchGrid());
+        rc = dev_upload(&ctx->d_bc_round.back(), tasks);
+        if (!rc) rc = make_grid(ctx->bg_bc.back(), g);
+    }
     ctx->batch_ready = rc == 0;
     return rc;
 }
@@ -962,6 +989,14 @@ int st_update_q0(acfd_ctx *ctx, int m)
 int st_physical_bc(acfd_ctx *ctx)
 {
     StageTimer t(ctx, ACFD_STAGE_PHYSICAL_BC);
+    if (batched(ctx)) {
+        for (size_t r = 0; r < ctx->bg_bc.size(); r++)
+            KRN_TRY(launch_bc_batch(ctx->stream, ctx->d_bc_round[r], ctx->bg_bc[r], ctx->d_blocks, ctx->arena,
+                                    ctx->c));
+        if (ctx->c.turb == 2 && ctx->c.solver == 0)
+            KRN_TRY(launch_sa_clamp_batch(ctx->stream, ctx->d_blocks, ctx->bg_getpri, ctx->arena));
+        return 0;
+    }
     for (const BcTask &task : ctx->bc) {
         Block &b = ctx->blk[task.block];
         KRN_TRY(launch_bc(ctx->stream, b.e, b.f, ctx->c, task));
@@ -1339,6 +1374,10 @@ int acfd_destroy(acfd_ctx *ctx)
     for (BatchGrid *g : {&ctx->bg_getpri, &ctx->bg_inv[0], &ctx->bg_inv[1], &ctx->bg_inv[2], &ctx->bg_inv[3],
                          &ctx->bg_visc, &ctx->bg_dt, &ctx->bg_prep})
         free_grid(*g);
+    for (size_t r = 0; r < ctx->bg_bc.size(); r++) {
+        if (ctx->d_bc_round[r]) (void)hipFree(ctx->d_bc_round[r]);
+        free_grid(ctx->bg_bc[r]);
+    }
     void *sp[] = {ctx->s_w, ctx->s_m, ctx->s_v, ctx->s_grad, ctx->s_loss, ctx->s_part, ctx->s_lpart, ctx->s_net,
                   ctx->s_X, ctx->s_Y, ctx->s_stats, ctx->s_dpart};
     for (void *p : sp)

@@ -18,8 +18,8 @@ constexpr double kNutInf = 1.341946;  // module_tursa.f90:10
 // condition file differ only in the axis and the side, so they share one
 // kernel: boundary index lo (odd face) or hi (even face) along the axis,
 // interior neighbour one step inwards.
-__global__ void k_bc(Ext e, Consts c, BcTask t, double *__restrict__ q, double *__restrict__ nut,
-                     const double *__restrict__ dxidx)
+__device__ inline void bc_node(const Ext &e, const Consts &c, const BcTask &t, long x, double *__restrict__ q,
+                               double *__restrict__ nut, const double *__restrict__ dxidx)
 {
     const int axis = (t.face - 1) / 2;
     const bool minus = (t.face & 1) == 1;
@@ -27,7 +27,6 @@ __global__ void k_bc(Ext e, Consts c, BcTask t, double *__restrict__ q, double *
     int fix = minus ? lo[axis] : hi[axis];
     lo[axis] = hi[axis] = fix;
     int n0 = hi[0] - lo[0] + 1, n1 = hi[1] - lo[1] + 1, n2 = hi[2] - lo[2] + 1;
-    long x = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (x >= (long)n0 * n1 * n2) return;
     int i = lo[0] + (int)(x % n0);
     int j = lo[1] + (int)((x / n0) % n1);
@@ -121,12 +120,38 @@ __global__ void k_bc(Ext e, Consts c, BcTask t, double *__restrict__ q, double *
     }
 }
 
-int launch_bc(hipStream_t s, const Ext &e, const Fields &f, const Consts &c, const BcTask &t)
+__global__ void __launch_bounds__(256) k_bc(Ext e, Consts c, BcTask t, double *__restrict__ q,
+                                            double *__restrict__ nut, const double *__restrict__ dxidx)
+{
+    bc_node(e, c, t, (long)blockIdx.x * blockDim.x + threadIdx.x, q, nut, dxidx);
+}
+
+// One subface of each of several blocks in one launch (tasks[m] for the
+// workgroups of entry m of the grid).  Blocks touch only their own arrays,
+// so the subfaces of different blocks are independent; the subfaces of one
+// block go to successive launches, in the reference's order.
+__global__ void __launch_bounds__(256) k_bc_batch(const BcTask *__restrict__ tasks, const int *__restrict__ map,
+                                                  const KGrid *__restrict__ kg, const BlockRec *__restrict__ blks,
+                                                  double *arena, Consts c)
+{
+    int bx, by, bz;
+    const BcTask t = tasks[batch_wg(map, kg, bx, by, bz)];
+    const BlockRec &b = blks[t.block];
+    bc_node(b.e, c, t, (long)bx * blockDim.x + threadIdx.x, arena + b.o.q, b.o.nut >= 0 ? arena + b.o.nut : nullptr,
+            arena + b.o.dxidx);
+}
+
+long bc_nodes(const BcTask &t)
 {
     const int axis = (t.face - 1) / 2;
     int lo[3] = {t.is, t.js, t.ks}, hi[3] = {t.ie, t.je, t.ke};
     lo[axis] = hi[axis] = 0;
-    long cnt = (long)(hi[0] - lo[0] + 1) * (hi[1] - lo[1] + 1) * (hi[2] - lo[2] + 1);
+    return (long)(hi[0] - lo[0] + 1) * (hi[1] - lo[1] + 1) * (hi[2] - lo[2] + 1);
+}
+
+int launch_bc(hipStream_t s, const Ext &e, const Fields &f, const Consts &c, const BcTask &t)
+{
+    const long cnt = bc_nodes(t);
     if (cnt <= 0) return 0;
     int bs = 256;
     hipLaunchKernelGGL(k_bc, dim3((unsigned)((cnt + bs - 1) / bs)), dim3(bs), 0, s, e, c, t, f.q, f.nut, f.dxidx);
@@ -134,13 +159,21 @@ int launch_bc(hipStream_t s, const Ext &e, const Fields &f, const Consts &c, con
     return 0;
 }
 
+int launch_bc_batch(hipStream_t s, const BcTask *tasks, const BatchGrid &bg, const BlockRec *blks, double *arena,
+                    const Consts &c)
+{
+    if (bg.total <= 0) return 0;
+    hipLaunchKernelGGL(k_bc_batch, dim3((unsigned)bg.total), dim3(256), 0, s, tasks, bg.map, bg.kg, blks, arena, c);
+    ACFD_CHECK_LAUNCH();
+    return 0;
+}
+
 // SA clamp of sub_bc.f90:65-87 over the whole extended block: nut keeps
 // min(nut, 1e6) (the max(nut,1e-4) of :80 is overwritten by :81) and
 // mu_t = nut * fv1 * rho with rho from pri_v.
-__global__ void k_sa_clamp(Ext e, double *__restrict__ nut, double *__restrict__ amu,
-                           const double *__restrict__ pri)
+__device__ inline void sa_clamp_node(const Ext &e, long n, double *__restrict__ nut, double *__restrict__ amu,
+                                     const double *__restrict__ pri)
 {
-    long n = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= e.n) return;
     const long N = e.n;
     const double cv1 = 7.1;
@@ -153,6 +186,30 @@ __global__ void k_sa_clamp(Ext e, double *__restrict__ nut, double *__restrict__
     double nn = fmin(nu, 1.0e6);
     nut[n] = nn;
     amu[N + n] = nn * fv1 * d;
+}
+
+__global__ void __launch_bounds__(256) k_sa_clamp(Ext e, double *__restrict__ nut, double *__restrict__ amu,
+                                                  const double *__restrict__ pri)
+{
+    sa_clamp_node(e, (long)blockIdx.x * blockDim.x + threadIdx.x, nut, amu, pri);
+}
+
+// every block (1-D grid over each block's extended nodes)
+__global__ void __launch_bounds__(256) k_sa_clamp_batch(const BlockRec *__restrict__ blks,
+                                                        const int *__restrict__ map, const KGrid *__restrict__ kg,
+                                                        double *arena)
+{
+    int bx, by, bz;
+    const BlockRec &b = blks[batch_wg(map, kg, bx, by, bz)];
+    sa_clamp_node(b.e, (long)bx * blockDim.x + threadIdx.x, arena + b.o.nut, arena + b.o.amu, arena + b.o.pri);
+}
+
+int launch_sa_clamp_batch(hipStream_t s, const BlockRec *blks, const BatchGrid &bg, double *arena)
+{
+    if (bg.total <= 0) return 0;
+    hipLaunchKernelGGL(k_sa_clamp_batch, dim3((unsigned)bg.total), dim3(256), 0, s, blks, bg.map, bg.kg, arena);
+    ACFD_CHECK_LAUNCH();
+    return 0;
 }
 
 int launch_sa_clamp(hipStream_t s, const Ext &e, const Fields &f)

@@ -143,6 +143,11 @@ struct BcTask {       // one physical subface
 };
 int launch_bc(hipStream_t s, const Ext &e, const Fields &f, const Consts &c, const BcTask &t);
 int launch_sa_clamp(hipStream_t s, const Ext &e, const Fields &f);
+long bc_nodes(const BcTask &t);  // face nodes of a subface
+// tasks[m]: one subface of block tasks[m].block, m over the grid's entries
+int launch_bc_batch(hipStream_t s, const BcTask *tasks, const BatchGrid &bg, const BlockRec *blks, double *arena,
+                    const Consts &c);
+int launch_sa_clamp_batch(hipStream_t s, const BlockRec *blks, const BatchGrid &bg, double *arena);
 
 // halo copy lists (device arrays of node references)
 struct CopyList {
