@@ -79,7 +79,7 @@ MEASURED_TRAFFIC_256 = {
     "lusgs_forward": 5.073e9,
     "lusgs_backward": 5.075e9,
     "lusgs_gather": 4.763e9,   # a later gather (the grid operands in place)
-    "lusgs_add": 2.085e9,
+    "lusgs_add": 1.572e9,
     "lusgs_other": 4.537e9,
 }
 
