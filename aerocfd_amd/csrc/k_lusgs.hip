@@ -1039,18 +1039,20 @@ void lusgs_ws_sizes(const Ext &e, long *sl_n, long *ex_n)
 }
 
 // Whether the backward sweep adds dq to q itself (and the add pass is
-// skipped).  Its stores then go to the natural layout, one line per lane:
-// measured cheaper than the separate pass on launches of up to a few
-// million computed nodes, dearer on larger ones (one 256^3 block: +1.0 ms
-// in the sweep against 0.64 ms saved; 40 blocks of 98 M nodes: +8 ms
-// against 2.4 ms), so it is chosen by the computed nodes of the whole
-// launch; ACFD_LU_FUSED_ADD=0/1 forces it.
+// skipped).  Its stores then go to the natural layout, one line per lane.
+// On one block's launch that is free up to a few million computed nodes
+// (128^3: 0.644 against 0.642 ms per sweep, 0.1 ms of add pass saved) and
+// dearer on larger ones (256^3: +1.0 ms in the sweep against 0.64 ms
+// saved).  In the batched sweeps it costs more than the pass it saves
+// (flat plate, 12 blocks: +0.25 ms against 0.10 ms; 128^3 in 64 blocks:
+// +0.16 ms against 0.12 ms), so batched launches never fuse.
+// ACFD_LU_FUSED_ADD=0/1 forces either form.
 constexpr long kFuseAddMaxNodes = 1L << 22;
-static bool fused_add(long launch_nodes)
+static bool fused_add(long launch_nodes, bool batched = false)
 {
     const char *v = std::getenv("ACFD_LU_FUSED_ADD");
     if (v) return std::atoi(v) != 0;
-    return launch_nodes < kFuseAddMaxNodes;
+    return !batched && launch_nodes < kFuseAddMaxNodes;
 }
 static long computed_nodes(const Ext &e)
 {
@@ -1122,7 +1124,7 @@ int lusgs_batch_make(int nblk, const Ext *e, const Fields *f, double *arena, int
     batch_residency(out);
     long total = 0;
     for (const LuBatchEnt &b : ents) total += computed_nodes(b.e);
-    out->fuse_add = fused_add(total) ? 1 : 0;
+    out->fuse_add = fused_add(total, true) ? 1 : 0;
     if (hipMalloc(&out->ents, ents.size() * sizeof(LuBatchEnt)) != hipSuccess) return 1;
     if (hipMalloc((void **)&out->tile_ent, te.size() * sizeof(int)) != hipSuccess) return 1;
     if (hipMemcpy(out->ents, ents.data(), ents.size() * sizeof(LuBatchEnt), hipMemcpyHostToDevice) != hipSuccess ||

@@ -49,8 +49,9 @@ BYTES_PER_NODE = {
     "lusgs_add": 40 + 40,                           # q + dq (sweep layout) -> q
     "lusgs_other": 32 + 72 + 8 + 80 + 40 + 32,      # u,v,w,c, dxidx, dt, rhsi+rhsv -> rhs, r_abc, diag
 }
-# the tile sweep backward pass adds dq to q itself on launches under 2^22
-# computed nodes: q in and q out instead of dq out (the add pass is skipped)
+# the tile sweep backward pass adds dq to q itself on one block's launch
+# under 2^22 computed nodes (never in the batched sweeps): q in and q out
+# instead of dq out (the add pass is skipped)
 FUSED_ADD_MAX_NODES = 1 << 22
 FUSED_BACKWARD_BYTES = 24 * 8 + 40 + 40
 # scheme -> (inviscid_scheme, iflag_timeadvance, workload suffix)
@@ -288,10 +289,10 @@ def main():
         table.update(HYPERPLANE_BYTES)
     else:
         env = os.environ.get("ACFD_LU_FUSED_ADD")
-        # computed nodes of one backward launch: every block when batched
-        launch = comp_all if args.lusgs == 2 else max(
-            (r[13] - r[12] + 1) * (r[15] - r[14] + 1) * (r[17] - r[16] + 1) for r in rr)
-        if (int(env) != 0) if env is not None else launch < FUSED_ADD_MAX_NODES:
+        # batched sweeps never fuse; a block's own launch does under 2^22
+        # computed nodes
+        launch = max((r[13] - r[12] + 1) * (r[15] - r[14] + 1) * (r[17] - r[16] + 1) for r in rr)
+        if (int(env) != 0) if env is not None else (args.lusgs != 2 and launch < FUSED_ADD_MAX_NODES):
             table["lusgs_backward"] = FUSED_BACKWARD_BYTES
         if os.environ.get("ACFD_SL_GEO_CACHE", "1") == "0":
             table["lusgs_gather"] = 2 * 24 * 8

@@ -357,13 +357,14 @@ def test_steps(kind, nsteps):
         sol.close()
 
 
-@pytest.mark.parametrize("variant,fused", [(1, None), (1, "0"), (2, "0")])
+@pytest.mark.parametrize("variant,fused", [(1, None), (1, "0"), (2, None), (2, "1")])
 def test_lusgs_variants_identical(variant, fused):
     """The pipelined tile sweep computes each node from the same operands as
     the hyperplane sweep: results must agree to the last bit.  fused "0"
     moves q + dq to q in a separate pass (the form of blocks over 2^22
-    computed nodes) instead of in the backward sweep; variant 2 is the
-    batched sweep (one block cut in eight)."""
+    computed nodes and of batched sweeps) instead of in the backward sweep,
+    "1" in the backward sweep; variant 2 is the batched sweep (one block cut
+    in eight)."""
     say("test_lusgs_variants_identical tgv 40^3 variant", variant, "fused", fused)
     outs = []
     for v in (0, variant):
