@@ -39,11 +39,24 @@ struct BatchGrid {
     int total = 0;        // workgroups
 };
 
-// Block index and per-block workgroup coordinates of this workgroup.
+// XCD-aware order of a 1-D launch: workgroup w goes to XCD w % 8, each XCD
+// with its own L2; this hands XCD x the x-th eighth of the workgroup range,
+// so each XCD walks a contiguous slab of the block and a node's j and k
+// neighbours are mostly read through the same L2.  A bijection for any
+// grid size.
+__device__ inline long xcd_slab(long w, long total)
+{
+    const long q = total / 8, r = total % 8, x = w % 8;
+    return x * q + (x < r ? x : r) + w / 8;
+}
+
+// Block index and per-block workgroup coordinates of this workgroup (flat
+// workgroup numbers in XCD slabs, so that each XCD walks contiguous parts
+// of the blocks).
 __device__ inline int batch_wg(const int *__restrict__ map, const KGrid *__restrict__ kg, int &bx, int &by,
                                int &bz)
 {
-    const int flat = (int)blockIdx.x;
+    const int flat = (int)xcd_slab(blockIdx.x, gridDim.x);
     const int m = map[flat];
     const KGrid g = kg[m];
     int local = flat - g.wg0;
@@ -74,17 +87,6 @@ __device__ inline bool lin_node_at(const Ext &e, const Box &b, long w, int &i, i
 }
 
 inline dim3 lin_grid(const Ext &e) { return dim3((unsigned)((e.n + 255) / 256)); }
-
-// XCD-aware order of a 1-D launch: workgroup w goes to XCD w % 8, each XCD
-// with its own L2; this hands XCD x the x-th eighth of the workgroup range,
-// so each XCD walks a contiguous slab of the block and a node's j and k
-// neighbours are mostly read through the same L2.  A bijection for any
-// grid size.
-__device__ inline long xcd_slab(long w, long total)
-{
-    const long q = total / 8, r = total % 8, x = w % 8;
-    return x * q + (x < r ? x : r) + w / 8;
-}
 
 __device__ inline bool box_node_at(const Box &b, int bx, int by, int bz, int &i, int &j, int &k)
 {
