@@ -47,7 +47,7 @@ int fail(int code, const std::string &msg)
 const int kNcomp[28] = {0, 5, 7, 5, 5, 5, 9, 6, 1, 3, 1, 9, 1, 3, 3, 3, 1, 1, 1, 1, 10, 1, 1, 1, 1, 1, 1, 1};
 constexpr int kLastField = 27;
 
-constexpr int kTimeSlots = 20;  // acfd_timing entries
+constexpr int kTimeSlots = 32;  // acfd_timing entries
 constexpr int kSensorParams = 1505, kSensorFeatures = 12;  // the sensor network's size
 
 struct Block {
@@ -758,6 +758,10 @@ int st_find_dt(acfd_ctx *ctx, int m)
 // sub-stage timing ids of the LU-SGS phases
 constexpr int kTimeLuForward = 12, kTimeLuBackward = 13, kTimeLuGather = 14, kTimeLuOther = 15,
               kTimeLuAdd = 16;
+// single-kernel timers of the batched stages (recorded only with timing on,
+// when each stage's kernels are launched one by one)
+constexpr int kTimeSaForward = 17, kTimeSaBackward = 18, kTimeSaPre = 19, kTimeSaAdd = 20, kTimeInvZero = 21,
+              kTimeInvI = 22, kTimeInvJ = 23, kTimeInvK = 24, kTimeViscNode = 25, kTimeViscDiff = 26;
 
 // Autotuner of the tile sweep: on a block's first update, time the gather
 // and forward sweep once per tile shape (they recompute dq* in the sweep
@@ -971,9 +975,23 @@ int st_sa_all(acfd_ctx *ctx, F cfl_of)
         return 0;
     }
     StageTimer t(ctx, ACFD_STAGE_SA);
-    for (int m = 0; m < nb; m++)
-        KRN_TRY(launch_sa_pre(ctx->stream, ctx->blk[m].e, ctx->blk[m].f, ctx->c, cfl_of(m)));
-    KRN_TRY(launch_sa_batch(ctx->stream, ctx->sab[shape], ctx->lu.err, ctx->sa_ctr ? ctx->sa_ctr : ctx->lu_ctr));
+    {
+        StageTimer ts(ctx, kTimeSaPre);
+        for (int m = 0; m < nb; m++)
+            KRN_TRY(launch_sa_pre(ctx->stream, ctx->blk[m].e, ctx->blk[m].f, ctx->c, cfl_of(m)));
+    }
+    int *ctr = ctx->sa_ctr ? ctx->sa_ctr : ctx->lu_ctr;
+    if (ctx->timing) {
+        {
+            StageTimer ts(ctx, kTimeSaForward);
+            KRN_TRY(launch_sa_batch(ctx->stream, ctx->sab[shape], ctx->lu.err, ctr, 1));
+        }
+        StageTimer ts(ctx, kTimeSaBackward);
+        KRN_TRY(launch_sa_batch(ctx->stream, ctx->sab[shape], ctx->lu.err, ctr, -1));
+    } else {
+        KRN_TRY(launch_sa_batch(ctx->stream, ctx->sab[shape], ctx->lu.err, ctr, 0));
+    }
+    StageTimer ts(ctx, kTimeSaAdd);
     for (int m = 0; m < nb; m++) KRN_TRY(launch_sa_post(ctx->stream, ctx->blk[m].e, ctx->blk[m].f));
     return 0;
 }
@@ -1075,6 +1093,14 @@ int st_inviscid_all(acfd_ctx *ctx)
         return 0;
     }
     StageTimer t(ctx, ACFD_STAGE_INVISCID);
+    if (ctx->timing) {
+        for (int part = 0; part < 4; part++) {
+            StageTimer ts(ctx, kTimeInvZero + part);
+            KRN_TRY(launch_inviscid_batch(ctx->stream, ctx->d_blocks, ctx->bg_inv, ctx->arena, ctx->c,
+                                          ctx->cfg.precision, part));
+        }
+        return 0;
+    }
     KRN_TRY(launch_inviscid_batch(ctx->stream, ctx->d_blocks, ctx->bg_inv, ctx->arena, ctx->c, ctx->cfg.precision));
     return 0;
 }
@@ -1089,6 +1115,14 @@ int st_viscous_all(acfd_ctx *ctx, int store_aux)
         return 0;
     }
     StageTimer t(ctx, ACFD_STAGE_VISCOUS);
+    if (ctx->timing) {
+        for (int part = 1; part <= 2; part++) {
+            StageTimer ts(ctx, part == 1 ? kTimeViscNode : kTimeViscDiff);
+            KRN_TRY(launch_viscous_batch(ctx->stream, ctx->d_blocks, ctx->bg_visc, ctx->arena, ctx->c,
+                                         ctx->cfg.precision, store_aux, part));
+        }
+        return 0;
+    }
     KRN_TRY(launch_viscous_batch(ctx->stream, ctx->d_blocks, ctx->bg_visc, ctx->arena, ctx->c, ctx->cfg.precision,
                                  store_aux));
     return 0;

@@ -307,29 +307,31 @@ static void launch_inviscid_t(hipStream_t s, const Ext &e, const Fields &f, doub
 
 // WENO3 + Roe residual of every block: bg[0] = rhsi zeroing (1-D over 5n),
 // bg[1..3] = the i, j, k kernels
-int launch_inviscid_batch(hipStream_t s, const BlockRec *blks, const BatchGrid *bg, double *arena,
-                          const Consts &c, int precision)
+template <typename R>
+static void inviscid_batch_dirs(hipStream_t s, const BlockRec *blks, const BatchGrid *bg, double *arena, double gamma,
+                                int part)
 {
-    dim3 t(64, 4, 1);
-    if (bg[0].total > 0)
+    const dim3 t(64, 4, 1);
+    if (part < 0 || part == 1)
+        hipLaunchKernelGGL((k_inviscid_batch<R, 0>), dim3((unsigned)bg[1].total), t, 0, s, blks, bg[1].map,
+                           bg[1].kg, arena, gamma);
+    if (part < 0 || part == 2)
+        hipLaunchKernelGGL((k_inviscid_batch<R, 1>), dim3((unsigned)bg[2].total), t, 0, s, blks, bg[2].map,
+                           bg[2].kg, arena, gamma);
+    if (part < 0 || part == 3)
+        hipLaunchKernelGGL((k_inviscid_batch<R, 2>), dim3((unsigned)bg[3].total), t, 0, s, blks, bg[3].map,
+                           bg[3].kg, arena, gamma);
+}
+
+int launch_inviscid_batch(hipStream_t s, const BlockRec *blks, const BatchGrid *bg, double *arena,
+                          const Consts &c, int precision, int part)
+{
+    if (bg[0].total > 0 && (part < 0 || part == 0))
         hipLaunchKernelGGL(k_zero_rhsi_batch, dim3((unsigned)bg[0].total), dim3(256), 0, s, blks, bg[0].map,
                            bg[0].kg, arena);
-    if (bg[1].total <= 0) return 0;
-    if (precision == 1) {
-        hipLaunchKernelGGL((k_inviscid_batch<float, 0>), dim3((unsigned)bg[1].total), t, 0, s, blks, bg[1].map,
-                           bg[1].kg, arena, c.gamma);
-        hipLaunchKernelGGL((k_inviscid_batch<float, 1>), dim3((unsigned)bg[2].total), t, 0, s, blks, bg[2].map,
-                           bg[2].kg, arena, c.gamma);
-        hipLaunchKernelGGL((k_inviscid_batch<float, 2>), dim3((unsigned)bg[3].total), t, 0, s, blks, bg[3].map,
-                           bg[3].kg, arena, c.gamma);
-    } else {
-        hipLaunchKernelGGL((k_inviscid_batch<double, 0>), dim3((unsigned)bg[1].total), t, 0, s, blks, bg[1].map,
-                           bg[1].kg, arena, c.gamma);
-        hipLaunchKernelGGL((k_inviscid_batch<double, 1>), dim3((unsigned)bg[2].total), t, 0, s, blks, bg[2].map,
-                           bg[2].kg, arena, c.gamma);
-        hipLaunchKernelGGL((k_inviscid_batch<double, 2>), dim3((unsigned)bg[3].total), t, 0, s, blks, bg[3].map,
-                           bg[3].kg, arena, c.gamma);
-    }
+    if (bg[1].total <= 0 || part == 0) return 0;
+    if (precision == 1) inviscid_batch_dirs<float>(s, blks, bg, arena, c.gamma, part);
+    else inviscid_batch_dirs<double>(s, blks, bg, arena, c.gamma, part);
     ACFD_CHECK_LAUNCH();
     return 0;
 }
@@ -539,15 +541,18 @@ int launch_viscous(hipStream_t s, const Ext &e, const Fields &f, const Consts &c
 }
 
 int launch_viscous_batch(hipStream_t s, const BlockRec *blks, const BatchGrid &bg, double *arena, const Consts &c,
-                         int precision, int store_aux)
+                         int precision, int store_aux, int part)
 {
     if (bg.total <= 0) return 0;
     dim3 g((unsigned)bg.total), t(64, 4, 1);
-    if (precision == 1)
-        hipLaunchKernelGGL((k_viscous_node_batch<float>), g, t, 0, s, blks, bg.map, bg.kg, arena, c, store_aux);
-    else
-        hipLaunchKernelGGL((k_viscous_node_batch<double>), g, t, 0, s, blks, bg.map, bg.kg, arena, c, store_aux);
-    hipLaunchKernelGGL(k_viscous_diff_batch, g, t, 0, s, blks, bg.map, bg.kg, arena, c.re);
+    if (part < 0 || part == 1) {
+        if (precision == 1)
+            hipLaunchKernelGGL((k_viscous_node_batch<float>), g, t, 0, s, blks, bg.map, bg.kg, arena, c, store_aux);
+        else
+            hipLaunchKernelGGL((k_viscous_node_batch<double>), g, t, 0, s, blks, bg.map, bg.kg, arena, c, store_aux);
+    }
+    if (part < 0 || part == 2)
+        hipLaunchKernelGGL(k_viscous_diff_batch, g, t, 0, s, blks, bg.map, bg.kg, arena, c.re);
     ACFD_CHECK_LAUNCH();
     return 0;
 }

@@ -1315,24 +1315,26 @@ int sa_batch_make(int nblk, const Ext *e, const Fields *f, double *arena, int sh
 }
 
 template <int TJ, int TK>
-static void sa_batch_t(hipStream_t s, const LuBatch &b, int *err, int *ctr)
+static void sa_batch_t(hipStream_t s, const LuBatch &b, int *err, int *ctr, int dir)
 {
     const SaBatchEnt *ents = (const SaBatchEnt *)b.ents;
-    hipLaunchKernelGGL((k_sa_tiles_batch<1, TJ, TK>), dim3((unsigned)b.total), dim3(TJ * TK), 0, s, ents,
-                       b.tile_ent, b.total, b.resident[0], b.arena, b.ex, err, ctr);
-    hipLaunchKernelGGL((k_sa_tiles_batch<-1, TJ, TK>), dim3((unsigned)b.total), dim3(TJ * TK), 0, s, ents,
-                       b.tile_ent, b.total, b.resident[1], b.arena, b.ex, err, ctr);
+    if (dir >= 0)
+        hipLaunchKernelGGL((k_sa_tiles_batch<1, TJ, TK>), dim3((unsigned)b.total), dim3(TJ * TK), 0, s, ents,
+                           b.tile_ent, b.total, b.resident[0], b.arena, b.ex, err, ctr);
+    if (dir <= 0)
+        hipLaunchKernelGGL((k_sa_tiles_batch<-1, TJ, TK>), dim3((unsigned)b.total), dim3(TJ * TK), 0, s, ents,
+                           b.tile_ent, b.total, b.resident[1], b.arena, b.ex, err, ctr);
 }
 
-// both SA sweeps of every block of the batch
-int launch_sa_batch(hipStream_t s, const LuBatch &b, int *err, int *ctr)
+// the SA sweeps of every block of the batch (dir 0: both, in order)
+int launch_sa_batch(hipStream_t s, const LuBatch &b, int *err, int *ctr, int dir)
 {
     if (b.total <= 0) return 0;
     switch (b.shape) {
-    case 0: sa_batch_t<16, 16>(s, b, err, ctr); break;
-    case 1: sa_batch_t<16, 8>(s, b, err, ctr); break;
-    case 2: sa_batch_t<8, 8>(s, b, err, ctr); break;
-    default: sa_batch_t<16, 4>(s, b, err, ctr); break;
+    case 0: sa_batch_t<16, 16>(s, b, err, ctr, dir); break;
+    case 1: sa_batch_t<16, 8>(s, b, err, ctr, dir); break;
+    case 2: sa_batch_t<8, 8>(s, b, err, ctr, dir); break;
+    default: sa_batch_t<16, 4>(s, b, err, ctr, dir); break;
     }
     ACFD_CHECK_LAUNCH();
     return 0;

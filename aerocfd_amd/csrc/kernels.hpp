@@ -52,10 +52,13 @@ dim3 viscous_grid(const Ext &e);
 dim3 lu_prep_grid(const Ext &e);
 int launch_getpri_batch(hipStream_t s, const BlockRec *blks, const BatchGrid &bg, double *arena, const Consts &c,
                         unsigned int *nneg);
+// part -1: every kernel of the stage; otherwise only one of them (inviscid:
+// 0 clear rhsi, 1..3 the i, j, k directions; viscous: 1 node fluxes, 2
+// their difference), for per-kernel timing
 int launch_inviscid_batch(hipStream_t s, const BlockRec *blks, const BatchGrid *bg, double *arena,
-                          const Consts &c, int precision);
+                          const Consts &c, int precision, int part = -1);
 int launch_viscous_batch(hipStream_t s, const BlockRec *blks, const BatchGrid &bg, double *arena, const Consts &c,
-                         int precision, int store_aux);
+                         int precision, int store_aux, int part = -1);
 int launch_find_dt_batch(hipStream_t s, const BlockRec *blks, int nblk, const BatchGrid &bg, double *arena,
                          const Consts &c, double *cfl, double *cfl_blk, unsigned long long *stats);
 int launch_lu_prep_batch(hipStream_t s, const BlockRec *blks, const BatchGrid &bg, double *arena, const double *cfl,
@@ -118,7 +121,8 @@ int launch_fill_sentinel(hipStream_t s, double *p, long n);
 int launch_sa_sweeps(hipStream_t s, const Ext &e, int shape, const double *cu, const double *rhs, double *dns,
                      double *dn, double *ex, int *err);
 int sa_batch_make(int nblk, const Ext *e, const Fields *f, double *arena, int shape, double *ex, LuBatch *out);
-int launch_sa_batch(hipStream_t s, const LuBatch &b, int *err, int *ctr);
+// dir 1: the forward sweep only, -1: the backward sweep only, 0: both
+int launch_sa_batch(hipStream_t s, const LuBatch &b, int *err, int *ctr, int dir = 0);
 
 // k_bdf2.hip
 int launch_crank(hipStream_t s, const Ext &e, const Fields &f, const Consts &c);

@@ -11,8 +11,9 @@
                   subface_t ori fwh, one closing line)
   flowfield       sub_output_flowfield.f90:30-81 (writer) and
                   sub_init_flowfield.f90:126-262 (restart reader): per
-                  block a variables line, a zone line, then one node per
-                  line x y z d u v w p nut over the physical range
+                  block a variables line, a zone line, then one record per
+                  node x y z d u v w p nut over the physical range (a
+                  record may wrap over two lines)
   resi_history    sub_output_residual.f90:62-64 (I6.6, 6E11.4)
   clcd_history    sub_output_forceintegration.f90:472-475 (time, cl, cd)
 
@@ -213,6 +214,20 @@ def write_flowfield(path, ranges, coords, pri_v, nut):
                 fh.write(" ".join(repr(float(v)) for v in row) + "\n")
 
 
+def _record(fh, count):
+    """The first `count` values of one list-directed READ: the values may
+    run on over several lines (flang's list-directed output wraps a long
+    record, e.g. a flowfield node of nine doubles), and the rest of the last
+    line read is skipped."""
+    toks = []
+    while len(toks) < count:
+        line = fh.readline()
+        if not line:
+            raise ValueError("end of file inside a record of %d values" % count)
+        toks.extend(list_directed(line))
+    return toks[:count]
+
+
 def read_flowfield(path, dims, ma, gamma=1.4):
     """Restart file -> per block dict with x, y, z, q (nk, nj, ni, 5), nut
     and pri_v as sub_init_flowfield.f90:206-258 sets them (pri_v includes
@@ -225,7 +240,7 @@ def read_flowfield(path, dims, ma, gamma=1.4):
             fh.readline()
             fh.readline()
             n = ni * nj * nk
-            a = np.array([[_real(t) for t in list_directed(fh.readline())[:9]] for _ in range(n)])
+            a = np.array([[_real(t) for t in _record(fh, 9)] for _ in range(n)])
             x, y, z, d, u, v, w, p, tur = (a[:, c].reshape(nk, nj, ni) for c in range(9))
             t = p / d * gamma * ma * ma
             q = np.stack([d, d * u, d * v, d * w, d * t * cv + 0.5 * d * (u * u + v * v + w * w)], -1)
@@ -258,6 +273,21 @@ def residual_line(timestep, res):
     """One line of resi_history.dat: (I6.6, 6E11.4) of timestep, the five
     flow residuals and the SA residual (sub_output_residual.f90:63-64)."""
     return "%06d" % timestep + "".join(_e11_4(float(v)) for v in list(res[:5]) + [res[5]])
+
+
+def read_residuals(path):
+    """resi_history.dat -> array of rows (timestep, five flow residuals, SA
+    residual).  The header is list-directed output of one string, which
+    the reference's writer may wrap over two lines; data lines are the
+    (I6.6, 6E11.4) records."""
+    rows = []
+    with open(path) as fh:
+        for line in fh:
+            line = line.rstrip("\n")
+            if len(line) < 72 or not line[:6].isdigit():
+                continue
+            rows.append([int(line[:6])] + [float(line[6 + 11 * n:17 + 11 * n]) for n in range(6)])
+    return np.array(rows).reshape(-1, 7)
 
 
 def append_residual(path, timestep, res):

@@ -42,8 +42,11 @@ STAGES = {
 # sub-stage timers of the LU-SGS update: the two sweeps, the gather into the
 # sweep layout (tile sweeps), rhs / spectral radii / diag, and q += dq
 TIMED_KERNELS = {"lusgs_forward": 12, "lusgs_backward": 13, "lusgs_gather": 14, "lusgs_other": 15,
-                 "lusgs_add": 16}
-TIME_SLOTS = 20
+                 "lusgs_add": 16,
+                 # one kernel each, recorded for the batched stages
+                 "sa_forward": 17, "sa_backward": 18, "sa_pre": 19, "sa_add": 20, "inviscid_zero": 21,
+                 "inviscid_i": 22, "inviscid_j": 23, "inviscid_k": 24, "viscous_node": 25, "viscous_diff": 26}
+TIME_SLOTS = 32
 
 GAMMA = 1.4
 

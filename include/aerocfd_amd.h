@@ -135,11 +135,15 @@ int acfd_sync(acfd_ctx *ctx);
 /* Per-stage GPU time accumulated with HIP events on the context's stream.
  * Returns the totals recorded so far (ms/calls may be NULL); then, for
  * enable >= 0, clears them and turns recording on (enable > 0) or off
- * (enable == 0); enable < 0 only reads.  ms and calls have 20 entries: [stage] for
+ * (enable == 0); enable < 0 only reads.  ms and calls have 32 entries: [stage] for
  * stages 1..11, and sub-stages of ACFD_STAGE_IMPLICIT1ST: [12] forward
  * sweep, [13] backward sweep, [14] gather into the sweep layout (tile
- * sweeps), [15] rhs / spectral radii / diagonal, [16] q += dq.  Indices 0
- * and 17..19 are unused.  The
+ * sweeps), [15] rhs / spectral radii / diagonal, [16] q += dq.  With
+ * batch_blocks / lusgs_kernel 2 each kernel of a batched stage is also
+ * timed on its own: [17] SA forward sweep, [18] SA backward sweep, [19] SA
+ * length scale / right-hand side / diagonal, [20] nut += dnut, [21] rhsi
+ * cleared, [22..24] inviscid i, j, k direction, [25] viscous node fluxes,
+ * [26] their difference.  Indices 0 and 27..31 are unused.  The
  * reference's only instrument is a per-step wall time (main.f90:575-586). */
 int acfd_timing(acfd_ctx *ctx, int enable, double *ms, long *calls);
 

@@ -1,0 +1,192 @@
+"""The file formats against the reference's own program.
+
+The reference's whole program (main.f90 and every other source file,
+compiled in place into oracle/_ref/aerocfd_ref.x) runs under MPI on cases
+written by aerocfd_amd.formats: Control.in, the grid and bcinfo files and,
+for a restart start, the per-rank flowfield files.  What it writes back --
+the flowfield files after the last step and resi_history.dat -- is read
+with aerocfd_amd.formats and compared with the oracle driven through the
+same start-up and time loop (main.f90:150-613) on the same inputs.  The
+program's own readers (sub_init_readcontrolfile.f90:27-47,
+sub_init_allocatememory.f90:36-71,388-417, sub_init_readbcinfo.f90:45-104,
+sub_init_flowfield.f90:126-262) and writers (sub_output_flowfield.f90:30-81,
+sub_output_residual.f90:62-64) are thereby pinned to formats.py.
+
+Skipped where the compiled reference is not built (the GPU box).
+"""
+import numpy as np
+import pytest
+
+from aerocfd_amd import cases, formats
+from oracle import oracle as O
+from oracle import refrun as RR
+from tests.helpers import ora_config
+
+pytestmark = pytest.mark.skipif(not RR.available(), reason="compiled reference program not built")
+
+GAMMA = 1.4
+
+
+def _phys_slice(e):
+    return (slice(e.ks0 - e.ks, e.ke0 - e.ks + 1), slice(e.js0 - e.js, e.je0 - e.js + 1),
+            slice(e.is0 - e.is_, e.ie0 - e.is_ + 1))
+
+
+def oracle_whole_run(cfg, dims, subfaces, coords, nsteps, restart=None):
+    """main.f90:150-613 with the oracle: init_flowfield (uniform start, or
+    the values a restart file gives, :43-262), init_spacing, init_walldist
+    (SA), gridtransformation, physical_bc + getpri, the boundary refresh and
+    nsteps steps.  coords[m] = (x, y, z) over the extended extents with the
+    interface halos filled; restart[m] = the reader's dict of block m."""
+    totalnodes = int(sum(a * b * c for a, b, c in dims))
+    c = ora_config(cfg, totalnodes)
+    exts = [O.Ext.from_list(cases.ranges_of(d, s)) for d, s in zip(dims, subfaces)]
+    states = [O.BlockState(e, s) for e, s in zip(exts, subfaces)]
+    sa = cfg.iflag_turbulence == 2
+    wall = O.wall_points(exts, subfaces, coords) if sa and cfg.iflag_solver == 0 else None
+    for m, (st, e) in enumerate(zip(states, exts)):
+        x, y, z = coords[m]
+        if restart is None:
+            q = np.zeros(e.shape + (5,))
+            q[..., 0] = c.d0
+            q[..., 1] = c.d0 * c.u0 * np.cos(c.aoa)
+            q[..., 2] = c.d0 * c.u0 * np.sin(c.aoa)
+            q[..., 4] = c.d0 * 1.0 * c.cv + 0.5 * c.d0 * c.u0 * c.u0  # t_0 = 1 (sub_init_flowfield.f90:32)
+            st["q"] = q
+            st["dt"] = cfg.dt
+            amu = np.zeros(e.shape + (3,))
+            amu[..., 0] = 1.0
+            if sa:
+                st["nut"] = cases.NUT_INF
+                amu[..., 1] = 0.009
+            amu[..., 2] = amu[..., 0] + amu[..., 1]
+            st["amu"] = amu
+        else:
+            ph = _phys_slice(e)
+            q = np.zeros(e.shape + (5,))
+            q[ph] = restart[m]["q"]
+            st["q"] = q
+            pri = np.zeros(e.shape + (7,))
+            pri[ph] = restart[m]["pri_v"]
+            st["pri_v"] = pri
+            dt = np.zeros(e.shape)
+            dt[ph] = cfg.dt
+            st["dt"] = dt
+            amu = np.zeros(e.shape + (3,))
+            amu[ph + (0,)] = 1.0
+            st["amu"] = amu
+            if sa:
+                nut = np.zeros(e.shape)
+                nut[ph] = restart[m]["nut"]
+                st["nut"] = nut
+        f = dict(zip(("dxidx", "inv_j", "aalpha", "bbeta", "ggamma"), O.gridtransformation(e, x, y, z)))
+        if sa:
+            f["spacing"] = O.spacing(e, x, y, z)[3]
+            if wall is not None:
+                f["dist"] = O.walldist(e, x, y, z, wall)
+        for k, v in f.items():
+            st[k] = np.nan_to_num(v, nan=0.0)
+    O.physical_bc(states, c)
+    for st in states:
+        st["pri_v"] = O.getpri(st.ext, st["q"], c.gamma, c.cv)[0]
+    O.prepare(states, c)
+    res = []
+    for _ in range(nsteps):
+        O.step(states, c)
+        res.append(O.residual(states, totalnodes))
+    return states, np.array(res)
+
+
+def raw_columns(path, dims):
+    """The nine written values of every node of every block, as the
+    reference printed them (restated from the flowfield reader's record
+    rule)."""
+    out = []
+    with open(path) as fh:
+        for ni, nj, nk in dims:
+            fh.readline()
+            fh.readline()
+            out.append(np.array([[formats._real(t) for t in formats._record(fh, 9)]
+                                 for _ in range(ni * nj * nk)]))
+    return out
+
+
+def check_against_program(work, cfg, dims, subfaces, ranks, nsteps, states, res_o):
+    sa = cfg.iflag_turbulence == 2
+    for rank, blocks in enumerate(RR.rank_blocks(len(dims), ranks)):
+        path = "%s/result/flowfield_%010d_%04d.dat" % (work, nsteps, rank)
+        cols = raw_columns(path, [dims[m] for m in blocks])
+        for m, a in zip(blocks, cols):
+            st = states[m]
+            ph = _phys_slice(st.ext)
+            pri = st["pri_v"][ph]
+            want = np.stack([pri[..., 0], pri[..., 1], pri[..., 2], pri[..., 3], pri[..., 4],
+                             st["nut"][ph]], -1).reshape(-1, 6)
+            # the printed digits round-trip: the values agree bit for bit
+            assert np.array_equal(a[:, 3:9], want), "block %d: max diff %.3e" % (
+                m + 1, np.abs(a[:, 3:9] - want).max())
+    rows = RR.read_residuals(work)
+    assert rows.shape[0] == nsteps and list(rows[:, 0]) == list(range(1, nsteps + 1))
+    for n in range(nsteps):
+        want = [formats._e11_4(v) for v in list(res_o[n][:5]) + [res_o[n][5]]]
+        got = formats.residual_line(n + 1, list(res_o[n][:5]) + [res_o[n][5]])
+        line = "%06d" % (n + 1) + "".join(want)
+        assert got == line
+        assert np.allclose(rows[n, 1:], [float(w) for w in want], rtol=0, atol=0)
+    # the flowfield reader gives the reference's own q and pri_v
+    got = RR.read_output(work, dims, subfaces, ranks, nsteps, cfg.ma)
+    for m, g in enumerate(got):
+        ph = _phys_slice(states[m].ext)
+        assert np.array_equal(g["pri_v"][..., 0], states[m]["pri_v"][ph][..., 0])
+
+
+def vortex_case(n=10, cuts=((5,), (), ())):
+    case = cases.vortex_box(n, blocks=cuts)
+    h = 10.0 / (n - 1)
+    dims, subs, origins = cases.split_box(n, n, n, cuts, phys=(-10,) * 6)
+    coords = []
+    for m in range(len(dims)):
+        org, perm = origins[m]
+        xyz = cases.node_coords(case.ranges[m], org, (h, h, h), (-5.0, -5.0, -5.0), perm)
+        coords.append((xyz[0], xyz[1], xyz[2]))
+    return case, coords
+
+
+@pytest.mark.parametrize("ranks", [1, 2])
+def test_restart_run_matches_program(tmp_path, ranks):
+    """Euler vortex in two blocks, started from a restart file written by
+    formats.write_flowfield (iflag_init 1), 3 LU-SGS steps."""
+    case, coords = vortex_case()
+    case.cfg.dt = 0.05
+    nsteps = 3
+    cv = 1.0 / GAMMA / (GAMMA - 1.0) / case.cfg.ma ** 2
+    restart = []
+    for m in range(len(case.dims)):
+        pri = O.getpri(O.Ext.from_list(case.ranges[m]), case.fields[m]["q"], GAMMA, cv)[0]
+        restart.append((case.ranges[m], pri, np.zeros(pri.shape[:-1])))
+    work = str(tmp_path)
+    RR.write_case(work, case.cfg, case.dims, case.subfaces, coords, nsteps, ranks, restart=restart)
+    # what the program reads back as its start (the reader's arithmetic)
+    start = []
+    for rank, blocks in enumerate(RR.rank_blocks(len(case.dims), ranks)):
+        start += formats.read_flowfield("%s/result/flowfield_%010d_%04d.dat" % (work, 0, rank),
+                                        [case.dims[m] for m in blocks], case.cfg.ma)
+    out, _ = RR.run(work, ranks, timeout=120)
+    assert len(RR.step_times(out)) == nsteps
+    states, res_o = oracle_whole_run(case.cfg, case.dims, case.subfaces, coords, nsteps, restart=start)
+    check_against_program(work, case.cfg, case.dims, case.subfaces, ranks, nsteps, states, res_o)
+
+
+@pytest.mark.parametrize("ranks", [3])
+def test_uniform_start_sa_run_matches_program(tmp_path, ranks):
+    """A reduced 6-block SA flat plate from the uniform start (iflag_init
+    0): the program computes metrics, edge lengths and wall distance from
+    the grid file itself; 3 steps of local-time-step LU-SGS with SA."""
+    case = cases.flat_plate(nblk=(3, 2, 1), lo=(7, 7, 5), hi=(9, 9, 6), first=1e-3)
+    nsteps = 3
+    work = str(tmp_path)
+    RR.write_case(work, case.cfg, case.dims, case.subfaces, case.coords, nsteps, ranks)
+    out, _ = RR.run(work, ranks, timeout=120)
+    assert len(RR.step_times(out)) == nsteps
+    states, res_o = oracle_whole_run(case.cfg, case.dims, case.subfaces, case.coords, nsteps)
+    check_against_program(work, case.cfg, case.dims, case.subfaces, ranks, nsteps, states, res_o)

@@ -255,7 +255,7 @@ def test_stage_boundaries(sa):
         sol.close()
 
 
-@pytest.mark.parametrize("ori", [1, 3, 4])
+@pytest.mark.parametrize("ori", [1, 2, 3, 4])
 def test_halo_orientations(ori):
     """Interfaces with every orientation code on square faces (not a
     geometric pairing: the copy rule itself is what is compared)."""
