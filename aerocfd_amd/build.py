@@ -34,6 +34,18 @@ def build(verbose=False):
     return out
 
 
+def build_fortran_host():
+    """The Fortran host (aerocfd_amd/fortran/acfd_host): the ISO_C_BINDING
+    module include/aerocfd_amd.f90 and acfd_host.f90, linked with amdflang
+    against the library built above."""
+    d = os.path.join(HERE, "fortran")
+    r = subprocess.run(["make", "-B", "-C", d], capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("Fortran host build failed:\n" + r.stdout[-3000:] + r.stderr[-3000:])
+    return os.path.join(d, "acfd_host")
+
+
 if __name__ == "__main__":
     print(build(verbose=True))
+    print(build_fortran_host())
     sys.exit(0)

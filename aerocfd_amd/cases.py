@@ -100,13 +100,17 @@ def box_subfaces(ni, nj, nk, kinds):
 
 
 def split_box(NI, NJ, NK, cuts=((), (), ()), phys=(-10,) * 6, periodic=(False, False, False),
-              permute=None):
+              permute=None, flip=None):
     """Cut a box of NI x NJ x NK nodes into blocks at the given node indices
     (1-based global, shared by both sides).  phys: physical code per box face
     where not periodic.  permute: optional per-block axis permutation
-    (a tuple of 3) applied to the block's local axes, to exercise every
-    orientation code.  Returns dims, subfaces, origins (global index of local
-    node (1,1,1) and the axis map of each block)."""
+    (a tuple of 3: local axis a carries global axis permute[x][a]) applied
+    to the block's local axes; flip: optional per-block tuple of 3 booleans,
+    local axis a running against its global axis.  Permutations and flips
+    whose frame stays right-handed give interfaces of every orientation code.
+    Returns dims, subfaces, origins: per block the global index of local
+    node (1,1,1) and the axis map, plus the axis signs (+1 / -1) when flip
+    is given."""
     edges = []
     for ax, n in enumerate((NI, NJ, NK)):
         e = [1] + sorted(cuts[ax]) + [n]
@@ -121,16 +125,22 @@ def split_box(NI, NJ, NK, cuts=((), (), ()), phys=(-10,) * 6, periodic=(False, F
                 blocks.append((bi, bj, bk, lo, hi))
     bid = {(b[0], b[1], b[2]): x for x, b in enumerate(blocks)}
     perms = []
+    signs = []
     dims = []
     for x, (bi, bj, bk, lo, hi) in enumerate(blocks):
         p = tuple(permute[x]) if permute else (0, 1, 2)
+        sg = tuple(-1 if f else 1 for f in flip[x]) if flip else (1, 1, 1)
+        odd = sum(p[a] > p[b] for a in range(3) for b in range(a + 1, 3)) % 2
+        if (odd + sum(v < 0 for v in sg)) % 2:
+            raise ValueError("block %d: axis map %s with signs %s is left-handed" % (x + 1, p, sg))
         perms.append(p)
+        signs.append(sg)
         ext = [hi[a] - lo[a] + 1 for a in range(3)]
         dims.append(tuple(ext[p[a]] for a in range(3)))
     # face of block x (global axis ax, side sd) in local face numbering
     def local_face(x, ax, sd):
         la = perms[x].index(ax)
-        return 2 * la + 1 + sd
+        return 2 * la + 1 + (sd if signs[x][la] > 0 else 1 - sd)
     subfaces = [[None] * 6 for _ in blocks]
     for x, (bi, bj, bk, lo, hi) in enumerate(blocks):
         idx = (bi, bj, bk)
@@ -156,17 +166,25 @@ def split_box(NI, NJ, NK, cuts=((), (), ()), phys=(-10,) * 6, periodic=(False, F
                     subfaces[x][f - 1] = [f, f, *rng, y + 1, fy, 0]
     periods = (NI - 1, NJ - 1, NK - 1)
 
+    def first(m):
+        # global index of local node (1,1,1)
+        lo, hi = blocks[m][3], blocks[m][4]
+        return tuple(hi[a] if signs[m][perms[m].index(a)] < 0 else lo[a] for a in range(3))
+
     def gid(m, i, j, k):
         loc = (i, j, k)
         p = perms[m]
-        lo = blocks[m][3]
+        o = first(m)
         g = [0, 0, 0]
         for a in range(3):
-            g[p[a]] = lo[p[a]] + loc[a] - 1
+            g[p[a]] = o[p[a]] + signs[m][a] * (loc[a] - 1)
         return tuple(g[a] % periods[a] if periodic[a] else g[a] for a in range(3))
 
     assign_orientations(dims, subfaces, gid)
-    origins = [(b[3], perms[x]) for x, b in enumerate(blocks)]
+    if flip:
+        origins = [(first(x), perms[x], signs[x]) for x in range(len(blocks))]
+    else:
+        origins = [(b[3], perms[x]) for x, b in enumerate(blocks)]
     return dims, subfaces, origins
 
 
@@ -187,11 +205,12 @@ def ranges_of(dims_m, subs_m):
     return tuple(r + r0 + r1)
 
 
-def node_coords(ranges, origin, spacing, x0, perm=(0, 1, 2)):
+def node_coords(ranges, origin, spacing, x0, perm=(0, 1, 2), sign=(1, 1, 1)):
     """Cartesian coordinates (3, nk, nj, ni) of a uniform-grid block whose
-    local node (1,1,1) is global node ``origin``; halo nodes extrapolate the
-    lattice (the values the coordinate exchange delivers on a uniform grid
-    with periodic shifts)."""
+    local node (1,1,1) is global node ``origin`` (local axis a runs along
+    global axis perm[a] with sign[a]); halo nodes extrapolate the lattice
+    (the values the coordinate exchange delivers on a uniform grid with
+    periodic shifts)."""
     is_, ie, js, je, ks, ke = ranges[:6]
     li = np.arange(is_, ie + 1)
     lj = np.arange(js, je + 1)
@@ -201,11 +220,11 @@ def node_coords(ranges, origin, spacing, x0, perm=(0, 1, 2)):
     xyz = np.zeros((3,) + I.shape)
     for a in range(3):
         ga = perm.index(a)  # local axis that carries global axis a
-        xyz[a] = x0[a] + spacing[a] * (origin[a] + loc[ga] - 2)
+        xyz[a] = x0[a] + spacing[a] * (origin[a] - 1 + sign[ga] * (loc[ga] - 1))
     return xyz
 
 
-def uniform_metrics(ranges, spacing, perm=(0, 1, 2)):
+def uniform_metrics(ranges, spacing, perm=(0, 1, 2), sign=(1, 1, 1)):
     """Metrics of a uniform Cartesian block: dxidx(3(d-1)+c) = d xi_d/d x_c,
     inv_j = 1/J, and alpha/beta/gamma(c) = (grad xi_d . grad xi_c)/J -- the
     values gridtransformation (sub_init_gridtransformation.f90:27-874) returns
@@ -216,7 +235,7 @@ def uniform_metrics(ranges, spacing, perm=(0, 1, 2)):
     vol = h[0] * h[1] * h[2]
     dxidx = np.zeros(shp + (9,))
     for d in range(3):
-        dxidx[..., 3 * d + perm[d]] = 1.0 / h[d]
+        dxidx[..., 3 * d + perm[d]] = sign[d] / h[d]
     inv_j = np.full(shp, vol)
     abg = []
     for d in range(3):
@@ -308,7 +327,7 @@ def _common_fields(case, m, q, dxidx, inv_j, al, be, ga):
         fl["aalpha"], fl["bbeta"], fl["ggamma"] = al, be, ga
 
 
-def vortex_box(n=64, cfg=None, blocks=((), (), ()), permute=None):
+def vortex_box(n=64, cfg=None, blocks=((), (), ()), permute=None, flip=None):
     """Single (or cut) block n^3 on [-5,5]^3, farfield everywhere, isentropic
     vortex; Euler, first-order implicit, dt = 0.5*(10/(n-1))/(1+2)."""
     from .solver import FlowConfig
@@ -316,17 +335,19 @@ def vortex_box(n=64, cfg=None, blocks=((), (), ()), permute=None):
     if cfg is None:
         cfg = FlowConfig(ma=0.5, re=1.0e6, iflag_solver=1, iflag_time=1, iflag_timeadvance=2,
                          dt=0.5 * h / (1.0 + 2.0))
-    dims, subs, origins = split_box(n, n, n, blocks, phys=(-10,) * 6, permute=permute)
+    dims, subs, origins = split_box(n, n, n, blocks, phys=(-10,) * 6, permute=permute, flip=flip)
     case = Case("vortex_box_%d" % n, cfg, dims, subs)
+    case.coords = []
     for m in range(len(dims)):
-        org, perm = origins[m]
-        xyz = node_coords(case.ranges[m], org, (h, h, h), (-5.0, -5.0, -5.0), perm)
+        org, perm, sign = (origins[m] + ((1, 1, 1),))[:3]
+        xyz = node_coords(case.ranges[m], org, (h, h, h), (-5.0, -5.0, -5.0), perm, sign)
+        case.coords.append((xyz[0], xyz[1], xyz[2]))
         q = vortex_state(xyz, cfg.ma)
-        _common_fields(case, m, q, *uniform_metrics(case.ranges[m], (h, h, h), perm))
+        _common_fields(case, m, q, *uniform_metrics(case.ranges[m], (h, h, h), perm, sign))
     return case
 
 
-def tgv_box(n=256, cfg=None, blocks=((), (), ()), permute=None):
+def tgv_box(n=256, cfg=None, blocks=((), (), ()), permute=None, flip=None):
     """n^3 nodes on [0,2 pi]^3, node n == node 1 on every face pair (all
     faces self-connected or connected through the cut blocks), Taylor-Green
     vortex at Mach 0.1, Re = 1600, laminar NS, first-order implicit."""
@@ -335,13 +356,13 @@ def tgv_box(n=256, cfg=None, blocks=((), (), ()), permute=None):
     if cfg is None:
         cfg = FlowConfig(ma=0.1, re=1600.0, iflag_solver=0, iflag_time=1, iflag_timeadvance=2,
                          dt=0.5 * h / (1.0 + 10.0), prl=0.72)
-    dims, subs, origins = split_box(n, n, n, blocks, periodic=(True, True, True), permute=permute)
+    dims, subs, origins = split_box(n, n, n, blocks, periodic=(True, True, True), permute=permute, flip=flip)
     case = Case("tgv_box_%d" % n, cfg, dims, subs)
     for m in range(len(dims)):
-        org, perm = origins[m]
-        xyz = node_coords(case.ranges[m], org, (h, h, h), (0.0, 0.0, 0.0), perm)
+        org, perm, sign = (origins[m] + ((1, 1, 1),))[:3]
+        xyz = node_coords(case.ranges[m], org, (h, h, h), (0.0, 0.0, 0.0), perm, sign)
         q = tgv_state(xyz, cfg.ma)
-        _common_fields(case, m, q, *uniform_metrics(case.ranges[m], (h, h, h), perm))
+        _common_fields(case, m, q, *uniform_metrics(case.ranges[m], (h, h, h), perm, sign))
     return case
 
 

@@ -211,6 +211,9 @@ void ora_inviscid_ext(const ora_ext *e, double *rhsi, const double *q, const dou
     for (int d = 0; d < 3; d++) {
         long st = d == 0 ? 1 : d == 1 ? (long)(e->ie - e->is + 1) : (long)(e->ie - e->is + 1) * (e->je - e->js + 1);
         int lo = lo3[d], hi = hi3[d];
+        /* every node's fluxes are its own: the same results on any number
+         * of threads (only the run time of the checker changes) */
+#pragma omp parallel for collapse(2) schedule(static)
         for (int k = e->ks1; k <= e->ke1; k++)
             for (int j = e->js1; j <= e->je1; j++)
                 for (int i = e->is1; i <= e->ie1; i++) {

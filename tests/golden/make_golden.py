@@ -20,6 +20,9 @@ Cases (all small, a few hundred nodes per block):
   steps_s{solver}_t{turb}_a{tadv}  two 5^3 blocks with every physical
                condition and an interface: three whole time steps
                (main.f90:343-613)
+  steps100_{flat_plate,vortex,tgv}  100 whole time steps of reduced
+               generated cases: the 12-block SA flat plate, the vortex and
+               the Taylor-Green vortex in two blocks
   grid         a warped 4-block grid with interface halos and wall faces:
                gridtransformation, init_spacing, init_walldist (outputs
                NaN where the reference leaves them undefined)
@@ -215,6 +218,52 @@ def case_steps(solver, turb, tadv, nsteps=3):
 
 
 STEP_CASES = [(1, 0, 2), (0, 0, 2), (0, 2, 2), (0, 0, 3), (1, 0, 3)]
+
+
+def case_run(case, nsteps):
+    """nsteps whole time steps (main.f90:343-613) of a generated case
+    through the reference's harness: its inputs, and q, nut, pri_v and the
+    residual norms after the last step."""
+    c = case.cfg
+    R.setup(solver=c.iflag_solver, turb=c.iflag_turbulence, itime=c.iflag_time, des=c.iflag_des,
+            tadv=c.iflag_timeadvance, ma=c.ma, re=c.re, tinf=c.tinf, cfl=c.cfl, dt=c.dt, aoa_deg=c.aoa_deg)
+    exts = R.begin(case.dims, case.subfaces)
+    sc = R.scalars()
+    out = dict(dims=np.array(case.dims, dtype=np.int32), subfaces=subs_array(case.subfaces),
+               nsub=np.array([len(x) for x in case.subfaces], dtype=np.int32),
+               flags=np.array([c.iflag_solver, c.iflag_turbulence, c.iflag_time, c.iflag_timeadvance, nsteps,
+                               c.iflag_des], dtype=np.int32),
+               params=np.array([c.ma, c.re, c.tinf, c.cfl, c.dt, c.aoa_deg]))
+    for m, e in enumerate(exts):
+        f = dict(case.fields[m])
+        f["pri_v"] = O.getpri(e, f["q"], sc["gamma"], sc["cv"])[0]
+        out["ext%d" % m] = np.array(e.as_list(), dtype=np.int32)
+        for k, v in f.items():
+            if k in R.XFER:
+                R.put(m + 1, k, v)
+                out["in_%s%d" % (k, m)] = v
+    for _ in range(nsteps):
+        R.step()
+    for m, e in enumerate(exts):
+        for k in ("q", "nut", "pri_v"):
+            out["out_%s%d" % (k, m)] = R.get(m + 1, k, e)
+    return out
+
+
+def case_steps100_flat_plate():
+    """A reduced 12-block SA flat plate (the batched workload's layout,
+    every physical condition), local time step, 100 steps."""
+    return case_run(cases.flat_plate(nblk=(4, 3, 1), lo=(7, 7, 5), hi=(9, 9, 6), first=1e-3), 100)
+
+
+def case_steps100_vortex():
+    """The isentropic vortex in two blocks (Euler, unsteady), 100 steps."""
+    return case_run(cases.vortex_box(10, blocks=((5,), (), ())), 100)
+
+
+def case_steps100_tgv():
+    """Taylor-Green in two self-connected blocks (laminar NS), 100 steps."""
+    return case_run(cases.tgv_box(10, blocks=((5,), (), ())), 100)
 GRID_PHYS = (-20, -40, -20, -10, -30, -20)
 
 
@@ -305,6 +354,9 @@ def main():
         makers["halo_ori%d" % ori] = (lambda o=ori: case_halo(o))
     for s, t, a in STEP_CASES:
         makers["steps_s%d_t%d_a%d" % (s, t, a)] = (lambda s=s, t=t, a=a: case_steps(s, t, a))
+    makers["steps100_flat_plate"] = case_steps100_flat_plate
+    makers["steps100_vortex"] = case_steps100_vortex
+    makers["steps100_tgv"] = case_steps100_tgv
     names = sys.argv[1:] or list(makers)
     made = {}
     try:

@@ -26,7 +26,7 @@ def test_manifest_lists_every_fixture():
     with open(os.path.join(GOLD, "manifest.json")) as fh:
         man = json.load(fh)
     names = [k for k in man if not k.startswith("_")]
-    assert len(names) == 15
+    assert len(names) == 18
     for k in names:
         assert os.path.exists(os.path.join(GOLD, man[k]["file"]))
 
@@ -124,6 +124,41 @@ def test_steps_match_reference(name):
     for m in range(len(dims)):
         for k in STEP_OUTPUTS:
             assert np.array_equal(states[m][k], g["out_%s%d" % (k, m)]), (m, k)
+
+
+RUN100 = ["steps100_flat_plate", "steps100_vortex", "steps100_tgv"]
+
+
+def run100_setup(g):
+    """Oracle states and config of a 100-step fixture (make_golden.case_run)."""
+    solver, turb, itime, tadv, nsteps, des = [int(x) for x in g["flags"]]
+    ma, re, tinf, cfl, dt, aoa = g["params"]
+    subs = blocks_from(g)
+    dims = g["dims"]
+    cfg = O.make_config(ma=ma, re=re, tinf=tinf, cfl=cfl, aoa_deg=aoa, solver=solver, itime=itime,
+                        turb=turb, des=des, tadv=tadv, totalnodes=int(sum(int(np.prod(d)) for d in dims)))
+    states = []
+    for m in range(len(dims)):
+        st = O.BlockState(ext_of(g["ext%d" % m]), subs[m])
+        for k in g:
+            if k.startswith("in_") and k.endswith(str(m)) and k[3:-len(str(m))] in O.FIELDS:
+                st[k[3:-len(str(m))]] = g[k]
+        states.append(st)
+    return states, cfg, nsteps
+
+
+@pytest.mark.parametrize("name", RUN100)
+def test_100_steps_match_reference(name):
+    """100 whole steps of the reduced flat plate, vortex and Taylor-Green
+    cases: the oracle equals the reference bit for bit."""
+    g = load(name)
+    states, cfg, nsteps = run100_setup(g)
+    assert nsteps == 100
+    for _ in range(nsteps):
+        O.step(states, cfg)
+    for m, st in enumerate(states):
+        for k in ("q", "nut", "pri_v"):
+            assert np.array_equal(st[k], g["out_%s%d" % (k, m)]), (name, m, k)
 
 
 def same_where_defined(a, ref):

@@ -357,6 +357,45 @@ def test_steps(kind, nsteps):
         sol.close()
 
 
+# eight blocks of a 2 x 2 x 2 cut, seven of them rotated (right-handed axis
+# relabellings with reversed axes): their interfaces carry every
+# orientation code of sub_bc_updatebuffer.f90:358-395 (4 to 8 of each)
+ROTATED_PERM = [(0, 1, 2), (0, 2, 1), (2, 0, 1), (0, 1, 2), (1, 0, 2), (0, 1, 2), (1, 2, 0), (1, 2, 0)]
+ROTATED_FLIP = [(False, False, False), (False, False, True), (True, False, True), (True, False, True),
+                (False, False, True), (True, True, False), (True, True, False), (True, False, True)]
+
+
+@pytest.mark.parametrize("kind,variant", [("vortex", 0), ("vortex", 2), ("tgv", 1), ("tgv", 2)])
+def test_steps_rotated_blocks(kind, variant):
+    """A box cut into eight blocks whose local axes are rotated and
+    reversed against each other, so that the halo exchange and the face
+    average run every orientation code (1 flip m, 2 transpose, 3 flip l, 4
+    transpose and flip both): 10 whole steps against the oracle to 1e-10
+    (lusgs_kernel 0 hyperplanes / 1 tiles per block / 2 batched tiles with
+    batched stages)."""
+    say("test_steps_rotated_blocks", kind, "variant", variant)
+    cuts = ((6,), (6,), (6,))
+    if kind == "vortex":
+        case = cases.vortex_box(12, blocks=cuts, permute=ROTATED_PERM, flip=ROTATED_FLIP)
+    else:
+        case = cases.tgv_box(12, blocks=cuts, permute=ROTATED_PERM, flip=ROTATED_FLIP)
+    codes = sorted({s[10] for ss in case.subfaces for s in ss if s[8] > 0})
+    assert codes == [1, 2, 3, 4]
+    case.cfg.lusgs_kernel = variant
+    case.cfg.batch_blocks = 1 if variant == 2 else 0
+    sol, states, res_o, res_g = run_steps(case, 10)
+    try:
+        worst = 0.0
+        for m, st in enumerate(states):
+            e = rel_l2(sol.download(m + 1, "q"), st["q"])
+            worst = max(worst, float(np.max(e)))
+            assert np.all(e <= TOLN), "block %d q after 10 steps: %s" % (m + 1, e)
+        say("  worst q rel err %.3e" % worst)
+        assert np.allclose(res_g[:5], res_o[-1][:5], rtol=1e-9, atol=1e-300)
+    finally:
+        sol.close()
+
+
 @pytest.mark.parametrize("variant,fused", [(1, None), (1, "0"), (2, None), (2, "1")])
 def test_lusgs_variants_identical(variant, fused):
     """The pipelined tile sweep computes each node from the same operands as

@@ -449,16 +449,23 @@ def test_noslip_wall():
 # -------------------------------------------------------------------- halo
 # right-handed relabellings of the second block's axes (odd permutations
 # would mirror the grid)
-PERMS = [p for p in itertools.permutations((0, 1, 2))
-         if sum(p[a] > p[b] for a in range(3) for b in range(a + 1, 3)) % 2 == 0]
+# with reversed axes: every right-handed relabelling (24), which between
+# them give interfaces of all four orientation codes
+MAPS = [(p, f) for p in itertools.permutations((0, 1, 2)) for f in itertools.product((False, True), repeat=3)
+        if (sum(p[a] > p[b] for a in range(3) for b in range(a + 1, 3)) + sum(f)) % 2 == 0]
 
 
-@pytest.mark.parametrize("perm", PERMS)
-def test_halo_delivers_true_coordinates(perm):
-    """Two blocks of a split uniform grid, the second with permuted local
-    axes: after the exchange every halo node holds the coordinates of the
-    node it stands for, and averaging the shared face leaves them alone."""
-    dims, subs, origins = cases.split_box(9, 7, 7, ((5,), (), ()), permute=[(0, 1, 2), perm])
+@pytest.mark.parametrize("axis", [0, 1, 2])
+@pytest.mark.parametrize("perm,flip", MAPS)
+def test_halo_delivers_true_coordinates(perm, flip, axis):
+    """Two blocks of a split uniform grid (cut along `axis`), the second
+    with rotated / reversed local axes: after the exchange every halo node
+    holds the coordinates of the node it stands for, and averaging the
+    shared face leaves them alone."""
+    cut = [(), (), ()]
+    cut[axis] = ((5,), (4,), (4,))[axis]
+    dims, subs, origins = cases.split_box(9, 7, 7, tuple(cut), permute=[(0, 1, 2), perm],
+                                          flip=[(False, False, False), flip])
     cfg = O.make_config(solver=1)
     states = []
     truth = []
@@ -466,8 +473,8 @@ def test_halo_delivers_true_coordinates(perm):
     for m in range(2):
         rng_m = cases.ranges_of(dims[m], subs[m])
         st = O.BlockState(O.Ext.from_list(rng_m), subs[m])
-        org, pm = origins[m]
-        xyz = cases.node_coords(rng_m, org, h, (0.0, 0.0, 0.0), pm)
+        org, pm, sg = origins[m]
+        xyz = cases.node_coords(rng_m, org, h, (0.0, 0.0, 0.0), pm, sg)
         t = np.ones(st["q"].shape)
         t[..., 0], t[..., 1], t[..., 2] = xyz
         truth.append(t)
