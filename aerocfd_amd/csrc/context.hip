@@ -44,6 +44,30 @@ int fail(int code, const std::string &msg)
             return fail(ACFD_EDEVICE, std::string(#expr ": ") + hipGetErrorString((hipError_t)r__)); \
     } while (0)
 
+// Flow constants of a configuration: the non-dimensional freestream of
+// sub_init_flowfield.f90:30-39 and the flags.
+Consts consts_of(const acfd_config &cfg)
+{
+    const double gamma = 1.4, pi = 3.1415926535897932;
+    Consts c;
+    c.gamma = gamma;
+    c.d0 = 1.0;
+    c.u0 = 1.0;
+    c.p0 = 1.0 * 1.0 / gamma / cfg.ma / cfg.ma;
+    c.c0 = std::sqrt(1.0) / cfg.ma;
+    c.cv = 1.0 / gamma / (gamma - 1.0) / cfg.ma / cfg.ma;
+    c.re = cfg.re;
+    c.tinf = cfg.tinf;
+    c.prl = cfg.prl > 0 ? cfg.prl : 0.72;
+    c.prt = cfg.prt > 0 ? cfg.prt : 0.90;
+    c.aoa = cfg.aoa_deg * pi / 180.0;
+    c.solver = cfg.iflag_solver;
+    c.itime = cfg.iflag_time;
+    c.turb = cfg.iflag_turbulence;
+    c.des = cfg.iflag_des;
+    return c;
+}
+
 const int kNcomp[28] = {0, 5, 7, 5, 5, 5, 9, 6, 1, 3, 1, 9, 1, 3, 3, 3, 1, 1, 1, 1, 10, 1, 1, 1, 1, 1, 1, 1};
 constexpr int kLastField = 27;
 
@@ -131,6 +155,11 @@ struct acfd_ctx {
     // step that did so replays only while it holds.
     bool pri_current = false;
     bool graph_needs_pri = false;
+    // rhsi is zero outside every block's computed range (the arena starts
+    // zeroed and the inviscid kernels store only inside it): the batched
+    // inviscid stage then skips clearing it.  An upload of rhsi clears the
+    // flag (and drops a captured step that relied on it).
+    bool rhsi_clean = true;
     int *lu_ctr = nullptr;   // ticket counters of the batched sweeps
     // SA's update runs beside the flow's LU-SGS update on a second stream
     // (batched sweeps, SA): its own exchange slots and ticket counters
@@ -139,6 +168,10 @@ struct acfd_ctx {
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     double *sa_ex = nullptr;
     int *sa_ctr = nullptr;
+    // batched SA update (lusgs_kernel 2): the SA sweep layout and the
+    // linear-tile grid of its two per-node passes
+    double *sa_sl = nullptr;
+    BatchGrid bg_sa;
     double *d_cfl_blk = nullptr;  // cfl after each block's find_dt
     // batch_blocks: block table and the per-kind grids of the batched launches
     BlockRec *d_blocks = nullptr;
@@ -589,11 +622,27 @@ int alloc_lu_ws(acfd_ctx *ctx)
             HIP_TRY(hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming));
             HIP_TRY(hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming));
         }
+        if (sa) {
+            // the SA sweep layout, sized for the largest tile shape, and
+            // the grid of the SA model's per-node passes
+            long sa_sl_n = 0;
+            for (int sh = 0; sh < lusgs_tile_shapes(); sh++) {
+                long a = 0;
+                sa_batch_sl(nb, es.data(), sh, &a);
+                sa_sl_n = std::max(sa_sl_n, a);
+            }
+            if (sa_sl_n > 0) HIP_TRY(hipMalloc((void **)&ctx->sa_sl, (size_t)sa_sl_n * sizeof(double)));
+            std::vector<dim3> g;
+            for (auto &b : ctx->blk) g.push_back(lin_grid(b.e));
+            int rc = ensure_block_table(ctx);
+            if (!rc) rc = make_grid(ctx->bg_sa, g);
+            if (rc) return rc;
+        }
         for (int sh = 0; sh < lusgs_tile_shapes(); sh++) {
             if (lusgs_batch_make(nb, es.data(), fs.data(), ctx->arena, sh, ctx->lu.sl, ctx->lu.ex, &ctx->lub[sh]))
                 return fail(ACFD_EDEVICE, "batched LU-SGS tables: device allocation failed");
             if (sa && sa_batch_make(nb, es.data(), fs.data(), ctx->arena, sh, ctx->sa_concurrent ? ctx->sa_ex : ctx->lu.ex,
-                                    &ctx->sab[sh]))
+                                    ctx->sa_sl, &ctx->sab[sh]))
                 return fail(ACFD_EDEVICE, "batched SA sweep tables: device allocation failed");
             if (ctx->sa_concurrent) {
                 // two persistent launches share the device: tiles by ticket
@@ -761,7 +810,34 @@ constexpr int kTimeLuForward = 12, kTimeLuBackward = 13, kTimeLuGather = 14, kTi
 // single-kernel timers of the batched stages (recorded only with timing on,
 // when each stage's kernels are launched one by one)
 constexpr int kTimeSaForward = 17, kTimeSaBackward = 18, kTimeSaPre = 19, kTimeSaAdd = 20, kTimeInvZero = 21,
-              kTimeInvI = 22, kTimeInvJ = 23, kTimeInvK = 24, kTimeViscNode = 25, kTimeViscDiff = 26;
+              kTimeInvI = 22, kTimeInvJ = 23, kTimeInvK = 24, kTimeViscNode = 25, kTimeViscDiff = 26,
+              kTimeSaGather = 27, kTimeSaPreB = 28, kTimeViscNodeDt = 29;
+
+// With batched stages, NS and the steady local time step, find_dt runs in
+// the viscous node pass (ACFD_VISCOUS_DT_FUSED=0: its own pass; the same
+// results either way).
+bool viscous_dt_fused_on()
+{
+    static int v = -1;
+    if (v < 0) {
+        const char *e = std::getenv("ACFD_VISCOUS_DT_FUSED");
+        v = e && std::atoi(e) == 0 ? 0 : 1;
+    }
+    return v == 1;
+}
+
+// The batched SA update moves its sweep operands into a sweep layout
+// (ACFD_SA_LAYOUT=0: the sweeps read them in place, as the per-block sweeps
+// do; the same results either way).
+bool sa_layout_on()
+{
+    static int v = -1;
+    if (v < 0) {
+        const char *e = std::getenv("ACFD_SA_LAYOUT");
+        v = e && std::atoi(e) == 0 ? 0 : 1;
+    }
+    return v == 1;
+}
 
 // Autotuner of the tile sweep: on a block's first update, time the gather
 // and forward sweep once per tile shape (they recompute dq* in the sweep
@@ -975,12 +1051,38 @@ int st_sa_all(acfd_ctx *ctx, F cfl_of)
         return 0;
     }
     StageTimer t(ctx, ACFD_STAGE_SA);
+    int *ctr = ctx->sa_ctr ? ctx->sa_ctr : ctx->lu_ctr;
+    if (ctx->sa_sl && ctx->bg_sa.total > 0 && sa_layout_on()) {
+        // two per-node passes over all blocks, the operands into the SA
+        // sweep layout, the two sweeps over it, nut += dnut
+        const LuBatch &sb = ctx->sab[shape];
+        const double *c0 = cfl_of(0);
+        const int stride = nb > 1 && cfl_of(1) != c0 ? 1 : 0;
+        for (int part = ctx->timing ? 1 : 0; part <= (ctx->timing ? 2 : 0); part++) {
+            StageTimer ts(ctx, part == 2 ? kTimeSaPreB : kTimeSaPre);
+            KRN_TRY(launch_sa_pre_batch(ctx->stream, ctx->d_blocks, ctx->bg_sa, ctx->arena, ctx->c, c0, stride, part));
+        }
+        {
+            StageTimer ts(ctx, kTimeSaGather);
+            KRN_TRY(launch_sa_sl_batch(ctx->stream, sb, ctx->lu.err, ctr, 0));
+        }
+        {
+            StageTimer ts(ctx, kTimeSaForward);
+            KRN_TRY(launch_sa_sl_batch(ctx->stream, sb, ctx->lu.err, ctr, 1));
+        }
+        {
+            StageTimer ts(ctx, kTimeSaBackward);
+            KRN_TRY(launch_sa_sl_batch(ctx->stream, sb, ctx->lu.err, ctr, -1));
+        }
+        StageTimer ts(ctx, kTimeSaAdd);
+        KRN_TRY(launch_sa_sl_batch(ctx->stream, sb, ctx->lu.err, ctr, 2));
+        return 0;
+    }
     {
         StageTimer ts(ctx, kTimeSaPre);
         for (int m = 0; m < nb; m++)
             KRN_TRY(launch_sa_pre(ctx->stream, ctx->blk[m].e, ctx->blk[m].f, ctx->c, cfl_of(m)));
     }
-    int *ctr = ctx->sa_ctr ? ctx->sa_ctr : ctx->lu_ctr;
     if (ctx->timing) {
         {
             StageTimer ts(ctx, kTimeSaForward);
@@ -1093,15 +1195,19 @@ int st_inviscid_all(acfd_ctx *ctx)
         return 0;
     }
     StageTimer t(ctx, ACFD_STAGE_INVISCID);
+    // rhsi outside the computed ranges needs clearing only after an upload
+    const int first = ctx->rhsi_clean ? 1 : 0;
+    ctx->rhsi_clean = true;
     if (ctx->timing) {
-        for (int part = 0; part < 4; part++) {
+        for (int part = first; part < 4; part++) {
             StageTimer ts(ctx, kTimeInvZero + part);
             KRN_TRY(launch_inviscid_batch(ctx->stream, ctx->d_blocks, ctx->bg_inv, ctx->arena, ctx->c,
                                           ctx->cfg.precision, part));
         }
         return 0;
     }
-    KRN_TRY(launch_inviscid_batch(ctx->stream, ctx->d_blocks, ctx->bg_inv, ctx->arena, ctx->c, ctx->cfg.precision));
+    KRN_TRY(launch_inviscid_batch(ctx->stream, ctx->d_blocks, ctx->bg_inv, ctx->arena, ctx->c, ctx->cfg.precision,
+                                  first ? -2 : -1));
     return 0;
 }
 
@@ -1144,10 +1250,33 @@ int st_find_dt_all(acfd_ctx *ctx)
     return 0;
 }
 
+// viscous_flux and the steady find_dt of every block, the find_dt folded
+// into the viscous node pass (the unsteady find_dt evaluates one node per
+// block, nothing to fold)
+int st_viscous_dt_all(acfd_ctx *ctx, int store_aux)
+{
+    StageTimer t(ctx, ACFD_STAGE_VISCOUS);
+    KRN_TRY(launch_reset_stats(ctx->stream, ctx->d_stats));
+    const int nb = (int)ctx->blk.size();
+    if (ctx->timing) {
+        for (int part = 1; part <= 2; part++) {
+            StageTimer ts(ctx, part == 1 ? kTimeViscNodeDt : kTimeViscDiff);
+            KRN_TRY(launch_viscous_dt_batch(ctx->stream, ctx->d_blocks, nb, ctx->bg_visc, ctx->arena, ctx->c,
+                                            ctx->cfg.precision, store_aux, ctx->d_cfl, ctx->d_stats, part));
+        }
+    } else {
+        KRN_TRY(launch_viscous_dt_batch(ctx->stream, ctx->d_blocks, nb, ctx->bg_visc, ctx->arena, ctx->c,
+                                        ctx->cfg.precision, store_aux, ctx->d_cfl, ctx->d_stats));
+    }
+    return 0;
+}
+
 int residual_stage_all(acfd_ctx *ctx, int store_aux, bool pri_ok = false)
 {
     int rc = pri_ok ? 0 : st_getpri_all(ctx);
     if (!rc) rc = st_inviscid_all(ctx);
+    if (!rc && ctx->c.solver == 0 && ctx->c.itime == 0 && batched(ctx) && viscous_dt_fused_on())
+        return st_viscous_dt_all(ctx, store_aux);
     if (!rc && ctx->c.solver == 0) rc = st_viscous_all(ctx, store_aux);
     if (!rc) rc = st_find_dt_all(ctx);
     return rc;
@@ -1357,24 +1486,7 @@ int acfd_create(const acfd_config *cfg, acfd_ctx **out)
         return fail(ACFD_EDEVICE, "acfd_create: no HIP device");
     acfd_ctx *ctx = new acfd_ctx();
     ctx->cfg = *cfg;
-    // non-dimensional freestream, sub_init_flowfield.f90:30-39
-    const double gamma = 1.4, pi = 3.1415926535897932;
-    Consts &c = ctx->c;
-    c.gamma = gamma;
-    c.d0 = 1.0;
-    c.u0 = 1.0;
-    c.p0 = 1.0 * 1.0 / gamma / cfg->ma / cfg->ma;
-    c.c0 = std::sqrt(1.0) / cfg->ma;
-    c.cv = 1.0 / gamma / (gamma - 1.0) / cfg->ma / cfg->ma;
-    c.re = cfg->re;
-    c.tinf = cfg->tinf;
-    c.prl = cfg->prl > 0 ? cfg->prl : 0.72;
-    c.prt = cfg->prt > 0 ? cfg->prt : 0.90;
-    c.aoa = cfg->aoa_deg * pi / 180.0;
-    c.solver = cfg->iflag_solver;
-    c.itime = cfg->iflag_time;
-    c.turb = cfg->iflag_turbulence;
-    c.des = cfg->iflag_des;
+    ctx->c = consts_of(*cfg);
     auto bad = [&](const std::string &m) {
         acfd_destroy(ctx);
         return fail(ACFD_EDEVICE, m);
@@ -1434,6 +1546,8 @@ int acfd_destroy(acfd_ctx *ctx)
     if (ctx->ev_join) (void)hipEventDestroy(ctx->ev_join);
     if (ctx->sa_ctr) (void)hipFree(ctx->sa_ctr);
     if (ctx->sa_ex) uc_release(ctx->sa_ex);
+    if (ctx->sa_sl) (void)hipFree(ctx->sa_sl);
+    free_grid(ctx->bg_sa);
     if (ctx->stream2) (void)hipStreamDestroy(ctx->stream2);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -1627,6 +1741,10 @@ int acfd_upload(acfd_ctx *ctx, int m, int field, const double *host)
     double *dst = field_ptr(b, field);
     if (!dst) return fail(ACFD_EINVAL, "acfd_upload: field not allocated for this configuration");
     if (field == ACFD_DXIDX || field == ACFD_INV_J) sl_invalidate(ctx);
+    if (field == ACFD_RHSI) {
+        ctx->rhsi_clean = false;
+        drop_graph(ctx);
+    }
     int nc = kNcomp[field];
     long tot = b.e.n * nc;
     int rc = ensure_stage(ctx, tot);
@@ -2202,6 +2320,18 @@ int acfd_sensor_infer(const float *w, const float *X, long n, float *prob)
 namespace {
 
 // single-block context with explicit ranges and no interfaces
+// The Ext of one block from its 18 ranges (is..ke, is0..ke0, is1..ke1).
+Ext ext_of_ranges(const int *r)
+{
+    Ext e;
+    e.is = r[0]; e.ie = r[1]; e.js = r[2]; e.je = r[3]; e.ks = r[4]; e.ke = r[5];
+    e.is0 = r[6]; e.ie0 = r[7]; e.js0 = r[8]; e.je0 = r[9]; e.ks0 = r[10]; e.ke0 = r[11];
+    e.is1 = r[12]; e.ie1 = r[13]; e.js1 = r[14]; e.je1 = r[15]; e.ks1 = r[16]; e.ke1 = r[17];
+    e.ni = e.ie - e.is + 1; e.nj = e.je - e.js + 1; e.nk = e.ke - e.ks + 1;
+    e.n = (long)e.ni * e.nj * e.nk;
+    return e;
+}
+
 int make_single(const acfd_config *cfg, const int *r, acfd_ctx **out)
 {
     acfd_config c = cfg ? *cfg : acfd_config{};
@@ -2214,11 +2344,7 @@ int make_single(const acfd_config *cfg, const int *r, acfd_ctx **out)
     acfd_ctx *ctx = *out;
     ctx->blk.resize(1);
     Ext &e = ctx->blk[0].e;
-    e.is = r[0]; e.ie = r[1]; e.js = r[2]; e.je = r[3]; e.ks = r[4]; e.ke = r[5];
-    e.is0 = r[6]; e.ie0 = r[7]; e.js0 = r[8]; e.je0 = r[9]; e.ks0 = r[10]; e.ke0 = r[11];
-    e.is1 = r[12]; e.ie1 = r[13]; e.js1 = r[14]; e.je1 = r[15]; e.ks1 = r[16]; e.ke1 = r[17];
-    e.ni = e.ie - e.is + 1; e.nj = e.je - e.js + 1; e.nk = e.ke - e.ks + 1;
-    e.n = (long)e.ni * e.nj * e.nk;
+    e = ext_of_ranges(r);
     if (e.ni < 3 || e.nj < 3 || e.nk < 3) {
         acfd_destroy(ctx);
         return fail(ACFD_EINVAL, "ranges: block too small");

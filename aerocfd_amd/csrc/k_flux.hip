@@ -326,12 +326,13 @@ static void inviscid_batch_dirs(hipStream_t s, const BlockRec *blks, const Batch
 int launch_inviscid_batch(hipStream_t s, const BlockRec *blks, const BatchGrid *bg, double *arena,
                           const Consts &c, int precision, int part)
 {
-    if (bg[0].total > 0 && (part < 0 || part == 0))
+    if (bg[0].total > 0 && (part == -1 || part == 0))
         hipLaunchKernelGGL(k_zero_rhsi_batch, dim3((unsigned)bg[0].total), dim3(256), 0, s, blks, bg[0].map,
                            bg[0].kg, arena);
     if (bg[1].total <= 0 || part == 0) return 0;
-    if (precision == 1) inviscid_batch_dirs<float>(s, blks, bg, arena, c.gamma, part);
-    else inviscid_batch_dirs<double>(s, blks, bg, arena, c.gamma, part);
+    const int dirs = part < 0 ? -1 : part;
+    if (precision == 1) inviscid_batch_dirs<float>(s, blks, bg, arena, c.gamma, dirs);
+    else inviscid_batch_dirs<double>(s, blks, bg, arena, c.gamma, dirs);
     ACFD_CHECK_LAUNCH();
     return 0;
 }
@@ -562,8 +563,18 @@ int launch_viscous_batch(hipStream_t s, const BlockRec *blks, const BatchGrid &b
 // block (min/max folded into stats[0..1] as ordered bit patterns of the
 // positive values).  Unsteady: the local CFL; the reference keeps the value
 // of the last node of the block in the global cfl (:116), written to *cfl.
+__device__ inline double local_rate_a3(const Ext &e, long n, const double *pri, double a3, const double *dxidx,
+                                       double re);
+
 __device__ inline double local_rate(const Ext &e, long n, const double *pri, const double *amu,
                                     const double *dxidx, double re)
+{
+    return local_rate_a3(e, n, pri, amu[2 * e.n + n], dxidx, re);
+}
+
+// the rate with the total viscosity a3 of the node given
+__device__ inline double local_rate_a3(const Ext &e, long n, const double *pri, double a3, const double *dxidx,
+                                       double re)
 {
     const long N = e.n;
     double m[9];
@@ -576,12 +587,14 @@ __device__ inline double local_rate(const Ext &e, long n, const double *pri, con
     double uu = (m[0] * u0 + m[1] * v0 + m[2] * w0) / txi;
     double vv = (m[3] * u0 + m[4] * v0 + m[5] * w0) / teta;
     double ww = (m[6] * u0 + m[7] * v0 + m[8] * w0) / tzeta;
-    double a3 = amu[2 * N + n];
     double t1 = fabs(uu) + c0 + 2.0 * txi * a3 * 4.0 / 3.0 / re / d0;
     double t2 = fabs(vv) + c0 + 2.0 * teta * a3 * 4.0 / 3.0 / re / d0;
     double t3 = fabs(ww) + c0 + 2.0 * tzeta * a3 * 4.0 / 3.0 / re / d0;
     return txi * t1 + teta * t2 + tzeta * t3;
 }
+
+// block-wide min/max of finite positive dt values via ordered bit patterns
+__device__ inline void dt_stats(bool ok, double v, unsigned long long *stats);
 
 __device__ inline void find_dt_steady_body(const Ext &e, int bx, const double *__restrict__ pri,
                                            const double *__restrict__ amu, const double *__restrict__ dxidx,
@@ -595,7 +608,11 @@ __device__ inline void find_dt_steady_body(const Ext &e, int bx, const double *_
         v = *cfl / local_rate(e, n, pri, amu, dxidx, re);
         dt[n] = v;
     }
-    // block-wide min/max of finite positive values via ordered bit patterns
+    dt_stats(ok, v, stats);
+}
+
+__device__ inline void dt_stats(bool ok, double v, unsigned long long *stats)
+{
     unsigned long long bits = ok && v > 0.0 && isfinite(v) ? (unsigned long long)__double_as_longlong(v) : 0ull;
     unsigned long long mx = bits, mn = bits ? bits : ~0ull;
     for (int o = 32; o > 0; o >>= 1) {
@@ -662,6 +679,72 @@ __global__ void k_reset_stats(unsigned long long *st)
 {
     st[0] = 0ull;
     st[1] = ~0ull;
+}
+
+// viscous_flux's node stage and the steady find_dt in one pass over every
+// block's storage: a node of is0..ie0 evaluates its viscous node fluxes
+// (k_viscous_node_batch), then dt from the total viscosity it has just
+// formed; every other node of the extended block only dt (k_find_dt_steady
+// reads the stored viscosity there).  The same expressions as the two
+// kernels; the dt statistics from the last block.
+template <typename R>
+__global__ void __launch_bounds__(256) k_viscous_node_dt_batch(const BlockRec *__restrict__ blks,
+                                                               const int *__restrict__ map,
+                                                               const KGrid *__restrict__ kg, double *arena,
+                                                               Consts c, int store_aux, int nblk, const double *cfl,
+                                                               unsigned long long *stats)
+{
+    int bx, by, bz;
+    const int m = batch_wg(map, kg, bx, by, bz);
+    const BlockRec &b = blks[m];
+    const Ext e = b.e;
+    const long N = e.n;
+    const long n = (long)bx * 256 + (long)(threadIdx.x + blockDim.x * threadIdx.y);
+    const bool ok = n < N;
+    double v = 0.0;
+    if (ok) {
+        const unsigned plane = (unsigned)e.ni * (unsigned)e.nj, un = (unsigned)n;
+        const unsigned kk = un / plane, r = un - kk * plane, jj = r / (unsigned)e.ni;
+        const int i = e.is + (int)(r - jj * (unsigned)e.ni), j = e.js + (int)jj, k = e.ks + (int)kk;
+        const double *pri = arena + b.o.pri, *dxidx = arena + b.o.dxidx;
+        double *amu = arena + b.o.amu;
+        double a3;
+        if (i >= e.is0 && i <= e.ie0 && j >= e.js0 && j <= e.je0 && k >= e.ks0 && k <= e.ke0) {
+            double w12[12];
+            R am1, am3, du[9], ta[6];
+            viscous_node_eval<R>(e, i, j, k, c, pri, amu, dxidx, arena + b.o.inv_j, w12, am1, am3, du, ta);
+            viscous_node_store<R>(N, n, am1, am3, du, ta, amu, arena + b.o.vor, arena + b.o.dudx, arena + b.o.tao,
+                                  store_aux);
+            double *w = arena + b.o.w_flux;
+#pragma unroll
+            for (int x = 0; x < 12; x++) w[x * N + n] = w12[x];
+            a3 = (double)am3;
+        } else {
+            a3 = amu[2 * N + n];
+        }
+        v = *cfl / local_rate_a3(e, n, pri, a3, dxidx, c.re);
+        arena[b.o.dt + n] = v;
+    }
+    dt_stats(ok, v, m == nblk - 1 ? stats : nullptr);
+}
+
+int launch_viscous_dt_batch(hipStream_t s, const BlockRec *blks, int nblk, const BatchGrid &bg, double *arena,
+                            const Consts &c, int precision, int store_aux, const double *cfl,
+                            unsigned long long *stats, int part)
+{
+    if (bg.total <= 0) return 0;
+    const dim3 g((unsigned)bg.total), t(64, 4, 1);
+    if (part < 0 || part == 1) {
+        if (precision == 1)
+            hipLaunchKernelGGL((k_viscous_node_dt_batch<float>), g, t, 0, s, blks, bg.map, bg.kg, arena, c,
+                               store_aux, nblk, cfl, stats);
+        else
+            hipLaunchKernelGGL((k_viscous_node_dt_batch<double>), g, t, 0, s, blks, bg.map, bg.kg, arena, c,
+                               store_aux, nblk, cfl, stats);
+    }
+    if (part < 0 || part == 2) hipLaunchKernelGGL(k_viscous_diff_batch, g, t, 0, s, blks, bg.map, bg.kg, arena, c.re);
+    ACFD_CHECK_LAUNCH();
+    return 0;
 }
 
 int launch_reset_stats(hipStream_t s, unsigned long long *stats)

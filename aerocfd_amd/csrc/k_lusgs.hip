@@ -987,6 +987,8 @@ struct SaBatchEnt {
     Ext e;
     SaGeom g;
     long cu, rhs, dns, dn, ex;
+    long sl;  // the block's part of the SA sweep layout (offset from its base)
+    long nut;
     int tile0;
 };
 
@@ -1010,6 +1012,220 @@ __global__ void __launch_bounds__(TJ *TK) k_sa_tiles_batch(const SaBatchEnt *__r
     const int local = flat - b.tile0;
     sa_tiles_body<DIR, TJ, TK>(e, g, DIR > 0 ? local : g.ntiles - 1 - local, arena + b.cu, arena + b.rhs,
                                arena + b.dns, arena + b.dn, exb + b.ex, err, pj, pk);
+    if (!resident && threadIdx.x == 0 &&
+        __hip_atomic_fetch_add(ctr + 1, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == total - 1) {
+        __hip_atomic_store(ctr, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(ctr + 1, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+// ---- scalar sweeps over an SA sweep layout (every block in one launch)
+// The operands a, b, c, diag and the right-hand side of every computed node
+// are moved into a sweep layout ordered (tile, step, operand, lane): at
+// step t, lane l of a tile reads its five operands as five rows of TT
+// consecutive doubles, whole lines for the wavefront, where the natural
+// layout has each lane on a line of its own.  Row t of lane (tj, tk) holds
+// the node li = t - tj - tk of the lane's i-line; the backward sweep's step
+// s reads row nsteps-1-s for every lane (its skewed clock runs the same
+// rows in reverse).  The forward sweep leaves dnut* in the rhs operand's
+// slot and the backward sweep dnut there; a last pass adds it to nut.
+constexpr int kSaOps = 5;
+
+__host__ __device__ inline long sa_slot(const SaGeom &g, int tt, int tile, int t, int f, int l)
+{
+    return (((long)tile * g.nsteps + t) * kSaOps + f) * tt + l;
+}
+
+// node of (tile, line, li) of a block and whether the line exists
+__device__ inline long sa_nat(const Ext &e, const SaGeom &g, int tile, int line, int li)
+{
+    const int tj = line % g.TJ, tk = line / g.TJ;
+    return e.ix(e.is1 + li, e.js1 + (tile % g.nbj) * g.TJ + tj, e.ks1 + (tile / g.nbj) * g.TK + tk);
+}
+__device__ inline bool sa_line_ok(const SaGeom &g, int tile, int line)
+{
+    return (tile % g.nbj) * g.TJ + line % g.TJ < g.Lj && (tile / g.nbj) * g.TK + line / g.TJ < g.Lk;
+}
+
+// Natural layout <-> SA sweep layout for one tile (one workgroup of TT
+// threads), the walk of the flow sweep's layout passes: chunks of CT steps,
+// a natural-side thread handling CT consecutive nodes of a line, an
+// XOR-swizzled [CT][TT] LDS buffer in between.  TO_SL: cu (4 operands,
+// component stride N) and rhs into the layout; otherwise nut += the layout's
+// last operand.
+template <int TT, bool TO_SL>
+__device__ inline void sa_layout_tile(const Ext &e, const SaGeom &g, int tile, const double *__restrict__ cu,
+                                      const double *__restrict__ rhs, double *__restrict__ nut, double *__restrict__ L,
+                                      double *buf /* [CT][TT] */)
+{
+    constexpr int CT = sl_ct_of(TT), lstep = TT / CT;
+    const int x = threadIdx.x;
+    const int ii_n = x % CT;
+    const int tj = x % g.TJ, tk = x / g.TJ;
+    const bool me_ok = sa_line_ok(g, tile, x);
+    const long N = e.n;
+    for (int t0 = 0; t0 < g.nsteps; t0 += CT) {
+        int nat[CT];  // node offsets fit 32 bits (blocks below 2^31 nodes)
+#pragma unroll
+        for (int m = 0; m < CT; m++) {
+            const int line = m * lstep + x / CT;
+            const int li = t0 + ii_n - line % g.TJ - line / g.TJ;
+            nat[m] = (t0 + ii_n < g.nsteps && li >= 0 && li < g.Li && sa_line_ok(g, tile, line))
+                         ? (int)sa_nat(e, g, tile, line, li) : -1;
+        }
+        if (TO_SL) {
+            for (int f = 0; f < kSaOps; f++) {
+                const double *src = f < 4 ? cu + f * N : rhs;
+                double v[CT];
+#pragma unroll
+                for (int m = 0; m < CT; m++) v[m] = nat[m] >= 0 ? src[nat[m]] : 0.0;
+#pragma unroll
+                for (int m = 0; m < CT; m++) {
+                    const int line = m * lstep + x / CT;
+                    buf[ii_n * TT + (line ^ ii_n)] = v[m];
+                }
+                __syncthreads();
+                if (me_ok) {
+#pragma unroll
+                    for (int ii = 0; ii < CT; ii++) {
+                        const int t = t0 + ii, li = t - tj - tk;
+                        if (t < g.nsteps && li >= 0 && li < g.Li) L[sa_slot(g, TT, tile, t, f, x)] = buf[ii * TT + (x ^ ii)];
+                    }
+                }
+                __syncthreads();
+            }
+        } else {
+            if (me_ok) {
+#pragma unroll
+                for (int ii = 0; ii < CT; ii++) {
+                    const int t = t0 + ii, li = t - tj - tk;
+                    buf[ii * TT + (x ^ ii)] =
+                        (t < g.nsteps && li >= 0 && li < g.Li) ? L[sa_slot(g, TT, tile, t, kSaOps - 1, x)] : 0.0;
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int m = 0; m < CT; m++) {
+                const int line = m * lstep + x / CT;
+                if (nat[m] >= 0) nut[nat[m]] = nut[nat[m]] + buf[ii_n * TT + (line ^ ii_n)];
+            }
+            __syncthreads();
+        }
+    }
+}
+
+template <int TT, bool TO_SL>
+__global__ void __launch_bounds__(TT) k_sa_layout_batch(const SaBatchEnt *__restrict__ ents,
+                                                        const int *__restrict__ tile_ent, double *arena,
+                                                        double *slbase)
+{
+    __shared__ double buf[sl_ct_of(TT) * TT];
+    const int flat = (int)xcd_slab(blockIdx.x, gridDim.x);
+    const SaBatchEnt &b = ents[tile_ent[flat]];
+    const Ext e = b.e;
+    const SaGeom g = b.g;
+    sa_layout_tile<TT, TO_SL>(e, g, flat - b.tile0, arena + b.cu, arena + b.rhs, arena + b.nut, slbase + b.sl, buf);
+}
+
+// The scalar sweep of sa_tiles_body with its operands in the SA sweep
+// layout (the same expressions, the same exchange and skewed clock).
+template <int DIR, int TJ, int TK>
+__device__ inline void sa_sl_body(const SaGeom &g, int tile, double *__restrict__ L, double *ex, int *err,
+                                  double (*pj)[TJ * TK], double (*pk)[TJ * TK])
+{
+    constexpr int TT = TJ * TK;
+    const int bj = tile % g.nbj, bk = tile / g.nbj;
+    const int l = threadIdx.x, tj = l % TJ, tk = l / TJ;
+    const int lj = bj * TJ + tj, lk = bk * TK + tk;
+    const bool active = lj < g.Lj && lk < g.Lk;
+    const int sj = DIR > 0 ? tj : TJ - 1 - tj, sk = DIR > 0 ? tk : TK - 1 - tk;
+    const bool j_ring = DIR > 0 ? lj == 0 : lj + 1 >= g.Lj;
+    const bool k_ring = DIR > 0 ? lk == 0 : lk + 1 >= g.Lk;
+    const bool j_in = !j_ring && sj == 0, k_in = !k_ring && sk == 0;
+    const int lpj = l - DIR, lpk = l - DIR * TJ;
+    const bool j_out = sj == TJ - 1 && (DIR > 0 ? bj + 1 < g.nbj : bj > 0);
+    const bool k_out = sk == TK - 1 && (DIR > 0 ? bk + 1 < g.nbk : bk > 0);
+    double *exj = ex, *exk = ex + (long)g.ntiles * g.exj_tile;
+    double *jin = exj + (long)(DIR > 0 ? tile - 1 : tile + 1) * g.exj_tile + tk;
+    double *kin = exk + (long)(DIR > 0 ? tile - g.nbj : tile + g.nbj) * g.exk_tile + tj;
+    double *jout = exj + (long)tile * g.exj_tile + tk;
+    double *kout = exk + (long)tile * g.exk_tile + tj;
+    const double *jsrc = j_in ? jin : ex;
+    const double *ksrc = k_in ? kin : ex;
+    const int xj_last = g.Li + TK - 2, xk_last = g.Li + TJ - 2;
+    bool failed = false;
+    double vi = 0.0;
+    auto row_of = [&](int s) -> long {
+        const int sc = min(s, g.nsteps - 1);
+        return ((long)tile * g.nsteps + (DIR > 0 ? sc : g.nsteps - 1 - sc)) * kSaOps * TT + l;
+    };
+    auto fetch = [&](int s, double *op, unsigned long long &xj, unsigned long long &xk) {
+        const double *p = L + row_of(s);
+#pragma unroll
+        for (int f = 0; f < kSaOps; f++) op[f] = p[f * TT];
+        const int p0 = s - sj - sk;
+        const int dj = j_in ? min(max(p0 + sk, 0), xj_last) : 0;
+        const int dk = k_in ? min(max(p0 + sj, 0), xk_last) : 0;
+        xj = xload_bits(jsrc + (long)dj * TK);
+        xk = xload_bits(ksrc + (long)dk * TJ);
+    };
+    auto step = [&](int s, const double *op, unsigned long long xj, unsigned long long xk) {
+        const int par = s & 1;
+        const int pos = s - sj - sk;
+        if (active && pos >= 0 && pos < g.Li) {
+            const double vj = j_ring ? 0.0 : j_in ? ex_take1(jin + (long)(pos + sk) * TK, xj, err, failed)
+                                                  : pj[par ^ 1][lpj];
+            const double vk = k_ring ? 0.0 : k_in ? ex_take1(kin + (long)(pos + sj) * TJ, xk, err, failed)
+                                                  : pk[par ^ 1][lpk];
+            double res;
+            if (DIR > 0) {
+                const double ta = 0.5 * (op[0] + fabs(op[0])) * vi;
+                const double tb = 0.5 * (op[1] + fabs(op[1])) * vj;
+                const double tc = 0.5 * (op[2] + fabs(op[2])) * vk;
+                res = (op[4] + (ta + tb + tc)) / op[3];
+            } else {
+                const double ta = 0.5 * (op[0] - fabs(op[0])) * vi;
+                const double tb = 0.5 * (op[1] - fabs(op[1])) * vj;
+                const double tc = 0.5 * (op[2] - fabs(op[2])) * vk;
+                res = op[4] - (ta + tb + tc) / op[3];
+            }
+            L[row_of(s) + (kSaOps - 1) * TT] = res;
+            vi = res;
+            pj[par][l] = res;
+            pk[par][l] = res;
+            if (j_out) xstore(jout + (long)(pos + sk) * TK, res);
+            if (k_out) xstore(kout + (long)(pos + sj) * TJ, res);
+        }
+        lds_barrier();
+    };
+    double opA[kSaOps], opB[kSaOps];
+    unsigned long long xjA, xkA, xjB, xkB;
+    fetch(0, opA, xjA, xkA);
+    for (int s = 0; s < g.nsteps; s += 2) {
+        fetch(s + 1, opB, xjB, xkB);
+        step(s, opA, xjA, xkA);
+        fetch(s + 2, opA, xjA, xkA);
+        step(s + 1, opB, xjB, xkB);
+    }
+}
+
+template <int DIR, int TJ, int TK>
+__global__ void __launch_bounds__(TJ *TK) k_sa_sl_batch(const SaBatchEnt *__restrict__ ents,
+                                                         const int *__restrict__ tile_ent, int total, int resident,
+                                                         double *slbase, double *exb, int *err, int *ctr)
+{
+    __shared__ double pj[2][TJ * TK], pk[2][TJ * TK];
+    __shared__ int ticket;
+    if (!resident) {
+        if (threadIdx.x == 0)
+            ticket = __hip_atomic_fetch_add(ctr, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __syncthreads();
+    }
+    const int flat = resident ? (int)blockIdx.x : __builtin_amdgcn_readfirstlane(ticket);
+    const SaBatchEnt &b = ents[tile_ent[flat]];
+    const SaGeom g = b.g;
+    const int local = flat - b.tile0;
+    sa_sl_body<DIR, TJ, TK>(g, DIR > 0 ? local : g.ntiles - 1 - local, slbase + b.sl, exb + b.ex, err, pj, pk);
     if (!resident && threadIdx.x == 0 &&
         __hip_atomic_fetch_add(ctr + 1, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == total - 1) {
         __hip_atomic_store(ctr, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -1232,13 +1448,15 @@ int launch_lusgs_batch(hipStream_t s, const LuBatch &b, int phase, double gamma,
 template <int DIR, int TJ, int TK>
 static bool sa_fits_resident(int total)
 {
-    int dev = 0, cus = 0, per_cu = 0;
+    int dev = 0, cus = 0, per_cu = 0, per_cu_sl = 0;
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
         hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_sa_tiles_batch<DIR, TJ, TK>, TJ * TK, 0) !=
+            hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_sl, k_sa_sl_batch<DIR, TJ, TK>, TJ * TK, 0) !=
             hipSuccess)
         return false;
-    return (long)cus * per_cu >= total;
+    return (long)cus * std::min(per_cu, per_cu_sl) >= total;
 }
 
 template <int TJ, int TK>
@@ -1268,11 +1486,23 @@ int launch_sa_sweeps(hipStream_t s, const Ext &e, int shape, const double *cu, c
     return 0;
 }
 
-int sa_batch_make(int nblk, const Ext *e, const Fields *f, double *arena, int shape, double *ex, LuBatch *out)
+void sa_batch_sl(int nblk, const Ext *e, int shape, long *sl_n)
+{
+    long so = 0;
+    for (int m = 0; m < nblk; m++) {
+        const SaGeom g = sa_geom(e[m], shape);
+        if (g.Li <= 0 || g.Lj <= 0 || g.Lk <= 0) continue;
+        so += round_up32((long)g.ntiles * g.nsteps * kSaOps * g.TJ * g.TK);
+    }
+    *sl_n = so;
+}
+
+int sa_batch_make(int nblk, const Ext *e, const Fields *f, double *arena, int shape, double *ex, double *sl,
+                  LuBatch *out)
 {
     std::vector<SaBatchEnt> ents;
     std::vector<int> te;
-    long xo = 0;
+    long xo = 0, so = 0;
     for (int m : batch_order(nblk, e, shape)) {
         const SaGeom g = sa_geom(e[m], shape);
         if (!f[m].w_sa) continue;
@@ -1283,15 +1513,19 @@ int sa_batch_make(int nblk, const Ext *e, const Fields *f, double *arena, int sh
         b.rhs = f[m].sa_rhs - arena;
         b.dns = f[m].w_sa + 8 * e[m].n - arena;
         b.dn = f[m].w_dq - arena;
+        b.nut = f[m].nut - arena;
         b.ex = xo;
+        b.sl = so;
         b.tile0 = (int)te.size();
         xo += round_up32((long)g.ntiles * (g.exj_tile + g.exk_tile));
+        so += round_up32((long)g.ntiles * g.nsteps * kSaOps * g.TJ * g.TK);
         for (int t = 0; t < g.ntiles; t++) te.push_back((int)ents.size());
         ents.push_back(b);
     }
     out->shape = shape;
     out->arena = arena;
     out->ex = ex;
+    out->sl = sl;
     out->nent = (int)ents.size();
     out->total = (int)te.size();
     if (ents.empty()) return 0;
@@ -1324,6 +1558,53 @@ static void sa_batch_t(hipStream_t s, const LuBatch &b, int *err, int *ctr, int 
     if (dir <= 0)
         hipLaunchKernelGGL((k_sa_tiles_batch<-1, TJ, TK>), dim3((unsigned)b.total), dim3(TJ * TK), 0, s, ents,
                            b.tile_ent, b.total, b.resident[1], b.arena, b.ex, err, ctr);
+}
+
+template <int TJ, int TK>
+static void sa_sl_batch_t(hipStream_t s, const LuBatch &b, int *err, int *ctr, int dir)
+{
+    const SaBatchEnt *ents = (const SaBatchEnt *)b.ents;
+    if (dir >= 0)
+        hipLaunchKernelGGL((k_sa_sl_batch<1, TJ, TK>), dim3((unsigned)b.total), dim3(TJ * TK), 0, s, ents,
+                           b.tile_ent, b.total, b.resident[0], b.sl, b.ex, err, ctr);
+    if (dir <= 0)
+        hipLaunchKernelGGL((k_sa_sl_batch<-1, TJ, TK>), dim3((unsigned)b.total), dim3(TJ * TK), 0, s, ents,
+                           b.tile_ent, b.total, b.resident[1], b.sl, b.ex, err, ctr);
+}
+
+template <int TT>
+static void sa_layout_batch_t(hipStream_t s, const LuBatch &b, bool to_sl)
+{
+    const SaBatchEnt *ents = (const SaBatchEnt *)b.ents;
+    if (to_sl)
+        hipLaunchKernelGGL((k_sa_layout_batch<TT, true>), dim3((unsigned)b.total), dim3(TT), 0, s, ents, b.tile_ent,
+                           b.arena, b.sl);
+    else
+        hipLaunchKernelGGL((k_sa_layout_batch<TT, false>), dim3((unsigned)b.total), dim3(TT), 0, s, ents,
+                           b.tile_ent, b.arena, b.sl);
+}
+
+// SA sweeps over the SA sweep layout: phase 0 moves a, b, c, diag and rhs
+// into the layout, 1 / -1 run the forward / backward sweep, 2 adds dnut
+// to nut
+int launch_sa_sl_batch(hipStream_t s, const LuBatch &b, int *err, int *ctr, int phase)
+{
+    if (b.total <= 0 || !b.sl) return 0;
+    const int tt = kTileShapes[b.shape][0] * kTileShapes[b.shape][1];
+    if (phase == 0 || phase == 2) {
+        if (tt == 256) sa_layout_batch_t<256>(s, b, phase == 0);
+        else if (tt == 128) sa_layout_batch_t<128>(s, b, phase == 0);
+        else sa_layout_batch_t<64>(s, b, phase == 0);
+    } else {
+        switch (b.shape) {
+        case 0: sa_sl_batch_t<16, 16>(s, b, err, ctr, phase); break;
+        case 1: sa_sl_batch_t<16, 8>(s, b, err, ctr, phase); break;
+        case 2: sa_sl_batch_t<8, 8>(s, b, err, ctr, phase); break;
+        default: sa_sl_batch_t<16, 4>(s, b, err, ctr, phase); break;
+        }
+    }
+    ACFD_CHECK_LAUNCH();
+    return 0;
 }
 
 // the SA sweeps of every block of the batch (dir 0: both, in order)

@@ -27,19 +27,16 @@ constexpr double cdes = 0.65;
 // beta.grad, gamma.grad; 6 nu + (1+cb2) nut; 7 diasd; 8 dnut*.
 // f.w_dqs holds contravariant velocities (0-2) and diag (3); f.w_dq dnut.
 
-// length (sub_tur_sa.f90:50-100) on the whole extended block
-__global__ void k_sa_length(Ext e, Consts c, const double *__restrict__ dist,
-                            const double *__restrict__ spac, const double *__restrict__ pri,
-                            const double *__restrict__ nut, const double *__restrict__ amu,
-                            const double *__restrict__ dudx, double *__restrict__ length)
+// length (sub_tur_sa.f90:50-100) of node n of the extended block
+__device__ inline double sa_length_at(const Ext &e, const Consts &c, long n, const double *__restrict__ dist,
+                                      const double *__restrict__ spac, const double *__restrict__ pri,
+                                      const double *__restrict__ nut, const double *__restrict__ amu,
+                                      const double *__restrict__ dudx)
 {
-    long n = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= e.n) return;
     const long N = e.n;
     double dist0 = dist[n];
-    if (c.des == 1) {
-        length[n] = fmin(dist0, sa::cdes * spac[n]);
-    } else if (c.des == 2) {
+    if (c.des == 1) return fmin(dist0, sa::cdes * spac[n]);
+    if (c.des == 2) {
         // dudx is declared dudx(3,is:ie,js:je,ks:ke,3) in sa_model
         // (sub_tur_sa.f90:29) while the array holds dudx(9,...): element
         // (a,n,b) of that view is flat element (a-1) + 3n + 3N(b-1) of the
@@ -57,10 +54,19 @@ __global__ void k_sa_length(Ext e, Consts c, const double *__restrict__ dist,
         double rd = (nu0 + nut0) / (temp * c.re * (sa::kar * sa::kar) * (dist0 * dist0));
         double x = 8.0 * rd;
         double fd = 1.0 - tanh(x * x * x);
-        length[n] = dist0 - fd * fmax(0.0, dist0 - sa::cdes * spac0);
-    } else {
-        length[n] = dist0;
+        return dist0 - fd * fmax(0.0, dist0 - sa::cdes * spac0);
     }
+    return dist0;
+}
+
+__global__ void k_sa_length(Ext e, Consts c, const double *__restrict__ dist, const double *__restrict__ spac,
+                            const double *__restrict__ pri, const double *__restrict__ nut,
+                            const double *__restrict__ amu, const double *__restrict__ dudx,
+                            double *__restrict__ length)
+{
+    long n = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= e.n) return;
+    length[n] = sa_length_at(e, c, n, dist, spac, pri, nut, amu, dudx);
 }
 
 // one-sided at the physical ends, central inside; the operand is zero
@@ -78,21 +84,17 @@ __device__ inline double sa_d(const Ext &e, const double *a, long n, long st, in
     return (val(n + st, x + 1) - val(n - st, x - 1)) / 2.0;
 }
 
-// stage A on the computed range: convection (sub_tur_sa.f90:152-208) into
-// sa_rhs, nut gradient (:294-338), alpha/beta/gamma products (:340-356) and
-// nu + (1+cb2) nut (:405-413)
-__global__ void __launch_bounds__(256) k_sa_a(Ext e, Box b, const double *__restrict__ pri,
-                                              const double *__restrict__ dxidx,
-                                              const double *__restrict__ nut,
-                                              const double *__restrict__ amu,
-                                              const double *__restrict__ aal,
-                                              const double *__restrict__ bbe,
-                                              const double *__restrict__ gga,
-                                              double *__restrict__ sa_rhs, double *__restrict__ w)
+// stage A at node (i,j,k) = n of the computed range: convection
+// (sub_tur_sa.f90:152-208) into sa_rhs, nut gradient (:294-338) into w[0..2]
+// when W012, alpha/beta/gamma products (:340-356) into w[3..5] and
+// nu + (1+cb2) nut (:405-413) into w[6]
+template <bool W012>
+__device__ inline void sa_a_at(const Ext &e, int i, int j, int k, long n, const double *__restrict__ pri,
+                               const double *__restrict__ dxidx, const double *__restrict__ nut,
+                               const double *__restrict__ amu, const double *__restrict__ aal,
+                               const double *__restrict__ bbe, const double *__restrict__ gga,
+                               double *__restrict__ sa_rhs, double *__restrict__ w)
 {
-    int i, j, k;
-    long n;
-    if (!lin_node_at(e, b, xcd_slab(blockIdx.x, gridDim.x), i, j, k, n)) return;
     const long N = e.n;
     const long st[3] = {1, (long)e.ni, (long)e.ni * e.nj};
     double u = pri[N + n], v = pri[2 * N + n], ww = pri[3 * N + n];
@@ -108,9 +110,11 @@ __global__ void __launch_bounds__(256) k_sa_a(Ext e, Box b, const double *__rest
     double g0 = sa_d<false>(e, nut, n, st[0], i, e.is0, e.ie0, 0, 0);
     double g1 = sa_d<false>(e, nut, n, st[1], j, e.js0, e.je0, 0, 0);
     double g2 = sa_d<false>(e, nut, n, st[2], k, e.ks0, e.ke0, 0, 0);
-    w[n] = g0;
-    w[N + n] = g1;
-    w[2 * N + n] = g2;
+    if (W012) {
+        w[n] = g0;
+        w[N + n] = g1;
+        w[2 * N + n] = g2;
+    }
     w[3 * N + n] = aal[n] * g0 + aal[N + n] * g1 + aal[2 * N + n] * g2;
     w[4 * N + n] = bbe[n] * g0 + bbe[N + n] * g1 + bbe[2 * N + n] * g2;
     w[5 * N + n] = gga[n] * g0 + gga[N + n] * g1 + gga[2 * N + n] * g2;
@@ -118,20 +122,34 @@ __global__ void __launch_bounds__(256) k_sa_a(Ext e, Box b, const double *__rest
     w[6 * N + n] = nu0 + (1.0 + sa::cb2) * nut[n];
 }
 
-// stage B on the computed range: diffusion (sub_tur_sa.f90:359-512) and
-// source with the diagonal Jacobian (:561-623)
-__global__ void __launch_bounds__(256) k_sa_b(Ext e, Box b, Consts c, const double *__restrict__ pri,
+__global__ void __launch_bounds__(256) k_sa_a(Ext e, Box b, const double *__restrict__ pri,
                                               const double *__restrict__ dxidx,
-                                              const double *__restrict__ inv_j,
                                               const double *__restrict__ nut,
                                               const double *__restrict__ amu,
-                                              const double *__restrict__ vor,
-                                              const double *__restrict__ length,
+                                              const double *__restrict__ aal,
+                                              const double *__restrict__ bbe,
+                                              const double *__restrict__ gga,
                                               double *__restrict__ sa_rhs, double *__restrict__ w)
 {
     int i, j, k;
     long n;
     if (!lin_node_at(e, b, xcd_slab(blockIdx.x, gridDim.x), i, j, k, n)) return;
+    sa_a_at<true>(e, i, j, k, n, pri, dxidx, nut, amu, aal, bbe, gga, sa_rhs, w);
+}
+
+// stage B at node (i,j,k) = n of the computed range: diffusion
+// (sub_tur_sa.f90:359-512) and source with the diagonal Jacobian (:561-623).
+// sa_rhs is completed; the Jacobian term diasd (min(spn + sdn, 0)) is
+// returned.  The nut gradient at n comes from w[0..2] (W012) or is formed
+// again from nut (the same expressions as stage A).
+template <bool W012>
+__device__ inline double sa_b_at(const Ext &e, int i, int j, int k, long n, const Consts &c,
+                                 const double *__restrict__ pri, const double *__restrict__ dxidx,
+                                 const double *__restrict__ inv_j, const double *__restrict__ nut,
+                                 const double *__restrict__ amu, const double *__restrict__ vor,
+                                 const double *__restrict__ length, double *__restrict__ sa_rhs,
+                                 const double *__restrict__ w)
+{
     const long N = e.n;
     const long st[3] = {1, (long)e.ni, (long)e.ni * e.nj};
     const double re = c.re;
@@ -149,7 +167,16 @@ __global__ void __launch_bounds__(256) k_sa_b(Ext e, Box b, Consts c, const doub
     double tz = txi * m[2] + teta * m[5] + tzeta * m[8];
     double nut0 = nut[n];
     double deltanut = (tempa + tempb + tempc) / inv_j[n];
-    double g0 = w[n], g1 = w[N + n], g2 = w[2 * N + n];
+    double g0, g1, g2;
+    if (W012) {
+        g0 = w[n];
+        g1 = w[N + n];
+        g2 = w[2 * N + n];
+    } else {
+        g0 = sa_d<false>(e, nut, n, st[0], i, e.is0, e.ie0, 0, 0);
+        g1 = sa_d<false>(e, nut, n, st[1], j, e.js0, e.je0, 0, 0);
+        g2 = sa_d<false>(e, nut, n, st[2], k, e.ks0, e.ke0, 0, 0);
+    }
     double dnutdx = g0 * m[0] + g1 * m[3] + g2 * m[6];
     double dnutdy = g0 * m[1] + g1 * m[4] + g2 * m[7];
     double dnutdz = g0 * m[2] + g1 * m[5] + g2 * m[8];
@@ -208,10 +235,53 @@ __global__ void __launch_bounds__(256) k_sa_b(Ext e, Box b, Consts c, const doub
     double sdn = (1.0 / re) * (sa::cb1 * (fv2 - ft2 * fv2 + ft2) / kar2 - sa::cw1 * fw) * 2.0 * nut0 / dist2;
     sdn = (1.0 / re) * (sa::cb1 * (fv2n - ft2n * fv2 - ft2 * fv2n + ft2n) / kar2 - sa::cw1 * fwn)
               * ((nut0 * nut0) / dist2) + sdn;
-    w[7 * N + n] = fmin(spn + sdn, 0.0);
+    return fmin(spn + sdn, 0.0);
 }
 
-// diag and contravariant velocities of sa_lusgs (sub_tur_sa.f90:719-753)
+__global__ void __launch_bounds__(256) k_sa_b(Ext e, Box b, Consts c, const double *__restrict__ pri,
+                                              const double *__restrict__ dxidx,
+                                              const double *__restrict__ inv_j,
+                                              const double *__restrict__ nut,
+                                              const double *__restrict__ amu,
+                                              const double *__restrict__ vor,
+                                              const double *__restrict__ length,
+                                              double *__restrict__ sa_rhs, double *__restrict__ w)
+{
+    int i, j, k;
+    long n;
+    if (!lin_node_at(e, b, xcd_slab(blockIdx.x, gridDim.x), i, j, k, n)) return;
+    w[7 * e.n + n] = sa_b_at<true>(e, i, j, k, n, c, pri, dxidx, inv_j, nut, amu, vor, length, sa_rhs, w);
+}
+
+// contravariant velocities a, b, c and diag of sa_lusgs (sub_tur_sa.f90:
+// 719-753) at node n; ds = diasd at n (0 outside the computed range)
+__device__ inline void sa_prep_at(const Ext &e, long n, const Consts &c, double phi, double ds,
+                                  const double *__restrict__ pri, const double *__restrict__ dxidx,
+                                  const double *__restrict__ nut, const double *__restrict__ dt, double *cu4)
+{
+    const long N = e.n;
+    double u = pri[N + n], v = pri[2 * N + n], ww = pri[3 * N + n];
+    double m[9];
+#pragma unroll
+    for (int x = 0; x < 9; x++) m[x] = dxidx[x * N + n];
+    double x2 = m[0] * m[0] + m[1] * m[1] + m[2] * m[2];
+    double c0 = m[0] * u + m[1] * v + m[2] * ww;
+    x2 = x2 + m[3] * m[3] + m[4] * m[4] + m[5] * m[5];
+    double c1 = m[3] * u + m[4] * v + m[5] * ww;
+    x2 = x2 + m[6] * m[6] + m[7] * m[7] + m[8] * m[8];
+    double c2 = m[6] * u + m[7] * v + m[8] * ww;
+    double tm = 4.0 * x2 * nut[n] / c.re;
+    cu4[0] = c0;
+    cu4[1] = c1;
+    cu4[2] = c2;
+    cu4[3] = 1.0 / dt[n] * phi + fabs(c0) + fabs(c1) + fabs(c2) + tm - ds;
+}
+
+__device__ inline double sa_phi(const Consts &c, const double *cfl)
+{
+    return (*cfl >= 1.0e3 && c.itime == 0) ? 0.0 : 1.0;
+}
+
 // on the ring; dnut* and dnut cleared there.
 __global__ void __launch_bounds__(256) k_sa_prep(Ext e, Box ring, Consts c, const double *__restrict__ pri,
                                                  const double *__restrict__ dxidx,
@@ -224,24 +294,14 @@ __global__ void __launch_bounds__(256) k_sa_prep(Ext e, Box ring, Consts c, cons
     long n;
     if (!lin_node_at(e, ring, xcd_slab(blockIdx.x, gridDim.x), i, j, k, n)) return;
     const long N = e.n;
-    double phi = (*cfl >= 1.0e3 && c.itime == 0) ? 0.0 : 1.0;
-    double u = pri[N + n], v = pri[2 * N + n], ww = pri[3 * N + n];
-    double m[9];
-#pragma unroll
-    for (int x = 0; x < 9; x++) m[x] = dxidx[x * N + n];
-    double x2 = m[0] * m[0] + m[1] * m[1] + m[2] * m[2];
-    double c0 = m[0] * u + m[1] * v + m[2] * ww;
-    x2 = x2 + m[3] * m[3] + m[4] * m[4] + m[5] * m[5];
-    double c1 = m[3] * u + m[4] * v + m[5] * ww;
-    x2 = x2 + m[6] * m[6] + m[7] * m[7] + m[8] * m[8];
-    double c2 = m[6] * u + m[7] * v + m[8] * ww;
-    double tm = 4.0 * x2 * nut[n] / c.re;
     bool inner = i >= e.is1 && i <= e.ie1 && j >= e.js1 && j <= e.je1 && k >= e.ks1 && k <= e.ke1;
     double ds = inner ? w[7 * N + n] : 0.0;
-    cu[n] = c0;
-    cu[N + n] = c1;
-    cu[2 * N + n] = c2;
-    cu[3 * N + n] = 1.0 / dt[n] * phi + fabs(c0) + fabs(c1) + fabs(c2) + tm - ds;
+    double cu4[4];
+    sa_prep_at(e, n, c, sa_phi(c, cfl), ds, pri, dxidx, nut, dt, cu4);
+    cu[n] = cu4[0];
+    cu[N + n] = cu4[1];
+    cu[2 * N + n] = cu4[2];
+    cu[3 * N + n] = cu4[3];
     w[8 * N + n] = 0.0;
     dnut[n] = 0.0;
 }
@@ -282,6 +342,67 @@ __global__ void __launch_bounds__(256) k_add1(Ext e, Box b, double *__restrict__
     long n;
     if (!lin_node_at(e, b, xcd_slab(blockIdx.x, gridDim.x), i, j, k, n)) return;
     x[n] = x[n] + d[n];
+}
+
+// ------------------------------------------------ every block in one launch
+// Pass A over each block's whole storage (linear tiles of 256 nodes): the
+// length scale on the extended block, stage A on the computed range
+// (without the nut gradient, which pass B forms again).
+__global__ void __launch_bounds__(256) k_sa_a_batch(const BlockRec *__restrict__ blks, const int *__restrict__ map,
+                                                    const KGrid *__restrict__ kg, double *arena, Consts c)
+{
+    int bx, by, bz;
+    const BlockRec &b = blks[batch_wg(map, kg, bx, by, bz)];
+    const Ext e = b.e;
+    const Box all{e.is, e.ie, e.js, e.je, e.ks, e.ke};
+    int i, j, k;
+    long n;
+    if (!lin_node_at(e, all, bx, i, j, k, n)) return;
+    const double *pri = arena + b.o.pri, *nut = arena + b.o.nut, *amu = arena + b.o.amu;
+    arena[b.o.length + n] = sa_length_at(e, c, n, arena + b.o.dist, arena + b.o.spacing, pri, nut, amu,
+                                         arena + b.o.dudx);
+    if (i < e.is1 || i > e.ie1 || j < e.js1 || j > e.je1 || k < e.ks1 || k > e.ke1) return;
+    sa_a_at<false>(e, i, j, k, n, pri, arena + b.o.dxidx, nut, amu, arena + b.o.aal, arena + b.o.bbe,
+                   arena + b.o.gga, arena + b.o.sa_rhs, arena + b.o.w_sa);
+}
+
+// Pass B on the computed range: stage B (sa_rhs completed) and the sweep
+// operands a, b, c, diag of the node into w_dqs (components 0..3); block m
+// reads its cfl at cfl[m * cfl_stride].
+__global__ void __launch_bounds__(256) k_sa_b_batch(const BlockRec *__restrict__ blks, const int *__restrict__ map,
+                                                    const KGrid *__restrict__ kg, double *arena, Consts c,
+                                                    const double *cfl, int cfl_stride)
+{
+    int bx, by, bz;
+    const int m = batch_wg(map, kg, bx, by, bz);
+    const BlockRec &b = blks[m];
+    const Ext e = b.e;
+    const Box inner{e.is1, e.ie1, e.js1, e.je1, e.ks1, e.ke1};
+    int i, j, k;
+    long n;
+    if (!lin_node_at(e, inner, bx, i, j, k, n)) return;
+    const double *pri = arena + b.o.pri, *dxidx = arena + b.o.dxidx, *nut = arena + b.o.nut;
+    const double ds = sa_b_at<false>(e, i, j, k, n, c, pri, dxidx, arena + b.o.inv_j, nut, arena + b.o.amu,
+                                     arena + b.o.vor, arena + b.o.length, arena + b.o.sa_rhs, arena + b.o.w_sa);
+    double cu4[4];
+    sa_prep_at(e, n, c, sa_phi(c, cfl + (long)m * cfl_stride), ds, pri, dxidx, nut, arena + b.o.dt, cu4);
+    double *cu = arena + b.o.w_dqs;
+    const long N = e.n;
+    cu[n] = cu4[0];
+    cu[N + n] = cu4[1];
+    cu[2 * N + n] = cu4[2];
+    cu[3 * N + n] = cu4[3];
+}
+
+int launch_sa_pre_batch(hipStream_t s, const BlockRec *blks, const BatchGrid &bg, double *arena, const Consts &c,
+                        const double *cfl, int cfl_stride, int part)
+{
+    if (bg.total <= 0) return 0;
+    const dim3 g((unsigned)bg.total), t(64, 4, 1);
+    if (part != 2) hipLaunchKernelGGL(k_sa_a_batch, g, t, 0, s, blks, bg.map, bg.kg, arena, c);
+    if (part != 1) hipLaunchKernelGGL(k_sa_b_batch, g, t, 0, s, blks, bg.map, bg.kg, arena, c, cfl, cfl_stride);
+    ACFD_CHECK_LAUNCH();
+    return 0;
 }
 
 // Everything before the sweeps: length scale, right-hand side, diagonal.

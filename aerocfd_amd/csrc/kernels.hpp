@@ -52,13 +52,19 @@ dim3 viscous_grid(const Ext &e);
 dim3 lu_prep_grid(const Ext &e);
 int launch_getpri_batch(hipStream_t s, const BlockRec *blks, const BatchGrid &bg, double *arena, const Consts &c,
                         unsigned int *nneg);
-// part -1: every kernel of the stage; otherwise only one of them (inviscid:
-// 0 clear rhsi, 1..3 the i, j, k directions; viscous: 1 node fluxes, 2
-// their difference), for per-kernel timing
+// part -1: every kernel of the stage (inviscid -2: all but the clearing of
+// rhsi); otherwise only one of them (inviscid: 0 clear rhsi, 1..3 the i, j,
+// k directions; viscous: 1 node fluxes, 2 their difference), for per-kernel
+// timing
 int launch_inviscid_batch(hipStream_t s, const BlockRec *blks, const BatchGrid *bg, double *arena,
                           const Consts &c, int precision, int part = -1);
 int launch_viscous_batch(hipStream_t s, const BlockRec *blks, const BatchGrid &bg, double *arena, const Consts &c,
                          int precision, int store_aux, int part = -1);
+// the viscous stage with the steady find_dt folded into its node pass
+// (statistics into stats from the last block; reset them first)
+int launch_viscous_dt_batch(hipStream_t s, const BlockRec *blks, int nblk, const BatchGrid &bg, double *arena,
+                            const Consts &c, int precision, int store_aux, const double *cfl,
+                            unsigned long long *stats, int part = -1);
 int launch_find_dt_batch(hipStream_t s, const BlockRec *blks, int nblk, const BatchGrid &bg, double *arena,
                          const Consts &c, double *cfl, double *cfl_blk, unsigned long long *stats);
 int launch_lu_prep_batch(hipStream_t s, const BlockRec *blks, const BatchGrid &bg, double *arena, const double *cfl,
@@ -116,11 +122,18 @@ int lusgs_batch_make(int nblk, const Ext *e, const Fields *f, double *arena, int
 void lusgs_batch_free(LuBatch *b);
 int launch_lusgs_batch(hipStream_t s, const LuBatch &b, int phase, double gamma, int *err, int *ctr,
                        int geo_cached = 0);
+
 int launch_fill_sentinel(hipStream_t s, double *p, long n);
 // tile sweeps of the SA model's scalar LU-SGS (cu = a, b, c, diag)
 int launch_sa_sweeps(hipStream_t s, const Ext &e, int shape, const double *cu, const double *rhs, double *dns,
                      double *dn, double *ex, int *err);
-int sa_batch_make(int nblk, const Ext *e, const Fields *f, double *arena, int shape, double *ex, LuBatch *out);
+// sl: the SA sweep layout (sa_batch_sl doubles for this shape) or null
+void sa_batch_sl(int nblk, const Ext *e, int shape, long *sl_n);
+int sa_batch_make(int nblk, const Ext *e, const Fields *f, double *arena, int shape, double *ex, double *sl,
+                  LuBatch *out);
+// the batched SA sweeps over the SA sweep layout: phase 0 operands into
+// the layout, 1 forward sweep, -1 backward sweep, 2 nut += dnut
+int launch_sa_sl_batch(hipStream_t s, const LuBatch &b, int *err, int *ctr, int phase);
 // dir 1: the forward sweep only, -1: the backward sweep only, 0: both
 int launch_sa_batch(hipStream_t s, const LuBatch &b, int *err, int *ctr, int dir = 0);
 
@@ -138,6 +151,11 @@ int launch_sa(hipStream_t s, const Ext &e, const Fields &f, const Consts &c, con
               const SaSweep *sweep = nullptr);
 int launch_sa_pre(hipStream_t s, const Ext &e, const Fields &f, const Consts &c, const double *cfl);
 int launch_sa_post(hipStream_t s, const Ext &e, const Fields &f);
+// length scale, right-hand side and the sweep operands of every block in two
+// launches (linear-tile grid); block m reads its cfl at cfl[m * cfl_stride];
+// part 1 / 2 launches only the first / second pass (per-kernel timing)
+int launch_sa_pre_batch(hipStream_t s, const BlockRec *blks, const BatchGrid &bg, double *arena, const Consts &c,
+                        const double *cfl, int cfl_stride, int part = 0);
 
 // k_bc.hip
 struct BcTask {       // one physical subface

@@ -45,7 +45,8 @@ TIMED_KERNELS = {"lusgs_forward": 12, "lusgs_backward": 13, "lusgs_gather": 14, 
                  "lusgs_add": 16,
                  # one kernel each, recorded for the batched stages
                  "sa_forward": 17, "sa_backward": 18, "sa_pre": 19, "sa_add": 20, "inviscid_zero": 21,
-                 "inviscid_i": 22, "inviscid_j": 23, "inviscid_k": 24, "viscous_node": 25, "viscous_diff": 26}
+                 "inviscid_i": 22, "inviscid_j": 23, "inviscid_k": 24, "viscous_node": 25, "viscous_diff": 26,
+                 "sa_gather": 27, "sa_pre_b": 28, "viscous_node_dt": 29}
 TIME_SLOTS = 32
 
 GAMMA = 1.4

@@ -59,6 +59,15 @@ BYTES_PER_NODE = {
     # in, dnut* / dnut out
     "sa_forward": 5 * 8 + 8,
     "sa_backward": 5 * 8 + 8,
+    # the batched SA update's passes: stage A (length, convection, alpha /
+    # beta / gamma products): dist, spacing, rho, u, v, w, nut, mu,
+    # dxidx, alpha/beta/gamma in; length, sa_rhs, 4 work fields out
+    "sa_pre": 16 + 32 + 8 + 8 + 72 + 72 + 8 + 8 + 32,
+    # stage B + sweep operands: 4 work fields, dxidx, inv_j, nut, sa_rhs,
+    # rho, u, v, w, mu, vorticity, length, dt in; sa_rhs, a, b, c, diag out
+    "sa_pre_b": 32 + 72 + 8 + 8 + 8 + 32 + 8 + 8 + 8 + 8 + 8 + 32,
+    "sa_gather": 2 * 5 * 8,                         # a, b, c, diag, rhs -> SA sweep layout
+    "sa_add": 8 + 16,                               # dnut (layout), nut -> nut
     # single kernels of the batched inviscid / viscous stages
     "inviscid_zero": 40,                            # rhsi cleared
     "inviscid_i": 40 + 24 + 8 + 40,                 # q, grad xi, inv_j -> rhsi
@@ -84,7 +93,9 @@ SCHEMES = {
 LAUNCHES_PER_CALL = {"inviscid": 3, "viscous": 2}
 SINGLE_KERNEL = {"getpri", "lusgs_forward", "lusgs_backward", "lusgs_gather", "lusgs_add", "lusgs_other",
                  "sa_forward", "sa_backward", "inviscid_zero", "inviscid_i", "inviscid_j", "inviscid_k",
-                 "viscous_node", "viscous_diff"}
+                 "viscous_node", "viscous_diff", "sa_gather", "sa_pre_b", "sa_pre", "sa_add"}
+# timers that are one kernel only on the batched SA path (per block otherwise)
+SA_BATCHED_ONLY = {"sa_pre", "sa_add"}
 HYPERPLANE_BYTES = {  # the hyperplane sweeps read the operands in place
     "lusgs_forward": 40 + 72 + 8 + 24 + 8 + 40 + 40,
     "lusgs_backward": 40 + 72 + 8 + 24 + 8 + 40 + 40,
@@ -321,7 +332,7 @@ def roofline_of(args, tm, rr, sa):
     achieved algorithmic GB/s."""
     comp_all = sum((r[13] - r[12] + 1) * (r[15] - r[14] + 1) * (r[17] - r[16] + 1) for r in rr)
     batched = {"lusgs_forward", "lusgs_backward", "lusgs_gather", "lusgs_add", "sa_forward",
-               "sa_backward"} if args.lusgs == 2 else set()
+               "sa_backward", "sa_gather", "sa_pre", "sa_pre_b", "sa_add"} if args.lusgs == 2 else set()
     if args.batch:
         batched |= {"lusgs_other", "getpri", "inviscid", "viscous", "find_dt", "inviscid_zero", "inviscid_i",
                     "inviscid_j", "inviscid_k", "viscous_node", "viscous_diff"}
@@ -339,7 +350,12 @@ def roofline_of(args, tm, rr, sa):
             table["lusgs_backward"] = FUSED_BACKWARD_BYTES
         if os.environ.get("ACFD_SL_GEO_CACHE", "1") == "0":
             table["lusgs_gather"] = 2 * 24 * 8
-    cand = {k: v for k, v in tm.items() if k in table and k in SINGLE_KERNEL and v[1] > 0}
+
+    # sa_pre / sa_add are single kernels only on the batched SA path (the
+    # per-block path times its launches of every block together)
+    sa_batched = "sa_gather" in tm and tm["sa_gather"][1] > 0
+    cand = {k: v for k, v in tm.items() if k in table and k in SINGLE_KERNEL and v[1] > 0
+            and (sa_batched or k not in SA_BATCHED_ONLY)}
     if not cand:
         # explicit RK3 on one block: no sweep; the stages themselves (one
         # call = all kernels of the stage for one block)

@@ -141,9 +141,13 @@ int acfd_sync(acfd_ctx *ctx);
  * sweeps), [15] rhs / spectral radii / diagonal, [16] q += dq.  With
  * batch_blocks / lusgs_kernel 2 each kernel of a batched stage is also
  * timed on its own: [17] SA forward sweep, [18] SA backward sweep, [19] SA
- * length scale / right-hand side / diagonal, [20] nut += dnut, [21] rhsi
- * cleared, [22..24] inviscid i, j, k direction, [25] viscous node fluxes,
- * [26] their difference.  Indices 0 and 27..31 are unused.  The
+ * length scale / right-hand side / diagonal (with lusgs_kernel 2: the
+ * first of its two per-node passes, [28] the second), [20] nut += dnut,
+ * [21] rhsi cleared, [22..24] inviscid i, j, k direction, [25] viscous node
+ * fluxes, [26] their difference, [27] the SA operands moved into their
+ * sweep layout, [29] the viscous node fluxes with the steady local time
+ * step (NS, steady, batched: find_dt folded into that pass).  Indices 0, 30
+ * and 31 are unused.  The
  * reference's only instrument is a per-step wall time (main.f90:575-586). */
 int acfd_timing(acfd_ctx *ctx, int enable, double *ms, long *calls);
 
