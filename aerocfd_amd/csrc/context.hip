@@ -10,6 +10,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -2542,6 +2543,83 @@ int acfd_sa_step(const acfd_config *cfg, const int *r, double *nut, double *sa_r
     ST_TRY(acfd_download(s.ctx, 1, ACFD_NUT, nut));
     ST_TRY(acfd_download(s.ctx, 1, ACFD_SA_RHS, sa_rhs));
     return acfd_download(s.ctx, 1, ACFD_LENGTH, length);
+}
+
+}  // extern "C"
+
+// ------------------------------------------------- no-allocation viscous_flux
+// The viscous node fluxes (12 doubles per node) of acfd_viscous_rhs_noalloc
+// live in one file-scope device array instead of an allocation.  Each GPU
+// that loads the library holds its own copy (48 MiB) until the process
+// ends.  Host code never uses the array's name as an address: the address
+// on each device comes from hipGetSymbolAddress, once per device, with that
+// device current.
+namespace {
+
+constexpr long kNoAllocMaxNodes = 1L << 19;  // extended nodes of one call
+constexpr int kMaxDevices = 64;
+
+__device__ __align__(256) double scratch[12 * kNoAllocMaxNodes];
+
+double *g_scratch_ptr[kMaxDevices] = {nullptr};
+std::mutex g_scratch_mutex;
+
+int scratch_on_current_device(double **out)
+{
+    int dev = -1;
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return fail(ACFD_EDEVICE, "acfd_viscous_rhs_noalloc: device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return fail(ACFD_EDEVICE, "acfd_viscous_rhs_noalloc: no address for the scratch array");
+        g_scratch_ptr[dev] = (double *)p;
+    }
+    *out = g_scratch_ptr[dev];
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int acfd_viscous_rhs_noalloc(const acfd_config *cfg, const int *r, const double *pri_v, double *amu,
+                             const double *dxidx, const double *inv_j, double *rhsv, double *vor, double *dudx,
+                             double *tao, void *stream)
+{
+    if (!cfg || !r || !pri_v || !amu || !dxidx || !inv_j || !rhsv)
+        return fail(ACFD_EINVAL, "acfd_viscous_rhs_noalloc: null argument");
+    if ((dudx || tao) && !(vor && dudx && tao))
+        return fail(ACFD_EINVAL, "acfd_viscous_rhs_noalloc: dudx and tao need vor, dudx and tao together");
+    if (cfg->precision != 0 && cfg->precision != 1)
+        return fail(ACFD_EINVAL, "acfd_viscous_rhs_noalloc: precision must be 0 or 1");
+    const Ext e = ext_of_ranges(r);
+    if (e.ni < 3 || e.nj < 3 || e.nk < 3 || e.is0 < e.is || e.ie0 > e.ie || e.js0 < e.js || e.je0 > e.je ||
+        e.ks0 < e.ks || e.ke0 > e.ke || e.ie0 - e.is0 < 1 || e.je0 - e.js0 < 1 || e.ke0 - e.ks0 < 1)
+        return fail(ACFD_EINVAL, "acfd_viscous_rhs_noalloc: bad ranges");
+    if (e.n > kNoAllocMaxNodes)
+        return fail(ACFD_EINVAL, "acfd_viscous_rhs_noalloc: block of " + std::to_string(e.n) +
+                                     " extended nodes exceeds the scratch limit of " +
+                                     std::to_string(kNoAllocMaxNodes));
+    double *w = nullptr;
+    int rc = scratch_on_current_device(&w);
+    if (rc) return rc;
+    Consts c = consts_of(*cfg);
+    c.solver = 0;
+    Fields f{};
+    f.pri = const_cast<double *>(pri_v);
+    f.amu = amu;
+    f.dxidx = const_cast<double *>(dxidx);
+    f.inv_j = const_cast<double *>(inv_j);
+    f.rhsv = rhsv;
+    f.vor = vor;
+    f.dudx = dudx;
+    f.tao = tao;
+    f.w_flux = w;
+    const int store_aux = dudx ? 2 : vor ? 1 : 0;
+    KRN_TRY(launch_viscous((hipStream_t)stream, e, f, c, cfg->precision, store_aux));
+    return 0;
 }
 
 }  // extern "C"

@@ -185,6 +185,25 @@ int acfd_sa_step(const acfd_config *cfg, const int *ranges, double *nut, double 
                  const double *aalpha, const double *bbeta, const double *ggamma,
                  double *length, const double *dist, const double *spacing, double cfl);
 
+/* No counterpart in the reference: viscous_flux (the same computation and
+ * results as acfd_viscous_rhs) for callers that cannot allocate device
+ * memory -- code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc.  It allocates nothing: every array is a DEVICE
+ * pointer in the library's own layout (structure of arrays over the
+ * extended extents: component c of node (i,j,k) at
+ * a[c*n + (i-is) + ni*((j-js) + nj*(k-ks))], n = ni*nj*nk), and the node
+ * fluxes are staged in a static device array of the library sized for at
+ * most 524288 (2^19) extended nodes per call; larger blocks return
+ * ACFD_EINVAL.  rhsv is written on the physical range is0..ie0 (elsewhere
+ * untouched); amu components 1 and 3 are written there; vor (and dudx, tao,
+ * which need vor) may be NULL.  Work is queued on `stream` (a hipStream_t,
+ * NULL = the null stream) and not waited for.  Every call on one GPU
+ * shares that GPU's staging array: calls that overlap on one GPU (other
+ * threads, other streams without ordering) race. */
+int acfd_viscous_rhs_noalloc(const acfd_config *cfg, const int *ranges, const double *pri_v,
+                             double *amu, const double *dxidx, const double *inv_j, double *rhsv,
+                             double *vor, double *dudx, double *tao, void *stream);
+
 /* ------------------------------------------------------------- grid setup */
 /* Halo coordinate exchange over the context's topology, as
  * UpdateBufferCoordinate (sub_bc_updatebuffercoordinate.f90:10-473), with an
