@@ -161,6 +161,10 @@ struct acfd_ctx {
     // inviscid stage then skips clearing it.  An upload of rhsi clears the
     // flag (and drops a captured step that relied on it).
     bool rhsi_clean = true;
+    // the last residual stage also formed implicit1st's rhs, spectral radii
+    // and diagonal (batched, steady NS): the next batched update skips its
+    // preparation pass
+    bool lu_prep_ready = false;
     int *lu_ctr = nullptr;   // ticket counters of the batched sweeps
     // SA's update runs beside the flow's LU-SGS update on a second stream
     // (batched sweeps, SA): its own exchange slots and ticket counters
@@ -990,7 +994,11 @@ int st_implicit_all(acfd_ctx *ctx, const double *cfl_per_block)
         return 0;
     }
     StageTimer t(ctx, ACFD_STAGE_IMPLICIT1ST);
-    if (batched(ctx)) {
+    const bool prep_ready = ctx->lu_prep_ready;
+    ctx->lu_prep_ready = false;
+    if (prep_ready) {
+        // formed by the residual's viscous passes
+    } else if (batched(ctx)) {
         StageTimer ts(ctx, kTimeLuOther);
         KRN_TRY(launch_lu_prep_batch(ctx->stream, ctx->d_blocks, ctx->bg_prep, ctx->arena,
                                      cfl_per_block ? cfl_per_block : ctx->d_cfl, cfl_per_block ? 1 : 0, 0));
@@ -1259,16 +1267,20 @@ int st_viscous_dt_all(acfd_ctx *ctx, int store_aux)
     StageTimer t(ctx, ACFD_STAGE_VISCOUS);
     KRN_TRY(launch_reset_stats(ctx->stream, ctx->d_stats));
     const int nb = (int)ctx->blk.size();
+    // the batched LU-SGS update's spectral radii, diagonal and rhs come from
+    // these two passes too (implicit1st then skips its own preparation)
+    const int prep = ctx->cfg.lusgs_kernel == 2 && ctx->cfg.iflag_timeadvance == 2 ? 1 : 0;
     if (ctx->timing) {
         for (int part = 1; part <= 2; part++) {
             StageTimer ts(ctx, part == 1 ? kTimeViscNodeDt : kTimeViscDiff);
             KRN_TRY(launch_viscous_dt_batch(ctx->stream, ctx->d_blocks, nb, ctx->bg_visc, ctx->arena, ctx->c,
-                                            ctx->cfg.precision, store_aux, ctx->d_cfl, ctx->d_stats, part));
+                                            ctx->cfg.precision, store_aux, ctx->d_cfl, ctx->d_stats, part, prep));
         }
     } else {
         KRN_TRY(launch_viscous_dt_batch(ctx->stream, ctx->d_blocks, nb, ctx->bg_visc, ctx->arena, ctx->c,
-                                        ctx->cfg.precision, store_aux, ctx->d_cfl, ctx->d_stats));
+                                        ctx->cfg.precision, store_aux, ctx->d_cfl, ctx->d_stats, -1, prep));
     }
+    ctx->lu_prep_ready = prep == 1;
     return 0;
 }
 
@@ -1742,6 +1754,7 @@ int acfd_upload(acfd_ctx *ctx, int m, int field, const double *host)
     double *dst = field_ptr(b, field);
     if (!dst) return fail(ACFD_EINVAL, "acfd_upload: field not allocated for this configuration");
     if (field == ACFD_DXIDX || field == ACFD_INV_J) sl_invalidate(ctx);
+    ctx->lu_prep_ready = false;
     if (field == ACFD_RHSI) {
         ctx->rhsi_clean = false;
         drop_graph(ctx);
@@ -1777,6 +1790,7 @@ int acfd_run_stage(acfd_ctx *ctx, int stage)
 {
     if (!ctx || !ctx->topo) return fail(ACFD_ESTATE, "acfd_run_stage: no topology");
     ctx->pri_current = false;
+    ctx->lu_prep_ready = false;
     const int nb = (int)ctx->blk.size();
     int rc = 0;
     switch (stage) {

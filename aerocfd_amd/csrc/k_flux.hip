@@ -473,9 +473,12 @@ __global__ void __launch_bounds__(256) k_viscous_node(Ext e, Consts c, const dou
                          store_aux);
 }
 
-// Stage 2 (sub_viscous.f90:296-365): rhsv = (d/dxi fv + d/deta gv + d/dzeta hv)/re
+// Stage 2 (sub_viscous.f90:296-365): rhsv = (d/dxi fv + d/deta gv + d/dzeta hv)/re;
+// with rhsi given, also rhs = rhsi + rhsv on the computed range (the
+// first line of implicit1st, sub_timeadvancement_1stimplicit.f90:72-78)
 __device__ inline void viscous_diff_body(const Ext &e, long wg, const double *__restrict__ w,
-                                         double *__restrict__ rhsv, double re)
+                                         double *__restrict__ rhsv, double re, const double *__restrict__ rhsi = nullptr,
+                                         double *__restrict__ rhs = nullptr)
 {
     const Box b{e.is0, e.ie0, e.js0, e.je0, e.ks0, e.ke0};
     int i, j, k;
@@ -483,13 +486,18 @@ __device__ inline void viscous_diff_body(const Ext &e, long wg, const double *__
     if (!lin_node_at(e, b, wg, i, j, k, n)) return;
     const long N = e.n;
     const long sj = e.ni, sk = (long)e.ni * e.nj;
-    rhsv[n] = (0.0 + 0.0 + 0.0) / re;
+    const double v0 = (0.0 + 0.0 + 0.0) / re;
+    rhsv[n] = v0;
+    const bool inner = rhs && i >= e.is1 && i <= e.ie1 && j >= e.js1 && j <= e.je1 && k >= e.ks1 && k <= e.ke1;
+    if (inner) rhs[n] = rhsi[n] + v0;
 #pragma unroll
     for (int c = 0; c < 4; c++) {
         double a = ndiff(w + (0 + c) * N, n, 1, i, e.is0, e.ie0);
         double bb = ndiff(w + (4 + c) * N, n, sj, j, e.js0, e.je0);
         double h = ndiff(w + (8 + c) * N, n, sk, k, e.ks0, e.ke0);
-        rhsv[(c + 1) * N + n] = (a + bb + h) / re;
+        const double v = (a + bb + h) / re;
+        rhsv[(c + 1) * N + n] = v;
+        if (inner) rhs[(c + 1) * N + n] = rhsi[(c + 1) * N + n] + v;
     }
 }
 
@@ -512,14 +520,19 @@ __global__ void __launch_bounds__(256) k_viscous_node_batch(const BlockRec *__re
                          arena + b.o.w_flux, arena + b.o.vor, arena + b.o.dudx, arena + b.o.tao, store_aux);
 }
 
+// with_rhs: also rhs = rhsi + rhsv on the computed range
 __global__ void __launch_bounds__(256) k_viscous_diff_batch(const BlockRec *__restrict__ blks,
                                                             const int *__restrict__ map,
-                                                            const KGrid *__restrict__ kg, double *arena, double re)
+                                                            const KGrid *__restrict__ kg, double *arena, double re,
+                                                            int with_rhs = 0)
 {
     int bx, by, bz;
     const BlockRec &b = blks[batch_wg(map, kg, bx, by, bz)];
     const Ext e = b.e;
-    viscous_diff_body(e, bx, arena + b.o.w_flux, arena + b.o.rhsv, re);
+    if (with_rhs)
+        viscous_diff_body(e, bx, arena + b.o.w_flux, arena + b.o.rhsv, re, arena + b.o.rhsi, arena + b.o.rhs);
+    else
+        viscous_diff_body(e, bx, arena + b.o.w_flux, arena + b.o.rhsv, re);
 }
 
 // per-block grid of both viscous kernels: linear tiles of 256 nodes over the
@@ -687,7 +700,13 @@ __global__ void k_reset_stats(unsigned long long *st)
 // formed; every other node of the extended block only dt (k_find_dt_steady
 // reads the stored viscosity there).  The same expressions as the two
 // kernels; the dt statistics from the last block.
-template <typename R>
+//
+// With PREP, the node also forms implicit1st's spectral radii and diagonal
+// (sub_timeadvancement_1stimplicit.f90:98-156, the expressions of
+// lu_prep_body) from its dt, for the tile sweeps of the batched update; the
+// difference pass then forms rhs.  Every node of the extended block gets
+// them (the ring of the computed range is what the update reads).
+template <typename R, bool PREP>
 __global__ void __launch_bounds__(256) k_viscous_node_dt_batch(const BlockRec *__restrict__ blks,
                                                                const int *__restrict__ map,
                                                                const KGrid *__restrict__ kg, double *arena,
@@ -724,25 +743,39 @@ __global__ void __launch_bounds__(256) k_viscous_node_dt_batch(const BlockRec *_
         }
         v = *cfl / local_rate_a3(e, n, pri, a3, dxidx, c.re);
         arena[b.o.dt + n] = v;
+        if (PREP) {
+            const double phi = *cfl >= 1.0e3 ? 0.0 : 1.0;  // implicit1st :98-102
+            arena[b.o.w_diag + n] = lu_radii_diag(pri, dxidx, N, n, v, phi, arena + b.o.w_r);
+        }
     }
     dt_stats(ok, v, m == nblk - 1 ? stats : nullptr);
 }
 
+template <bool PREP>
+static void viscous_dt_node(hipStream_t s, const BlockRec *blks, int nblk, const BatchGrid &bg, double *arena,
+                            const Consts &c, int precision, int store_aux, const double *cfl, unsigned long long *stats)
+{
+    const dim3 g((unsigned)bg.total), t(64, 4, 1);
+    if (precision == 1)
+        hipLaunchKernelGGL((k_viscous_node_dt_batch<float, PREP>), g, t, 0, s, blks, bg.map, bg.kg, arena, c,
+                           store_aux, nblk, cfl, stats);
+    else
+        hipLaunchKernelGGL((k_viscous_node_dt_batch<double, PREP>), g, t, 0, s, blks, bg.map, bg.kg, arena, c,
+                           store_aux, nblk, cfl, stats);
+}
+
 int launch_viscous_dt_batch(hipStream_t s, const BlockRec *blks, int nblk, const BatchGrid &bg, double *arena,
                             const Consts &c, int precision, int store_aux, const double *cfl,
-                            unsigned long long *stats, int part)
+                            unsigned long long *stats, int part, int prep)
 {
     if (bg.total <= 0) return 0;
     const dim3 g((unsigned)bg.total), t(64, 4, 1);
     if (part < 0 || part == 1) {
-        if (precision == 1)
-            hipLaunchKernelGGL((k_viscous_node_dt_batch<float>), g, t, 0, s, blks, bg.map, bg.kg, arena, c,
-                               store_aux, nblk, cfl, stats);
-        else
-            hipLaunchKernelGGL((k_viscous_node_dt_batch<double>), g, t, 0, s, blks, bg.map, bg.kg, arena, c,
-                               store_aux, nblk, cfl, stats);
+        if (prep) viscous_dt_node<true>(s, blks, nblk, bg, arena, c, precision, store_aux, cfl, stats);
+        else viscous_dt_node<false>(s, blks, nblk, bg, arena, c, precision, store_aux, cfl, stats);
     }
-    if (part < 0 || part == 2) hipLaunchKernelGGL(k_viscous_diff_batch, g, t, 0, s, blks, bg.map, bg.kg, arena, c.re);
+    if (part < 0 || part == 2)
+        hipLaunchKernelGGL(k_viscous_diff_batch, g, t, 0, s, blks, bg.map, bg.kg, arena, c.re, prep);
     ACFD_CHECK_LAUNCH();
     return 0;
 }

@@ -165,19 +165,8 @@ __device__ inline void lu_prep_body(const Ext &e, long wg, const double *__restr
     long n;
     if (!lin_node_at(e, ring, wg, i, j, k, n)) return;
     const long N = e.n;
-    double phi = *cfl >= 1.0e3 ? 0.0 : 1.0;  // :98-102
-    double u = pri[N + n], v = pri[2 * N + n], w = pri[3 * N + n], c = pri[6 * N + n];
-    double r3[3];
-#pragma unroll
-    for (int d = 0; d < 3; d++) {
-        double m0 = dxidx[(3 * d) * N + n], m1 = dxidx[(3 * d + 1) * N + n], m2 = dxidx[(3 * d + 2) * N + n];
-        double temp = m0 * m0 + m1 * m1 + m2 * m2;
-        temp = sqrt(temp);
-        double cu = m0 * u + m1 * v + m2 * w;
-        r3[d] = fabs(cu) + fabs(c * temp);
-        rr[d * N + n] = r3[d];
-    }
-    diag[n] = phi * 1.0 / dt[n] + r3[0] + r3[1] + r3[2];
+    const double phi = *cfl >= 1.0e3 ? 0.0 : 1.0;  // :98-102
+    diag[n] = lu_radii_diag(pri, dxidx, N, n, dt[n], phi, rr);
     bool inner = i >= e.is1 && i <= e.ie1 && j >= e.js1 && j <= e.je1 && k >= e.ks1 && k <= e.ke1;
 #pragma unroll
     for (int x = 0; x < 5; x++) {

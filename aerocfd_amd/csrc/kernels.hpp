@@ -61,10 +61,13 @@ int launch_inviscid_batch(hipStream_t s, const BlockRec *blks, const BatchGrid *
 int launch_viscous_batch(hipStream_t s, const BlockRec *blks, const BatchGrid &bg, double *arena, const Consts &c,
                          int precision, int store_aux, int part = -1);
 // the viscous stage with the steady find_dt folded into its node pass
-// (statistics into stats from the last block; reset them first)
+// (statistics into stats from the last block; reset them first); with
+// prep, also implicit1st's spectral radii, diagonal (node pass) and rhs =
+// rhsi + rhsv (difference pass), which launch_lu_prep_batch then need not
+// form
 int launch_viscous_dt_batch(hipStream_t s, const BlockRec *blks, int nblk, const BatchGrid &bg, double *arena,
                             const Consts &c, int precision, int store_aux, const double *cfl,
-                            unsigned long long *stats, int part = -1);
+                            unsigned long long *stats, int part = -1, int prep = 0);
 int launch_find_dt_batch(hipStream_t s, const BlockRec *blks, int nblk, const BatchGrid &bg, double *arena,
                          const Consts &c, double *cfl, double *cfl_blk, unsigned long long *stats);
 int launch_lu_prep_batch(hipStream_t s, const BlockRec *blks, const BatchGrid &bg, double *arena, const double *cfl,

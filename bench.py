@@ -93,7 +93,7 @@ SCHEMES = {
 LAUNCHES_PER_CALL = {"inviscid": 3, "viscous": 2}
 SINGLE_KERNEL = {"getpri", "lusgs_forward", "lusgs_backward", "lusgs_gather", "lusgs_add", "lusgs_other",
                  "sa_forward", "sa_backward", "inviscid_zero", "inviscid_i", "inviscid_j", "inviscid_k",
-                 "viscous_node", "viscous_diff", "sa_gather", "sa_pre_b", "sa_pre", "sa_add"}
+                 "viscous_node", "viscous_diff", "sa_gather", "sa_pre_b", "sa_pre", "sa_add", "viscous_node_dt"}
 # timers that are one kernel only on the batched SA path (per block otherwise)
 SA_BATCHED_ONLY = {"sa_pre", "sa_add"}
 HYPERPLANE_BYTES = {  # the hyperplane sweeps read the operands in place
@@ -335,10 +335,18 @@ def roofline_of(args, tm, rr, sa):
                "sa_backward", "sa_gather", "sa_pre", "sa_pre_b", "sa_add"} if args.lusgs == 2 else set()
     if args.batch:
         batched |= {"lusgs_other", "getpri", "inviscid", "viscous", "find_dt", "inviscid_zero", "inviscid_i",
-                    "inviscid_j", "inviscid_k", "viscous_node", "viscous_diff"}
+                    "inviscid_j", "inviscid_k", "viscous_node", "viscous_diff", "viscous_node_dt"}
     table = dict(BYTES_PER_NODE)
     if sa:
         table["viscous_node"] = VISCOUS_NODE_SA
+    # the viscous node pass with the steady find_dt: + rho, c in, dt out;
+    # when it also forms implicit1st's preparation (batched sweeps, no
+    # separate preparation timed): + radii, diag out, and the difference
+    # pass + rhsi in, rhs out
+    folded = args.lusgs == 2 and tm.get("lusgs_other", (0, 0))[1] == 0 and tm.get("viscous_node_dt", (0, 0))[1] > 0
+    table["viscous_node_dt"] = table["viscous_node"] + 16 + 8 + (32 if folded else 0)
+    if folded:
+        table["viscous_diff"] = BYTES_PER_NODE["viscous_diff"] + 80
     if args.lusgs == 0:
         table.update(HYPERPLANE_BYTES)
     else:
