@@ -983,6 +983,12 @@ int tune_batch(acfd_ctx *ctx)
 // residuals equals updating each after its own; block m's update reads the
 // cfl its find_dt left (cfl + m), or the global one when cfl_per_block is
 // null.
+//
+// The batched update never clears dq (nor writes w_dqs / w_dq at all: its
+// sweeps keep dq* and dq in the sweep layout).  The SA update uses those two
+// work fields for its cu and dnut, and runs beside this update on a second
+// stream only because of that.
+constexpr int kBatchedPrepClearsDq = 0;
 int st_implicit_all(acfd_ctx *ctx, const double *cfl_per_block)
 {
     const int nb = (int)ctx->blk.size();
@@ -1001,7 +1007,8 @@ int st_implicit_all(acfd_ctx *ctx, const double *cfl_per_block)
     } else if (batched(ctx)) {
         StageTimer ts(ctx, kTimeLuOther);
         KRN_TRY(launch_lu_prep_batch(ctx->stream, ctx->d_blocks, ctx->bg_prep, ctx->arena,
-                                     cfl_per_block ? cfl_per_block : ctx->d_cfl, cfl_per_block ? 1 : 0, 0));
+                                     cfl_per_block ? cfl_per_block : ctx->d_cfl, cfl_per_block ? 1 : 0,
+                                     kBatchedPrepClearsDq));
     } else {
         StageTimer ts(ctx, kTimeLuOther);
         for (int m = 0; m < nb; m++) {
@@ -1335,6 +1342,7 @@ int one_step(acfd_ctx *ctx)
         ctx->graph_needs_pri = true;
     if (ctx->cfg.iflag_timeadvance == 2 && stagewise) {
         rc = residual_stage_all(ctx, aux, pri_ok);
+        static_assert(kBatchedPrepClearsDq == 0, "the concurrent SA update owns w_dqs / w_dq");
         if (!rc && sa && ctx->sa_concurrent && ctx->lub_shape >= 0 && !ctx->timing) {
             // sa_model reads nothing implicit1st writes (pri_v, amu, vor, dt
             // and the cfl are from the residual) and implicit1st reads no

@@ -382,16 +382,32 @@ int launch_residual(hipStream_t s, const Ext &e, const Fields &f, double *partia
     return 0;
 }
 
-__global__ void k_residual_final(const double *__restrict__ partial, int count, double *__restrict__ out)
+// One workgroup of kResFinal threads.  A fixed order, so the sums are
+// reproducible and the same for every launch form that lays the partials
+// out alike: thread t adds partials t, t + kResFinal, ... in turn, then a
+// pairwise tree over the threads in LDS.
+constexpr int kResFinal = 256;
+__global__ void __launch_bounds__(kResFinal) k_residual_final(const double *__restrict__ partial, int count,
+                                                              double *__restrict__ out)
 {
-    int v = threadIdx.x;
-    if (v >= 6) return;
-    // in index order (the sums are reproducible); unrolled so that the
-    // loads are issued ahead of the dependent additions
-    double s = 0.0;
-#pragma unroll 16
-    for (int x = 0; x < count; x++) s += partial[6 * x + v];
-    out[v] = s;
+    __shared__ double red[6][kResFinal];
+    const int t = threadIdx.x;
+    double s[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int x = t; x < count; x += kResFinal) {
+#pragma unroll
+        for (int v = 0; v < 6; v++) s[v] += partial[6L * x + v];
+    }
+#pragma unroll
+    for (int v = 0; v < 6; v++) red[v][t] = s[v];
+    __syncthreads();
+    for (int h = kResFinal / 2; h > 0; h >>= 1) {
+        if (t < h) {
+#pragma unroll
+            for (int v = 0; v < 6; v++) red[v][t] += red[v][t + h];
+        }
+        __syncthreads();
+    }
+    if (t < 6) out[t] = red[t][0];
 }
 
 int launch_residual_batch(hipStream_t s, const BlockRec *blks, int nblk, double *arena, double *partial, int with_sa)
@@ -404,7 +420,7 @@ int launch_residual_batch(hipStream_t s, const BlockRec *blks, int nblk, double 
 
 int launch_residual_final(hipStream_t s, const double *partial, int count, double *out)
 {
-    hipLaunchKernelGGL(k_residual_final, dim3(1), dim3(64), 0, s, partial, count, out);
+    hipLaunchKernelGGL(k_residual_final, dim3(1), dim3(kResFinal), 0, s, partial, count, out);
     ACFD_CHECK_LAUNCH();
     return 0;
 }

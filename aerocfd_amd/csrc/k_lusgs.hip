@@ -338,16 +338,16 @@ __device__ inline void xstore(double *p, double v)
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Take the five values of one exchange slot: `pre` holds what was loaded a
-// step earlier; components still at the sentinel are polled until the
+// Take the five values of one exchange slot (component x at slot + x *
+// stride): `pre` holds what was loaded a step earlier; components still at the sentinel are polled until the
 // producer's store arrives, each poll reloading the later components too,
 // so that a slot that was late costs one round trip more, not one per
 // component.  The wait is bounded: running out sets *err and every later
 // wait of the launch gives up at once, so the sweep always ends (the host
 // then reports the failure).  With `reset` the slot is set back to the
 // sentinel for the next sweep (otherwise the host refills the region).
-__device__ inline void ex_take(double *slot, const unsigned long long *pre, double *v, int *err, bool &failed,
-                               bool reset)
+__device__ inline void ex_take(double *slot, int stride, const unsigned long long *pre, double *v, int *err,
+                               bool &failed, bool reset)
 {
     unsigned long long cur[5];
 #pragma unroll
@@ -357,10 +357,10 @@ __device__ inline void ex_take(double *slot, const unsigned long long *pre, doub
         unsigned long long b = cur[x];
         if (b == kSentinel) {
             for (long spins = 0; !failed; spins++) {
-                b = xload_bits(slot + x);
+                b = xload_bits(slot + x * stride);
 #pragma unroll
                 for (int y = x + 1; y < 5; y++)
-                    if (cur[y] == kSentinel) cur[y] = xload_bits(slot + y);
+                    if (cur[y] == kSentinel) cur[y] = xload_bits(slot + y * stride);
                 if (b != kSentinel) break;
                 if ((spins & 1023) == 1023 &&
                     (spins > (1L << 21) || __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
@@ -372,7 +372,7 @@ __device__ inline void ex_take(double *slot, const unsigned long long *pre, doub
             if (failed) b = 0;
         }
         v[x] = __longlong_as_double((long long)b);
-        if (reset) xstore_bits(slot + x, kSentinel);
+        if (reset) xstore_bits(slot + x * stride, kSentinel);
     }
 }
 
@@ -622,11 +622,13 @@ __device__ inline void lu_tiles_body(const SlGeom &g, int tile, double *__restri
     const bool j_out = sj == TJ - 1 && (DIR > 0 ? bj + 1 < g.nbj : bj > 0);
     const bool k_out = sk == TK - 1 && (DIR > 0 ? bk + 1 < g.nbk : bk > 0);
     double *exj = ex, *exk = ex + (long)g.ntiles * g.exj_tile;
-    // slots: j region [tile][d = pos + sk][tk][5], k region [tile][d = pos + sj][tj][5]
-    double *jin = exj + (long)(DIR > 0 ? tile - 1 : tile + 1) * g.exj_tile + (long)tk * 5;
-    double *kin = exk + (long)(DIR > 0 ? tile - g.nbj : tile + g.nbj) * g.exk_tile + (long)tj * 5;
-    double *jout = exj + (long)tile * g.exj_tile + (long)tk * 5;
-    double *kout = exk + (long)tile * g.exk_tile + (long)tj * 5;
+    // slots: j region [tile][d = pos + sk][5][tk], k region [tile][d = pos + sj][5][tj]:
+    // the lanes of a tile face store and load one component of their slots
+    // as one contiguous run (TK or TJ doubles)
+    double *jin = exj + (long)(DIR > 0 ? tile - 1 : tile + 1) * g.exj_tile + tk;
+    double *kin = exk + (long)(DIR > 0 ? tile - g.nbj : tile + g.nbj) * g.exk_tile + tj;
+    double *jout = exj + (long)tile * g.exj_tile + tk;
+    double *kout = exk + (long)tile * g.exk_tile + tj;
     bool failed = false;
     double ta[5] = {0, 0, 0, 0, 0};
     double opA[kSlF], opB[kSlF];
@@ -657,9 +659,9 @@ __device__ inline void lu_tiles_body(const SlGeom &g, int tile, double *__restri
         const int dj = j_in ? min(max(pos + sk, 0), xj_last) : 0;
         const int dk = k_in ? min(max(pos + sj, 0), xk_last) : 0;
 #pragma unroll
-        for (int x = 0; x < 5; x++) xj[x] = xload_bits(jsrc + (long)dj * TK * 5 + x);
+        for (int x = 0; x < 5; x++) xj[x] = xload_bits(jsrc + (long)dj * TK * 5 + x * TK);
 #pragma unroll
-        for (int x = 0; x < 5; x++) xk[x] = xload_bits(ksrc + (long)dk * TJ * 5 + x);
+        for (int x = 0; x < 5; x++) xk[x] = xload_bits(ksrc + (long)dk * TJ * 5 + x * TJ);
     };
     auto step = [&](int s, const double *op, const unsigned long long *xj, const unsigned long long *xk) {
         const int p = s & 1;
@@ -670,7 +672,7 @@ __device__ inline void lu_tiles_body(const SlGeom &g, int tile, double *__restri
 #pragma unroll
                 for (int x = 0; x < 5; x++) tb[x] = 0.0;
             } else if (j_in) {
-                ex_take(jin + (long)(pos + sk) * TK * 5, xj, tb, err, failed, !g.refill);
+                ex_take(jin + (long)(pos + sk) * TK * 5, TK, xj, tb, err, failed, !g.refill);
             } else {
 #pragma unroll
                 for (int x = 0; x < 5; x++) tb[x] = pj[p ^ 1][x][lpj];
@@ -679,7 +681,7 @@ __device__ inline void lu_tiles_body(const SlGeom &g, int tile, double *__restri
 #pragma unroll
                 for (int x = 0; x < 5; x++) tc[x] = 0.0;
             } else if (k_in) {
-                ex_take(kin + (long)(pos + sj) * TJ * 5, xk, tc, err, failed, !g.refill);
+                ex_take(kin + (long)(pos + sj) * TJ * 5, TJ, xk, tc, err, failed, !g.refill);
             } else {
 #pragma unroll
                 for (int x = 0; x < 5; x++) tc[x] = pk[p ^ 1][x][lpk];
@@ -691,10 +693,17 @@ __device__ inline void lu_tiles_body(const SlGeom &g, int tile, double *__restri
                 for (int x = 0; x < 5; x++) q[x * qN + n] = op[kSlQ + x] + res[x];
             } else {
                 // the backward sweep leaves q + dq (its q operand plus the
-                // result, the update's one addition), the forward sweep dq*
+                // result, the update's one addition), the forward sweep dq*;
+                // stored as whole 16-byte pairs, the first one with the
+                // diagonal the lane read (its value unchanged)
+                double o[5];
 #pragma unroll
-                for (int x = 0; x < 5; x++)
-                    sl[g.slot(tile, row(s), kSlOut + x, l)] = DIR < 0 ? op[kSlQ + x] + res[x] : res[x];
+                for (int x = 0; x < 5; x++) o[x] = DIR < 0 ? op[kSlQ + x] + res[x] : res[x];
+                static_assert(kSlOut == 19 && kSlF == 24, "results are fields 19-23, after diag in field 18");
+                double2 *dst = reinterpret_cast<double2 *>(sl + g.slot(tile, row(s), kSlOut - 1, l));
+                dst[0] = make_double2(op[kSlOut - 1], o[0]);
+                dst[TT] = make_double2(o[1], o[2]);
+                dst[2 * TT] = make_double2(o[3], o[4]);
             }
 #pragma unroll
             for (int x = 0; x < 5; x++) {
@@ -704,11 +713,11 @@ __device__ inline void lu_tiles_body(const SlGeom &g, int tile, double *__restri
             }
             if (j_out) {
 #pragma unroll
-                for (int x = 0; x < 5; x++) xstore(jout + (long)(pos + sk) * TK * 5 + x, pb[x]);
+                for (int x = 0; x < 5; x++) xstore(jout + (long)(pos + sk) * TK * 5 + x * TK, pb[x]);
             }
             if (k_out) {
 #pragma unroll
-                for (int x = 0; x < 5; x++) xstore(kout + (long)(pos + sj) * TJ * 5 + x, pc[x]);
+                for (int x = 0; x < 5; x++) xstore(kout + (long)(pos + sj) * TJ * 5 + x * TJ, pc[x]);
             }
         }
         lds_barrier();

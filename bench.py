@@ -104,10 +104,30 @@ HYPERPLANE_BYTES = {  # the hyperplane sweeps read the operands in place
 }
 
 
-# HBM bytes per launch of the timed kernels on the default workload (TGV
-# 256^3, FP64, tile sweeps), from two rocprofv3 --pmc runs (FETCH_SIZE,
-# WRITE_SIZE) scaled by a 1 GiB copy; profiles/tgv256_fp64_hbm_traffic.md.
-MEASURED_TRAFFIC_CHANNEL = {}  # filled from the channel workload's counter runs
+# HBM bytes per launch of the timed kernels, from two rocprofv3 --pmc runs
+# (FETCH_SIZE, WRITE_SIZE) scaled by a 1 GiB copy read through the same
+# counters.  The channel with bump at FP64, batched stages and sweeps
+# (profiles/channel_bump_fp64_hbm_traffic.md); the sweeps' figures include
+# the refill of the exchange slots that shares their timer.
+MEASURED_TRAFFIC_CHANNEL = {
+    "getpri": 10.035e9,
+    "inviscid_i": 11.015e9,
+    "inviscid_j": 17.100e9,
+    "inviscid_k": 16.498e9,
+    "viscous_node_dt": 39.294e9,
+    "viscous_diff": 25.357e9,
+    "lusgs_gather": 27.423e9,
+    "lusgs_forward": 30.329e9,
+    "lusgs_backward": 30.325e9,
+    "lusgs_add": 9.223e9,
+    "sa_pre": 26.744e9,
+    "sa_pre_b": 31.727e9,
+    "sa_gather": 9.872e9,
+    "sa_forward": 6.170e9,
+    "sa_backward": 6.159e9,
+    "sa_add": 2.849e9,
+}
+# The TGV 256^3 block, FP64, tile sweeps (profiles/tgv256_fp64_hbm_traffic.md).
 
 MEASURED_TRAFFIC_256 = {
     # each sweep plus the refill of the exchange region (132 MB written)

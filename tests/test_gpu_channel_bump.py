@@ -7,6 +7,8 @@ replayed from a HIP graph with FP32 fluxes against block-by-block launches
 bit for bit."""
 import sys
 
+import os
+
 import numpy as np
 import pytest
 
@@ -105,21 +107,34 @@ def test_channel_bump_matches_oracle():
     assert np.allclose(res[:6], res_o[:6], rtol=1e-8, atol=0)
 
 
-def test_channel_bump_batched_graph_identical():
-    """FP32 fluxes, one launch per stage and sweep over all 40 blocks,
-    replayed from a graph, against block-by-block eager launches."""
+@pytest.mark.parametrize("precision", [0, 1])
+def test_channel_bump_batched_graph_identical(precision):
+    """One launch per stage and sweep over all 40 blocks, replayed from a
+    graph with the SA update beside the flow update on a second stream, and
+    eager with the two in sequence, against block-by-block eager launches:
+    the same bits, at FP64 and with FP32 fluxes."""
     nsteps = 5
-    say("test_channel_bump_batched_graph_identical steps", nsteps)
+    say("test_channel_bump_batched_graph_identical precision", precision, "steps", nsteps)
     out = []
-    for lusgs, batch, graph in ((1, 0, 0), (2, 1, 1)):
-        case = cases.channel_bump(**SMALL)
-        sol = start(case, lusgs=lusgs, batch=batch, graph=graph, precision=1)
+    for lusgs, batch, graph, conc in ((1, 0, 0, None), (2, 1, 1, "1"), (2, 1, 0, "0")):
+        old = os.environ.get("ACFD_SA_CONCURRENT")
+        if conc is not None:
+            os.environ["ACFD_SA_CONCURRENT"] = conc
+        try:
+            case = cases.channel_bump(**SMALL)
+            sol = start(case, lusgs=lusgs, batch=batch, graph=graph, precision=precision)
+        finally:
+            if old is None:
+                os.environ.pop("ACFD_SA_CONCURRENT", None)
+            else:
+                os.environ["ACFD_SA_CONCURRENT"] = old
         try:
             sol.step(nsteps)
             out.append([(sol.download(m + 1, "q"), sol.download(m + 1, "nut")) for m in range(len(case.dims))])
         finally:
             sol.close()
-    for m, ((qa, na), (qb, nb)) in enumerate(zip(*out)):
-        assert np.array_equal(qa, qb), (m, np.abs(qa - qb).max())
-        assert np.array_equal(na, nb), (m, np.abs(na - nb).max())
+    for other in out[1:]:
+        for m, ((qa, na), (qb, nb)) in enumerate(zip(out[0], other)):
+            assert np.array_equal(qa, qb), (m, np.abs(qa - qb).max())
+            assert np.array_equal(na, nb), (m, np.abs(na - nb).max())
     say("  identical over %d blocks" % len(out[0]))
