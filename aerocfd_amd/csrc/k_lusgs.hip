@@ -644,8 +644,14 @@ __device__ inline void lu_tiles_body(const SlGeom &g, int tile, double *__restri
     // slot and ignore it -- so the compiler can count outstanding loads
     // exactly and the prefetch stays in flight across the previous step.
     const int xj_last = g.Li + TK - 2, xk_last = g.Li + TJ - 2;
-    const double *jsrc = j_in ? jin : ex;
-    const double *ksrc = k_in ? kin : ex;
+    // In a tile with a neighbour on that side every lane loads from the
+    // face lanes' slots of the step (so one load instruction touches one
+    // contiguous run); otherwise from the start of the region.  Only the
+    // face lanes use what they load.
+    const bool j_pred = DIR > 0 ? bj > 0 : bj + 1 < g.nbj;
+    const bool k_pred = DIR > 0 ? bk > 0 : bk + 1 < g.nbk;
+    const double *jsrc = j_pred ? jin : ex;
+    const double *ksrc = k_pred ? kin : ex;
     auto fetch = [&](int s, double *op, unsigned long long *xj, unsigned long long *xk) {
         const int sc = min(s, g.nsteps - 1);
         const double2 *src = reinterpret_cast<const double2 *>(sl + g.slot(tile, row(sc), 0, l));
@@ -655,9 +661,9 @@ __device__ inline void lu_tiles_body(const SlGeom &g, int tile, double *__restri
             op[2 * f] = v.x;
             op[2 * f + 1] = v.y;
         }
-        const int pos = s - sj - sk;
-        const int dj = j_in ? min(max(pos + sk, 0), xj_last) : 0;
-        const int dk = k_in ? min(max(pos + sj, 0), xk_last) : 0;
+        // a face lane (sj == 0 / sk == 0) takes slot pos + sk = s / pos + sj = s
+        const int dj = j_pred ? min(max(s, 0), xj_last) : 0;
+        const int dk = k_pred ? min(max(s, 0), xk_last) : 0;
 #pragma unroll
         for (int x = 0; x < 5; x++) xj[x] = xload_bits(jsrc + (long)dj * TK * 5 + x * TK);
 #pragma unroll
@@ -903,8 +909,12 @@ __device__ inline void sa_tiles_body(const Ext &e, const SaGeom &g, int tile, co
     double *kin = exk + (long)(DIR > 0 ? tile - g.nbj : tile + g.nbj) * g.exk_tile + tj;
     double *jout = exj + (long)tile * g.exj_tile + tk;
     double *kout = exk + (long)tile * g.exk_tile + tj;
-    const double *jsrc = j_in ? jin : ex;
-    const double *ksrc = k_in ? kin : ex;
+    // every lane of a tile with a neighbour on that side loads the face
+    // lanes' slot run of the step; only the face lanes use it
+    const bool j_pred = DIR > 0 ? bj > 0 : bj + 1 < g.nbj;
+    const bool k_pred = DIR > 0 ? bk > 0 : bk + 1 < g.nbk;
+    const double *jsrc = j_pred ? jin : ex;
+    const double *ksrc = k_pred ? kin : ex;
     const int xj_last = g.Li + TK - 2, xk_last = g.Li + TJ - 2;
     const long N = e.n;
     // first node of the lane's line in sweep order (inactive lanes read a
@@ -924,9 +934,8 @@ __device__ inline void sa_tiles_body(const Ext &e, const SaGeom &g, int tile, co
         op[2] = cu[2 * N + n];
         op[3] = cu[3 * N + n];
         op[4] = src[n];
-        const int p = s - sj - sk;
-        const int dj = j_in ? min(max(p + sk, 0), xj_last) : 0;
-        const int dk = k_in ? min(max(p + sj, 0), xk_last) : 0;
+        const int dj = j_pred ? min(max(s, 0), xj_last) : 0;
+        const int dk = k_pred ? min(max(s, 0), xk_last) : 0;
         xj = xload_bits(jsrc + (long)dj * TK);
         xk = xload_bits(ksrc + (long)dk * TJ);
     };
@@ -1148,8 +1157,13 @@ __device__ inline void sa_sl_body(const SaGeom &g, int tile, double *__restrict_
     double *kin = exk + (long)(DIR > 0 ? tile - g.nbj : tile + g.nbj) * g.exk_tile + tj;
     double *jout = exj + (long)tile * g.exj_tile + tk;
     double *kout = exk + (long)tile * g.exk_tile + tj;
-    const double *jsrc = j_in ? jin : ex;
-    const double *ksrc = k_in ? kin : ex;
+    // every lane of a tile with a neighbour on that side loads the face
+    // lanes' slot run of the step (one contiguous run per load); only the
+    // face lanes use it
+    const bool j_pred = DIR > 0 ? bj > 0 : bj + 1 < g.nbj;
+    const bool k_pred = DIR > 0 ? bk > 0 : bk + 1 < g.nbk;
+    const double *jsrc = j_pred ? jin : ex;
+    const double *ksrc = k_pred ? kin : ex;
     const int xj_last = g.Li + TK - 2, xk_last = g.Li + TJ - 2;
     bool failed = false;
     double vi = 0.0;
@@ -1161,9 +1175,8 @@ __device__ inline void sa_sl_body(const SaGeom &g, int tile, double *__restrict_
         const double *p = L + row_of(s);
 #pragma unroll
         for (int f = 0; f < kSaOps; f++) op[f] = p[f * TT];
-        const int p0 = s - sj - sk;
-        const int dj = j_in ? min(max(p0 + sk, 0), xj_last) : 0;
-        const int dk = k_in ? min(max(p0 + sj, 0), xk_last) : 0;
+        const int dj = j_pred ? min(max(s, 0), xj_last) : 0;
+        const int dk = k_pred ? min(max(s, 0), xk_last) : 0;
         xj = xload_bits(jsrc + (long)dj * TK);
         xk = xload_bits(ksrc + (long)dk * TJ);
     };
