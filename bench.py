@@ -421,8 +421,9 @@ def main():
                          "scheme, the default); weno5_rk3 / wcns_rk3: Steger-Warming + WENO5 or WCNS with "
                          "explicit RK3 (TGV only)")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-ranks", type=int, default=8,
-                    help="MPI ranks of the reference's own executable in the CPU baseline (0: skip that run)")
+    ap.add_argument("--cpu-ranks", type=int, default=min(16, os.cpu_count() or 1),
+                    help="MPI ranks of the reference's own executable in the CPU baseline (0: skip that run; "
+                         "default: 16, the host cores one GPU's share holds, or fewer where the host has fewer)")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB (a known-size kernel that "
                          "calibrates HBM byte counters)")
