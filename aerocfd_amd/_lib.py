@@ -18,8 +18,9 @@ EXPORTS = (
     "acfd_residual", "acfd_stats", "acfd_sync", "acfd_timing", "acfd_getpri",
     "acfd_inviscid_rhs", "acfd_viscous_rhs", "acfd_find_dt", "acfd_lusgs", "acfd_bdf2",
     "acfd_sa_step", "acfd_exchange_coordinates", "acfd_sensor_weights", "acfd_sensor_collect",
-    "acfd_sensor_train", "acfd_sensor_evaluate", "acfd_sensor_enable", "acfd_sensor_stats", "acfd_mlp_grad",
-    "acfd_sensor_infer", "acfd_grid_metrics", "acfd_forces", "acfd_viscous_rhs_noalloc",
+    "acfd_sensor_train", "acfd_sensor_evaluate", "acfd_sensor_enable", "acfd_sensor_refresh",
+    "acfd_sensor_stats", "acfd_mlp_grad", "acfd_sensor_infer", "acfd_grid_metrics", "acfd_forces",
+    "acfd_viscous_rhs_noalloc",
 )
 
 

@@ -74,6 +74,7 @@ constexpr int kLastField = 27;
 
 constexpr int kTimeSlots = 32;  // acfd_timing entries
 constexpr int kSensorParams = 1505, kSensorFeatures = 12;  // the sensor network's size
+constexpr size_t kSensorStatBytes = (size_t)kSensorStatSlots * kSensorStatStride * sizeof(unsigned long long);
 
 struct Block {
     Ext e;
@@ -227,6 +228,11 @@ struct acfd_ctx {
     double s_bound = 1e-6;       // largest relative deviation an audit accepts
     std::vector<long> s_calls;   // inviscid evaluations per block
     std::vector<char> s_off;     // blocks that fell back to plain WCNS
+    // the network decides on every s_refresh-th evaluation of a block; the
+    // evaluations between take its decisions from the block's byte mask
+    // (3 n bytes, one per face and direction)
+    int s_refresh = 1;
+    std::vector<unsigned char *> s_mask;
     int s_audits = 0;
     double s_audit_max = 0.0;
     hipGraphExec_t graph_exec = nullptr;  // one captured step (use_graph)
@@ -746,8 +752,12 @@ int st_getpri(acfd_ctx *ctx, int m)
 int inviscid_with_sensor(acfd_ctx *ctx, int m)
 {
     Block &b = ctx->blk[m];
-    SensorArgs sa{ctx->s_net, ctx->s_thr, ctx->s_stats};
     const long call = ctx->s_calls[m]++;
+    SensorArgs sa{ctx->s_net, ctx->s_thr, ctx->s_stats, nullptr, 0};
+    if (ctx->s_refresh > 1) {
+        sa.mask = ctx->s_mask[m];
+        sa.replay = call % ctx->s_refresh != 0;
+    }
     if (ctx->s_audit > 0 && call % ctx->s_audit == 0) {
         KRN_TRY(launch_inviscid_ext(ctx->stream, b.e, b.f, ctx->c, ctx->cfg.precision, 2));
         KRN_TRY(launch_copy(ctx->stream, b.f.rhsi, b.f.w_flux, 5 * b.e.n));
@@ -1556,6 +1566,8 @@ int acfd_destroy(acfd_ctx *ctx)
                   ctx->s_X, ctx->s_Y, ctx->s_stats, ctx->s_dpart};
     for (void *p : sp)
         if (p) (void)hipFree(p);
+    for (unsigned char *p : ctx->s_mask)
+        if (p) (void)hipFree(p);
     void *ptrs[] = {ctx->d_cfl, ctx->d_stats, ctx->d_nneg, ctx->d_res, ctx->d_partial,
                     ctx->d_stage, ctx->d_qtab, ctx->d_nuttab, ctx->d_ntab, ctx->d_h_sb, ctx->d_h_db,
                     ctx->d_h_sn, ctx->d_h_dn, ctx->d_a_sb, ctx->d_a_gb, ctx->d_a_gs, ctx->d_a_sn,
@@ -1856,9 +1868,10 @@ int acfd_step(acfd_ctx *ctx, int nsteps)
     int s = 0;
     // graph mode: the first step runs eagerly (it also tunes the tile
     // sweeps), the next one is captured, and every later step replays it;
-    // audited sensor steps read results on the host, and steps under the
-    // per-stage timers record events between launches: both stay eager
-    const bool graph = ctx->cfg.use_graph && !ctx->timing && !(ctx->s_on && ctx->s_audit > 0);
+    // audited sensor steps read results on the host, sensor steps that reuse
+    // decisions alternate between two kernels, and steps under the
+    // per-stage timers record events between launches: all stay eager
+    const bool graph = ctx->cfg.use_graph && !ctx->timing && !(ctx->s_on && (ctx->s_audit > 0 || ctx->s_refresh > 1));
     if (ctx->graph_exec && ctx->graph_geo_cached &&
         (ctx->sl_block != ctx->graph_sl_in[0] || ctx->sl_shape != ctx->graph_sl_in[1]))
         drop_graph(ctx);  // the sweep layout holds other grid operands than at capture
@@ -2092,8 +2105,8 @@ int sensor_ready(acfd_ctx *ctx)
     HIP_TRY(hipMalloc((void **)&ctx->s_grad, pb));
     HIP_TRY(hipMalloc((void **)&ctx->s_loss, sizeof(float)));
     HIP_TRY(hipMalloc((void **)&ctx->s_net, sensor_net_bytes()));
-    HIP_TRY(hipMalloc((void **)&ctx->s_stats, 2 * sizeof(unsigned long long)));
-    HIP_TRY(hipMemsetAsync(ctx->s_stats, 0, 2 * sizeof(unsigned long long), ctx->stream));
+    HIP_TRY(hipMalloc((void **)&ctx->s_stats, kSensorStatBytes));
+    HIP_TRY(hipMemsetAsync(ctx->s_stats, 0, kSensorStatBytes, ctx->stream));
     HIP_TRY(hipMemsetAsync(ctx->s_m, 0, pb, ctx->stream));
     HIP_TRY(hipMemsetAsync(ctx->s_v, 0, pb, ctx->stream));
     // He initialisation from a fixed generator (64-bit LCG + Box-Muller)
@@ -2260,8 +2273,23 @@ int acfd_sensor_enable(acfd_ctx *ctx, int on, float threshold, int audit_interva
     ctx->s_off.assign(ctx->blk.size(), 0);
     ctx->s_audits = 0;
     ctx->s_audit_max = 0.0;
-    HIP_TRY(hipMemsetAsync(ctx->s_stats, 0, 2 * sizeof(unsigned long long), ctx->stream));
+    HIP_TRY(hipMemsetAsync(ctx->s_stats, 0, kSensorStatBytes, ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+int acfd_sensor_refresh(acfd_ctx *ctx, int interval)
+{
+    if (!ctx || !ctx->topo || interval < 1) return fail(ACFD_EINVAL, "acfd_sensor_refresh: bad arguments");
+    drop_graph(ctx);
+    if (interval > 1) {
+        ctx->s_mask.resize(ctx->blk.size(), nullptr);
+        for (size_t m = 0; m < ctx->blk.size(); m++)
+            if (!ctx->s_mask[m]) HIP_TRY(hipMalloc((void **)&ctx->s_mask[m], 3 * (size_t)ctx->blk[m].e.n));
+    }
+    ctx->s_refresh = interval;
+    // the next evaluation of every block runs the network
+    ctx->s_calls.assign(ctx->blk.size(), 0);
     return 0;
 }
 
@@ -2270,8 +2298,13 @@ int acfd_sensor_stats(acfd_ctx *ctx, double *out)
     if (!ctx || !out) return fail(ACFD_EINVAL, "acfd_sensor_stats: bad arguments");
     unsigned long long st[2] = {0, 0};
     if (ctx->s_stats) {
+        std::vector<unsigned long long> h(kSensorStatBytes / sizeof(unsigned long long));
         HIP_TRY(hipStreamSynchronize(ctx->stream));
-        HIP_TRY(hipMemcpy(st, ctx->s_stats, sizeof(st), hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(h.data(), ctx->s_stats, kSensorStatBytes, hipMemcpyDeviceToHost));
+        for (int k = 0; k < kSensorStatSlots; k++) {
+            st[0] += h[(size_t)k * kSensorStatStride];
+            st[1] += h[(size_t)k * kSensorStatStride + 1];
+        }
     }
     long off = 0;
     for (char c : ctx->s_off) off += c;

@@ -216,7 +216,17 @@ __device__ inline double ext_diff(const R (*FR)[5], int x, int dord)
 // j and k directions: each thread marches along the axis over kExtRun nodes
 // with rolling windows of node quantities and face fluxes.
 constexpr int kExtRun = 32;
-template <typename R, int AXIS, int SCHEME, bool SENS>
+// one wavefront's face counts into its slot (called by one lane)
+__device__ inline void sensor_count(unsigned long long *stats, unsigned long long ne, unsigned long long nl)
+{
+    const unsigned slot = (blockIdx.x * 31u + blockIdx.y * 17u + blockIdx.z * 7u + threadIdx.y) % kSensorStatSlots;
+    atomicAdd(&stats[slot * kSensorStatStride], ne);
+    atomicAdd(&stats[slot * kSensorStatStride + 1], nl);
+}
+
+// SENS: 0 plain; 1 the network decides per face (and records the decisions
+// in sa.mask when given); 2 the decisions are read from sa.mask
+template <typename R, int AXIS, int SCHEME, int SENS>
 __global__ void __launch_bounds__(256) k_ext_jk(Ext e, const double *__restrict__ q,
                                                const double *__restrict__ dxidx,
                                                const double *__restrict__ inv_j, double *__restrict__ rhsi,
@@ -228,20 +238,21 @@ __global__ void __launch_bounds__(256) k_ext_jk(Ext e, const double *__restrict_
     const int a0 = e.lo1(AXIS) + (int)blockIdx.z * kExtRun;
     const int a1 = min(a0 + kExtRun - 1, e.hi1(AXIS));
     const bool active = ir <= e.ie1 && (AXIS == 1 ? e.ks1 + orr <= e.ke1 : e.js1 + orr <= e.je1);
-    // with the sensor every lane takes part in the wavefront-wide network;
-    // lanes outside the range work on a valid line and store nothing
-    __shared__ typename std::conditional<SENS, SensorLds, char>::type slds;
-    if constexpr (SENS) {
+    // with the sensor every lane takes part (in the wavefront-wide network,
+    // and in the wavefront's sum of its face counts); lanes outside the
+    // range work on a valid line and store nothing
+    __shared__ typename std::conditional<SENS == 1, SensorLds, char>::type slds;
+    if constexpr (SENS == 1) {
         sensor_load_net(sa.net, slds);
         __syncthreads();
-    } else {
+    } else if constexpr (SENS == 0) {
         if (!active) return;
     }
     const int i = min(ir, e.ie1);
     const int o = min(orr, AXIS == 1 ? e.ke1 - e.ks1 : e.je1 - e.js1);
     const int j = AXIS == 1 ? e.lo(1) : e.js1 + o;
     const int k = AXIS == 2 ? e.lo(2) : e.ks1 + o;
-    unsigned long long n_eval = 0, n_lin = 0;
+    unsigned n_eval = 0, n_lin = 0;
     const R gamma = (R)gamma_d;
     const long N = e.n, st = e.stride(AXIS);
     const int lo = e.lo(AXIS), hi = e.hi(AXIS);
@@ -278,18 +289,24 @@ __global__ void __launch_bounds__(256) k_ext_jk(Ext e, const double *__restrict_
 #pragma unroll
             for (int x = 0; x < 5; x++) FR[f][x] = FR[f + 1][x];
         const int ord = face_order(b, lo, hi);
+        const bool face = b >= lo && b + 1 <= hi;
         bool lin = false;
-        if constexpr (SENS) {
+        if constexpr (SENS == 1) {
             float feat[kSensorNF];
             sensor_features<R>(W, gamma_d, feat);
             const float p = sensor_eval_wave(slds, threadIdx.y, feat);
             lin = p < sa.threshold;
+            if (sa.mask && active && face) sa.mask[AXIS * N + nat(b)] = lin ? 1 : 0;
+        } else if constexpr (SENS == 2) {
+            lin = face && sa.mask[AXIS * N + nat(b)] != 0;
+        }
+        if constexpr (SENS != 0) {
             if (active && ord == 5) {
                 n_eval++;
                 n_lin += lin ? 1 : 0;
             }
         }
-        if (b >= lo && b + 1 <= hi) {
+        if (face) {
             R m[3] = {(R)0, (R)0, (R)0};
             if (SCHEME == 2) face_metric_r<R>(dxidx, inv_j, N, nat(b), st, c0, m);
             ext_face<R, SCHEME, NQ>(W, ord, m, gamma, FR[5], lin);
@@ -303,11 +320,13 @@ __global__ void __launch_bounds__(256) k_ext_jk(Ext e, const double *__restrict_
                 rhsi[x * N + n] = rhsi[x * N + n] - ext_diff<R, SCHEME, SCHEME == 1 ? 5 : 3>(FR, x, dord);
         }
     }
-    if constexpr (SENS) {
-        if (n_eval) {
-            atomicAdd(&sa.stats[0], n_eval);
-            atomicAdd(&sa.stats[1], n_lin);
+    if constexpr (SENS != 0) {
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            n_eval += __shfl_xor(n_eval, o);
+            n_lin += __shfl_xor(n_lin, o);
         }
+        if (threadIdx.x == 0 && n_eval) sensor_count(sa.stats, n_eval, n_lin);
     }
 }
 
@@ -319,7 +338,7 @@ template <int SCHEME> struct ExtI {
     static constexpr int NPW = SCHEME == 1 ? 58 : 54;
 };
 
-template <typename R, int SCHEME, bool SENS>
+template <typename R, int SCHEME, int SENS>
 __global__ void __launch_bounds__(256) k_ext_i(Ext e, const double *__restrict__ q,
                                               const double *__restrict__ dxidx,
                                               const double *__restrict__ inv_j, double *__restrict__ rhsi,
@@ -328,8 +347,8 @@ __global__ void __launch_bounds__(256) k_ext_i(Ext e, const double *__restrict__
     constexpr int NQ = SCHEME == 1 ? 10 : 5;
     __shared__ R nq[4][NQ][64];
     __shared__ R fq[4][5][64];
-    __shared__ typename std::conditional<SENS, SensorLds, char>::type slds;
-    if constexpr (SENS) sensor_load_net(sa.net, slds);
+    __shared__ typename std::conditional<SENS == 1, SensorLds, char>::type slds;
+    if constexpr (SENS == 1) sensor_load_net(sa.net, slds);
     const int lane = threadIdx.x, row = threadIdx.y;
     const int jr = e.js1 + (int)(blockIdx.y * blockDim.y + threadIdx.y);
     const bool row_ok = jr <= e.je1;
@@ -359,17 +378,19 @@ __global__ void __launch_bounds__(256) k_ext_i(Ext e, const double *__restrict__
             for (int x = 0; x < NQ; x++) W[w][x] = nq[row][x][min(max(lane - 2 + w, 0), 63)];
         const int ord = face_order(p, lo, hi);
         bool lin = false;
-        if constexpr (SENS) {
+        if constexpr (SENS == 1) {
             float feat[kSensorNF];
             sensor_features<R>(W, gamma_d, feat);
             const float pr = sensor_eval_wave(slds, row, feat);
             lin = pr < sa.threshold;
+            if (sa.mask && face_ok && row_ok) sa.mask[n] = lin ? 1 : 0;
+        } else if constexpr (SENS == 2) {
+            lin = face_ok && sa.mask[n] != 0;
+        }
+        if constexpr (SENS != 0) {
             const bool counted = face_ok && ord == 5 && row_ok && p >= e.is1 - 1 && p <= e.ie1;
             const unsigned long long ne = __popcll(__ballot(counted)), nl = __popcll(__ballot(counted && lin));
-            if (lane == 0 && ne) {
-                atomicAdd(&sa.stats[0], ne);
-                atomicAdd(&sa.stats[1], nl);
-            }
+            if (lane == 0 && ne) sensor_count(sa.stats, ne, nl);
         }
         if (face_ok) {
             R m[3] = {(R)0, (R)0, (R)0};
@@ -396,7 +417,7 @@ __global__ void __launch_bounds__(256) k_ext_i(Ext e, const double *__restrict__
     }
 }
 
-template <typename R, int SCHEME, bool SENS = false>
+template <typename R, int SCHEME, int SENS = 0>
 static void launch_ext_t(hipStream_t s, const Ext &e, const Fields &f, double gamma, SensorArgs sa = {})
 {
     const int Li = e.ie1 - e.is1 + 1, Lj = e.je1 - e.js1 + 1, Lk = e.ke1 - e.ks1 + 1;
@@ -415,8 +436,13 @@ int launch_inviscid_wcns_sensor(hipStream_t s, const Ext &e, const Fields &f, co
     int rc = launch_fill(s, f.rhsi, 5 * e.n, 0.0);
     if (rc) return rc;
     if (e.ie1 < e.is1 || e.je1 < e.js1 || e.ke1 < e.ks1) return 0;
-    if (precision == 1) launch_ext_t<float, 2, true>(s, e, f, c.gamma, sa);
-    else launch_ext_t<double, 2, true>(s, e, f, c.gamma, sa);
+    if (sa.replay) {
+        if (precision == 1) launch_ext_t<float, 2, 2>(s, e, f, c.gamma, sa);
+        else launch_ext_t<double, 2, 2>(s, e, f, c.gamma, sa);
+    } else {
+        if (precision == 1) launch_ext_t<float, 2, 1>(s, e, f, c.gamma, sa);
+        else launch_ext_t<double, 2, 1>(s, e, f, c.gamma, sa);
+    }
     ACFD_CHECK_LAUNCH();
     return 0;
 }

@@ -25,10 +25,19 @@ int launch_copy(hipStream_t s, const double *a, double *b, long n);
 
 // k_sensor.hip: learned WCNS sensor (training, data, inference)
 struct SensorNet;
+// the sensor's face counts are spread over slots on separate cache lines
+// (one add per wavefront, to the slot of its workgroup and row), so that the
+// adds of many wavefronts do not queue on one address
+constexpr int kSensorStatSlots = 64, kSensorStatStride = 16;  // stride in counters (128 bytes)
 struct SensorArgs {            // passed to the WCNS kernels when the sensor is on
     const SensorNet *net;
     float threshold;           // use the linear weights where p < threshold
-    unsigned long long *stats; // [0] faces evaluated, [1] faces given linear weights
+    unsigned long long *stats; // per slot: [0] faces evaluated, [1] faces given linear weights
+    // the block's decisions, one byte per face (1: linear weights) at
+    // mask[d * n + node] for the face between node and its successor along
+    // direction d; recorded when the network runs, read back when it does not
+    unsigned char *mask;
+    int replay;                // 1: take the decisions from mask instead of the network
 };
 int launch_mlp_grad(hipStream_t s, const float *w, const float *X, const float *Y, long n, long first, long count,
                     long stride, long offset, float *part, float *lpart, float *grad, float *loss);

@@ -235,6 +235,11 @@ class Solver:
         _lib.check(self.lib.acfd_sensor_enable(self.h, C.c_int(int(on)), C.c_float(threshold),
                                                C.c_int(audit_interval), C.c_double(audit_bound)))
 
+    def sensor_refresh(self, interval=1):
+        """Run the network on every interval-th WCNS evaluation of a block
+        and reuse its recorded per-face decisions in between."""
+        _lib.check(self.lib.acfd_sensor_refresh(self.h, C.c_int(interval)))
+
     def sensor_stats(self):
         out = np.zeros(6)
         _lib.check(self.lib.acfd_sensor_stats(self.h, _dp(out)))

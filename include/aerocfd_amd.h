@@ -266,6 +266,12 @@ int acfd_sensor_evaluate(acfd_ctx *ctx, float threshold, double *out);
  * deviation of the residual above audit_bound makes that block fall back
  * to plain WCNS for good (audit_interval 0: no audits) */
 int acfd_sensor_enable(acfd_ctx *ctx, int on, float threshold, int audit_interval, double audit_bound);
+/* reuse the network's decisions: it runs on every interval-th WCNS
+ * evaluation of a block, which records its per-face decisions in a byte
+ * mask of the block (3 bytes per node); the evaluations between read the
+ * mask instead (interval 1, the default: the network runs every time).
+ * The next evaluation of every block runs the network. */
+int acfd_sensor_refresh(acfd_ctx *ctx, int interval);
 /* out[6]: faces evaluated, faces given linear weights, audits, largest
  * audit deviation, blocks fallen back, 0 */
 int acfd_sensor_stats(acfd_ctx *ctx, double *out);

@@ -147,3 +147,32 @@ def test_sensor_cost_on_smooth_flow():
         assert dev <= max(1e-12, 10.0 * share)
     finally:
         sol.close()
+
+
+def test_sensor_decision_reuse_identical():
+    """Decisions recorded by the network and read back from the byte mask
+    give the same residual to the last bit: with refresh interval 3 the
+    first inviscid evaluation runs the network (and records), the next two
+    read the mask, all on the same state; the network run every time
+    (interval 1) gives the same bits too."""
+    say("test_sensor_decision_reuse_identical 128x64x4 Mach 2.9 WCNS")
+    case = cases.shock_vortex(128, 64, 4, mach=2.9, seed=4)
+    sol, _, _ = run(case, 5)
+    try:
+        sol.sensor_collect(stride=1)
+        sol.sensor_train(200, batch=4096, lr=3e-3, seed=3)
+        rhs = {}
+        for interval in (1, 3):
+            sol.sensor_enable(True, 0.5, 0, 1.0)
+            sol.sensor_refresh(interval)
+            for call in range(3):
+                sol.run_stage("inviscid")
+                rhs[(interval, call)] = sol.download(1, "rhsi")
+            st = sol.sensor_stats()
+            say("  interval %d: faces %d, linear %d" % (interval, st["faces"], st["linear"]))
+            assert 0 < st["linear"] < st["faces"]
+    finally:
+        sol.close()
+    ref = rhs[(1, 0)]
+    for key, r in rhs.items():
+        assert np.array_equal(r, ref), key
