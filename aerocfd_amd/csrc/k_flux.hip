@@ -351,12 +351,13 @@ int launch_inviscid(hipStream_t s, const Ext &e, const Fields &f, const Consts &
 
 // ------------------------------------------------------------------ viscous
 // Node difference of sub_viscous.f90:107-178: one-sided at the physical
-// range ends, central inside.
-__device__ inline double ndiff(const double *a, long n, long st, int x, int x0, int x1)
+// range ends, central inside; in FP64 whatever the storage type S.
+template <typename S>
+__device__ inline double ndiff(const S *a, long n, long st, int x, int x0, int x1)
 {
-    if (x == x0) return a[n + st] - a[n];
-    if (x == x1) return a[n] - a[n - st];
-    return (a[n + st] - a[n - st]) / 2.0;
+    if (x == x0) return (double)a[n + st] - (double)a[n];
+    if (x == x1) return (double)a[n] - (double)a[n - st];
+    return ((double)a[n + st] - (double)a[n - st]) / 2.0;
 }
 
 // Stage 1 (sub_viscous.f90:107-293) at node (i,j,k) of is0..ie0: velocity
@@ -442,11 +443,14 @@ __device__ inline void viscous_node_store(long N, long n, R am1, R am3, const R 
     }
 }
 
+// The node fluxes are stored in the arithmetic type R (FP32 in the FP32-flux
+// variant, where they are FP32 values already): w_flux holds 12 components
+// of R at component stride n.
 template <typename R>
 __device__ inline void viscous_node_body(const Ext &e, long wg, const Consts &c,
                                          const double *__restrict__ pri, double *__restrict__ amu,
                                          const double *__restrict__ dxidx, const double *__restrict__ inv_j,
-                                         double *__restrict__ w, double *__restrict__ vor, double *__restrict__ dudx,
+                                         R *__restrict__ w, double *__restrict__ vor, double *__restrict__ dudx,
                                          double *__restrict__ tao, int store_aux)
 {
     const Box b{e.is0, e.ie0, e.js0, e.je0, e.ks0, e.ke0};
@@ -459,7 +463,7 @@ __device__ inline void viscous_node_body(const Ext &e, long wg, const Consts &c,
     viscous_node_eval<R>(e, i, j, k, c, pri, amu, dxidx, inv_j, w12, am1, am3, du, ta);
     viscous_node_store<R>(N, n, am1, am3, du, ta, amu, vor, dudx, tao, store_aux);
 #pragma unroll
-    for (int x = 0; x < 12; x++) w[x * N + n] = w12[x];
+    for (int x = 0; x < 12; x++) w[x * N + n] = (R)w12[x];
 }
 
 template <typename R>
@@ -469,14 +473,15 @@ __global__ void __launch_bounds__(256) k_viscous_node(Ext e, Consts c, const dou
                                                       double *__restrict__ vor, double *__restrict__ dudx,
                                                       double *__restrict__ tao, int store_aux)
 {
-    viscous_node_body<R>(e, xcd_slab(blockIdx.x, gridDim.x), c, pri, amu, dxidx, inv_j, w, vor, dudx, tao,
-                         store_aux);
+    viscous_node_body<R>(e, xcd_slab(blockIdx.x, gridDim.x), c, pri, amu, dxidx, inv_j, reinterpret_cast<R *>(w),
+                         vor, dudx, tao, store_aux);
 }
 
 // Stage 2 (sub_viscous.f90:296-365): rhsv = (d/dxi fv + d/deta gv + d/dzeta hv)/re;
 // with rhsi given, also rhs = rhsi + rhsv on the computed range (the
 // first line of implicit1st, sub_timeadvancement_1stimplicit.f90:72-78)
-__device__ inline void viscous_diff_body(const Ext &e, long wg, const double *__restrict__ w,
+template <typename S>
+__device__ inline void viscous_diff_body(const Ext &e, long wg, const S *__restrict__ w,
                                          double *__restrict__ rhsv, double re, const double *__restrict__ rhsi = nullptr,
                                          double *__restrict__ rhs = nullptr)
 {
@@ -501,10 +506,12 @@ __device__ inline void viscous_diff_body(const Ext &e, long wg, const double *__
     }
 }
 
+// S: the storage type of the node fluxes
+template <typename S>
 __global__ void __launch_bounds__(256) k_viscous_diff(Ext e, const double *__restrict__ w,
                                                       double *__restrict__ rhsv, double re)
 {
-    viscous_diff_body(e, xcd_slab(blockIdx.x, gridDim.x), w, rhsv, re);
+    viscous_diff_body<S>(e, xcd_slab(blockIdx.x, gridDim.x), reinterpret_cast<const S *>(w), rhsv, re);
 }
 
 template <typename R>
@@ -517,10 +524,13 @@ __global__ void __launch_bounds__(256) k_viscous_node_batch(const BlockRec *__re
     const BlockRec &b = blks[batch_wg(map, kg, bx, by, bz)];
     const Ext e = b.e;
     viscous_node_body<R>(e, bx, c, arena + b.o.pri, arena + b.o.amu, arena + b.o.dxidx, arena + b.o.inv_j,
-                         arena + b.o.w_flux, arena + b.o.vor, arena + b.o.dudx, arena + b.o.tao, store_aux);
+                         reinterpret_cast<R *>(arena + b.o.w_flux), arena + b.o.vor, arena + b.o.dudx, arena + b.o.tao,
+                         store_aux);
 }
 
-// with_rhs: also rhs = rhsi + rhsv on the computed range
+// with_rhs: also rhs = rhsi + rhsv on the computed range; S: the storage
+// type of the node fluxes
+template <typename S>
 __global__ void __launch_bounds__(256) k_viscous_diff_batch(const BlockRec *__restrict__ blks,
                                                             const int *__restrict__ map,
                                                             const KGrid *__restrict__ kg, double *arena, double re,
@@ -529,10 +539,9 @@ __global__ void __launch_bounds__(256) k_viscous_diff_batch(const BlockRec *__re
     int bx, by, bz;
     const BlockRec &b = blks[batch_wg(map, kg, bx, by, bz)];
     const Ext e = b.e;
-    if (with_rhs)
-        viscous_diff_body(e, bx, arena + b.o.w_flux, arena + b.o.rhsv, re, arena + b.o.rhsi, arena + b.o.rhs);
-    else
-        viscous_diff_body(e, bx, arena + b.o.w_flux, arena + b.o.rhsv, re);
+    const S *w = reinterpret_cast<const S *>(arena + b.o.w_flux);
+    if (with_rhs) viscous_diff_body<S>(e, bx, w, arena + b.o.rhsv, re, arena + b.o.rhsi, arena + b.o.rhs);
+    else viscous_diff_body<S>(e, bx, w, arena + b.o.rhsv, re);
 }
 
 // per-block grid of both viscous kernels: linear tiles of 256 nodes over the
@@ -549,7 +558,8 @@ int launch_viscous(hipStream_t s, const Ext &e, const Fields &f, const Consts &c
     else
         hipLaunchKernelGGL((k_viscous_node<double>), g, t, 0, s, e, c, f.pri, f.amu, f.dxidx, f.inv_j, f.w_flux,
                            f.vor, f.dudx, f.tao, store_aux);
-    hipLaunchKernelGGL(k_viscous_diff, g, t, 0, s, e, f.w_flux, f.rhsv, c.re);
+    if (precision == 1) hipLaunchKernelGGL(k_viscous_diff<float>, g, t, 0, s, e, f.w_flux, f.rhsv, c.re);
+    else hipLaunchKernelGGL(k_viscous_diff<double>, g, t, 0, s, e, f.w_flux, f.rhsv, c.re);
     ACFD_CHECK_LAUNCH();
     return 0;
 }
@@ -565,8 +575,12 @@ int launch_viscous_batch(hipStream_t s, const BlockRec *blks, const BatchGrid &b
         else
             hipLaunchKernelGGL((k_viscous_node_batch<double>), g, t, 0, s, blks, bg.map, bg.kg, arena, c, store_aux);
     }
-    if (part < 0 || part == 2)
-        hipLaunchKernelGGL(k_viscous_diff_batch, g, t, 0, s, blks, bg.map, bg.kg, arena, c.re);
+    if (part < 0 || part == 2) {
+        if (precision == 1)
+            hipLaunchKernelGGL(k_viscous_diff_batch<float>, g, t, 0, s, blks, bg.map, bg.kg, arena, c.re, 0);
+        else
+            hipLaunchKernelGGL(k_viscous_diff_batch<double>, g, t, 0, s, blks, bg.map, bg.kg, arena, c.re, 0);
+    }
     ACFD_CHECK_LAUNCH();
     return 0;
 }
@@ -734,9 +748,9 @@ __global__ void __launch_bounds__(256) k_viscous_node_dt_batch(const BlockRec *_
             viscous_node_eval<R>(e, i, j, k, c, pri, amu, dxidx, arena + b.o.inv_j, w12, am1, am3, du, ta);
             viscous_node_store<R>(N, n, am1, am3, du, ta, amu, arena + b.o.vor, arena + b.o.dudx, arena + b.o.tao,
                                   store_aux);
-            double *w = arena + b.o.w_flux;
+            R *w = reinterpret_cast<R *>(arena + b.o.w_flux);
 #pragma unroll
-            for (int x = 0; x < 12; x++) w[x * N + n] = w12[x];
+            for (int x = 0; x < 12; x++) w[x * N + n] = (R)w12[x];
             a3 = (double)am3;
         } else {
             a3 = amu[2 * N + n];
@@ -774,8 +788,12 @@ int launch_viscous_dt_batch(hipStream_t s, const BlockRec *blks, int nblk, const
         if (prep) viscous_dt_node<true>(s, blks, nblk, bg, arena, c, precision, store_aux, cfl, stats);
         else viscous_dt_node<false>(s, blks, nblk, bg, arena, c, precision, store_aux, cfl, stats);
     }
-    if (part < 0 || part == 2)
-        hipLaunchKernelGGL(k_viscous_diff_batch, g, t, 0, s, blks, bg.map, bg.kg, arena, c.re, prep);
+    if (part < 0 || part == 2) {
+        if (precision == 1)
+            hipLaunchKernelGGL(k_viscous_diff_batch<float>, g, t, 0, s, blks, bg.map, bg.kg, arena, c.re, prep);
+        else
+            hipLaunchKernelGGL(k_viscous_diff_batch<double>, g, t, 0, s, blks, bg.map, bg.kg, arena, c.re, prep);
+    }
     ACFD_CHECK_LAUNCH();
     return 0;
 }
