@@ -75,27 +75,39 @@ __device__ inline bool box_node(const Box &b, int &i, int &j, int &k) {
     return i <= b.i1 && j <= b.j1 && k <= b.k1;
 }
 
-// implicit1st's spectral radii of node n, |grad xi_d . u| + |c |grad xi_d||
-// (sub_timeadvancement_1stimplicit.f90:110-150), stored at rr[d*N + n], and
-// its diagonal phi/dt + r_0 + r_1 + r_2 (:151), returned.  Never contracted,
-// so every kernel that forms them (the update's own preparation pass, or
-// the residual's viscous node pass) gives the same bits.
+// implicit1st's spectral radius of one direction at a node,
+// |grad xi . u| + |c |grad xi|| (sub_timeadvancement_1stimplicit.f90:110-150),
+// from the metric row m[0..2], the velocity and the sound speed, and the
+// diagonal phi/dt + r_0 + r_1 + r_2 (:151) from the time term phi*1/dt.
+// Never contracted, so every kernel that forms them (the preparation pass,
+// the residual's viscous node pass, the tile sweeps) gives the same bits.
+__device__ inline double lu_radius(double m0, double m1, double m2, double u, double v, double w, double c)
+{
+#pragma clang fp contract(off)
+    double temp = m0 * m0 + m1 * m1 + m2 * m2;
+    temp = sqrt(temp);
+    const double cu = m0 * u + m1 * v + m2 * w;
+    return fabs(cu) + fabs(c * temp);
+}
+
+__device__ inline double lu_diag(double tterm, const double *r)
+{
+#pragma clang fp contract(off)
+    return tterm + r[0] + r[1] + r[2];
+}
+
+// the radii of node n into rr[d * N + n] and its diagonal, returned
 __device__ inline double lu_radii_diag(const double *pri, const double *dxidx, long N, long n, double dt, double phi,
                                        double *rr)
 {
-#pragma clang fp contract(off)
     const double u = pri[N + n], v = pri[2 * N + n], w = pri[3 * N + n], c = pri[6 * N + n];
     double r3[3];
 #pragma unroll
     for (int d = 0; d < 3; d++) {
-        const double m0 = dxidx[(3 * d) * N + n], m1 = dxidx[(3 * d + 1) * N + n], m2 = dxidx[(3 * d + 2) * N + n];
-        double temp = m0 * m0 + m1 * m1 + m2 * m2;
-        temp = sqrt(temp);
-        const double cu = m0 * u + m1 * v + m2 * w;
-        r3[d] = fabs(cu) + fabs(c * temp);
+        r3[d] = lu_radius(dxidx[(3 * d) * N + n], dxidx[(3 * d + 1) * N + n], dxidx[(3 * d + 2) * N + n], u, v, w, c);
         rr[d * N + n] = r3[d];
     }
-    return phi * 1.0 / dt + r3[0] + r3[1] + r3[2];
+    return lu_diag(phi * 1.0 / dt, r3);
 }
 
 }  // namespace acfd

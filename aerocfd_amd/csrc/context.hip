@@ -994,11 +994,11 @@ int tune_batch(acfd_ctx *ctx)
 // cfl its find_dt left (cfl + m), or the global one when cfl_per_block is
 // null.
 //
-// The batched update never clears dq (nor writes w_dqs / w_dq at all: its
-// sweeps keep dq* and dq in the sweep layout).  The SA update uses those two
-// work fields for its cu and dnut, and runs beside this update on a second
-// stream only because of that.
-constexpr int kBatchedPrepClearsDq = 0;
+// The batched update never writes w_dqs / w_dq (its preparation does not
+// clear them, and its sweeps keep dq* and dq in the sweep layout).  The SA
+// update uses those two work fields for its cu and dnut, and runs beside
+// this update on a second stream only because of that.
+constexpr bool kBatchedUpdateWritesDq = false;
 int st_implicit_all(acfd_ctx *ctx, const double *cfl_per_block)
 {
     const int nb = (int)ctx->blk.size();
@@ -1017,8 +1017,7 @@ int st_implicit_all(acfd_ctx *ctx, const double *cfl_per_block)
     } else if (batched(ctx)) {
         StageTimer ts(ctx, kTimeLuOther);
         KRN_TRY(launch_lu_prep_batch(ctx->stream, ctx->d_blocks, ctx->bg_prep, ctx->arena,
-                                     cfl_per_block ? cfl_per_block : ctx->d_cfl, cfl_per_block ? 1 : 0,
-                                     kBatchedPrepClearsDq));
+                                     cfl_per_block ? cfl_per_block : ctx->d_cfl, cfl_per_block ? 1 : 0));
     } else {
         StageTimer ts(ctx, kTimeLuOther);
         for (int m = 0; m < nb; m++) {
@@ -1284,8 +1283,8 @@ int st_viscous_dt_all(acfd_ctx *ctx, int store_aux)
     StageTimer t(ctx, ACFD_STAGE_VISCOUS);
     KRN_TRY(launch_reset_stats(ctx->stream, ctx->d_stats));
     const int nb = (int)ctx->blk.size();
-    // the batched LU-SGS update's spectral radii, diagonal and rhs come from
-    // these two passes too (implicit1st then skips its own preparation)
+    // the batched LU-SGS update's time term and rhs come from these two
+    // passes too (implicit1st then skips its own preparation)
     const int prep = ctx->cfg.lusgs_kernel == 2 && ctx->cfg.iflag_timeadvance == 2 ? 1 : 0;
     if (ctx->timing) {
         for (int part = 1; part <= 2; part++) {
@@ -1352,7 +1351,7 @@ int one_step(acfd_ctx *ctx)
         ctx->graph_needs_pri = true;
     if (ctx->cfg.iflag_timeadvance == 2 && stagewise) {
         rc = residual_stage_all(ctx, aux, pri_ok);
-        static_assert(kBatchedPrepClearsDq == 0, "the concurrent SA update owns w_dqs / w_dq");
+        static_assert(!kBatchedUpdateWritesDq, "the concurrent SA update owns w_dqs / w_dq");
         if (!rc && sa && ctx->sa_concurrent && ctx->lub_shape >= 0 && !ctx->timing) {
             // sa_model reads nothing implicit1st writes (pri_v, amu, vor, dt
             // and the cfl are from the residual) and implicit1st reads no

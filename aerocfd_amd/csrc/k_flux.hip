@@ -715,11 +715,10 @@ __global__ void k_reset_stats(unsigned long long *st)
 // reads the stored viscosity there).  The same expressions as the two
 // kernels; the dt statistics from the last block.
 //
-// With PREP, the node also forms implicit1st's spectral radii and diagonal
-// (sub_timeadvancement_1stimplicit.f90:98-156, the expressions of
-// lu_prep_body) from its dt, for the tile sweeps of the batched update; the
-// difference pass then forms rhs.  Every node of the extended block gets
-// them (the ring of the computed range is what the update reads).
+// With PREP, the node also forms implicit1st's time term phi*1/dt
+// (sub_timeadvancement_1stimplicit.f90:98-102,151, as lu_prep_body forms it
+// for the tile sweeps, which add the spectral radii themselves) from its
+// dt; the difference pass then forms rhs.
 template <typename R, bool PREP>
 __global__ void __launch_bounds__(256) k_viscous_node_dt_batch(const BlockRec *__restrict__ blks,
                                                                const int *__restrict__ map,
@@ -759,7 +758,7 @@ __global__ void __launch_bounds__(256) k_viscous_node_dt_batch(const BlockRec *_
         arena[b.o.dt + n] = v;
         if (PREP) {
             const double phi = *cfl >= 1.0e3 ? 0.0 : 1.0;  // implicit1st :98-102
-            arena[b.o.w_diag + n] = lu_radii_diag(pri, dxidx, N, n, v, phi, arena + b.o.w_r);
+            arena[b.o.w_diag + n] = phi * 1.0 / v;
         }
     }
     dt_stats(ok, v, m == nblk - 1 ? stats : nullptr);

@@ -17,10 +17,12 @@
 //                clock: thread (tj,tk) handles the node at sweep position
 //                s - tj - tk at step s.
 //                * Sweep layout.  Before the forward sweep the operands of
-//                  every node (q, metric row, 1/J, spectral radii, diag, rhs:
-//                  24 values) are gathered into an array ordered by (tile,
-//                  step, field pair, lane), so at every step a tile reads 12
-//                  contiguous rows of TJ*TK 16-byte pairs -- fully coalesced, where
+//                  every node (metric rows, 1/J, q, sound speed, the time
+//                  term phi/dt, rhs: 22 values; the sweeps form the spectral
+//                  radii and the diagonal from them) are gathered into an
+//                  array ordered by (tile, step, field pair, lane), so at
+//                  every step a tile reads 11 contiguous rows of TJ*TK
+//                  16-byte pairs -- fully coalesced, where
 //                  the natural layout would have every lane on its own cache
 //                  line.  The sweeps write dq* and dq over the rhs slots and a
 //                  final pass adds dq to q.
@@ -150,15 +152,17 @@ __device__ inline void lu_backward(const LuArgs &a, long n, const double *vi, co
     lu_bwd_combine(dqs, ta, tb, tc, a.diag[n], a.inv_j[n], res);
 }
 
-// rhs = rhsi + rhsv on the computed range; spectral radii ra, rb, rc and
-// diag on the ring is1-1..ie1+1 (:72-156); with clear_dq, dq* and dq cleared
-// on the ring so that out-of-range neighbours read 0 (:105-106).
+// rhs = rhsi + rhsv on the computed range (:72-78).  For the hyperplane
+// sweeps (tiles = 0): spectral radii ra, rb, rc and diag on the ring
+// is1-1..ie1+1 (:98-156), and dq* and dq cleared on the ring so that
+// out-of-range neighbours read 0 (:105-106).  For the tile sweeps, which
+// form radii and diagonal themselves: only the time term phi*1/dt into diag.
 __device__ inline void lu_prep_body(const Ext &e, long wg, const double *__restrict__ pri,
                                     const double *__restrict__ dxidx, const double *__restrict__ dt,
                                     const double *__restrict__ rhsi, const double *__restrict__ rhsv,
                                     double *__restrict__ rhs, double *__restrict__ rr, double *__restrict__ diag,
                                     double *__restrict__ dqs, double *__restrict__ dq, const double *cfl,
-                                    int clear_dq)
+                                    int tiles)
 {
     const Box ring{e.is1 - 1, e.ie1 + 1, e.js1 - 1, e.je1 + 1, e.ks1 - 1, e.ke1 + 1};
     int i, j, k;
@@ -166,11 +170,12 @@ __device__ inline void lu_prep_body(const Ext &e, long wg, const double *__restr
     if (!lin_node_at(e, ring, wg, i, j, k, n)) return;
     const long N = e.n;
     const double phi = *cfl >= 1.0e3 ? 0.0 : 1.0;  // :98-102
-    diag[n] = lu_radii_diag(pri, dxidx, N, n, dt[n], phi, rr);
+    if (tiles) diag[n] = phi * 1.0 / dt[n];
+    else diag[n] = lu_radii_diag(pri, dxidx, N, n, dt[n], phi, rr);
     bool inner = i >= e.is1 && i <= e.ie1 && j >= e.js1 && j <= e.je1 && k >= e.ks1 && k <= e.ke1;
 #pragma unroll
     for (int x = 0; x < 5; x++) {
-        if (clear_dq) {
+        if (!tiles) {
             dqs[x * N + n] = 0.0;
             dq[x * N + n] = 0.0;
         }
@@ -183,17 +188,17 @@ __global__ void __launch_bounds__(256) k_lu_prep(Ext e, const double *__restrict
                                                  const double *__restrict__ rhsi, const double *__restrict__ rhsv,
                                                  double *__restrict__ rhs, double *__restrict__ rr,
                                                  double *__restrict__ diag, double *__restrict__ dqs,
-                                                 double *__restrict__ dq, const double *cfl, int clear_dq)
+                                                 double *__restrict__ dq, const double *cfl, int tiles)
 {
     lu_prep_body(e, xcd_slab(blockIdx.x, gridDim.x), pri, dxidx, dt, rhsi, rhsv, rhs, rr, diag, dqs, dq, cfl,
-                 clear_dq);
+                 tiles);
 }
 
-// every block; block m reads cfl[m * cfl_stride]
+// every block, for the batched tile sweeps (rhs and the time term; dq*
+// and dq are never touched); block m reads cfl[m * cfl_stride]
 __global__ void __launch_bounds__(256) k_lu_prep_batch(const BlockRec *__restrict__ blks,
                                                        const int *__restrict__ map, const KGrid *__restrict__ kg,
-                                                       double *arena, const double *cfl, int cfl_stride,
-                                                       int clear_dq)
+                                                       double *arena, const double *cfl, int cfl_stride)
 {
     int bx, by, bz;
     const int m = batch_wg(map, kg, bx, by, bz);
@@ -201,7 +206,7 @@ __global__ void __launch_bounds__(256) k_lu_prep_batch(const BlockRec *__restric
     const Ext e = b.e;
     lu_prep_body(e, bx, arena + b.o.pri, arena + b.o.dxidx, arena + b.o.dt, arena + b.o.rhsi,
                  arena + b.o.rhsv, arena + b.o.rhs, arena + b.o.w_r, arena + b.o.w_diag, arena + b.o.w_dqs,
-                 arena + b.o.w_dq, cfl + (long)m * cfl_stride, clear_dq);
+                 arena + b.o.w_dq, cfl + (long)m * cfl_stride, 1);
 }
 
 // linear tiles of 256 nodes over the block's storage (the ring of the
@@ -209,11 +214,11 @@ __global__ void __launch_bounds__(256) k_lu_prep_batch(const BlockRec *__restric
 dim3 lu_prep_grid(const Ext &e) { return lin_grid(e); }
 
 int launch_lu_prep_batch(hipStream_t s, const BlockRec *blks, const BatchGrid &bg, double *arena, const double *cfl,
-                         int cfl_stride, int clear_dq)
+                         int cfl_stride)
 {
     if (bg.total <= 0) return 0;
     hipLaunchKernelGGL(k_lu_prep_batch, dim3((unsigned)bg.total), dim3(64, 4, 1), 0, s, blks, bg.map, bg.kg, arena,
-                       cfl, cfl_stride, clear_dq);
+                       cfl, cfl_stride);
     ACFD_CHECK_LAUNCH();
     return 0;
 }
@@ -259,15 +264,22 @@ __global__ void __launch_bounds__(256) k_add5(Ext e, Box b, double *__restrict__
 }
 
 // ------------------------------------------------------------ tile sweeps
-// Sweep-layout fields: dxidx 0-8, inv_j 9, q 10-14, rr 15-17, diag 18,
-// rhs 19-23 (overwritten by dq* in the forward and dq in the backward sweep).
-// The grid's operands come first, as whole field pairs: while the sweep
-// layout holds the same block and tiling and the metrics are unchanged, a
-// gather can leave those pairs in place and move only the other 14 fields.
-constexpr int kSlF = 24;
+// Sweep-layout fields: dxidx 0-8, inv_j 9, q 10-14, sound speed 15, the
+// time term phi/dt 16, rhs 17-21 (overwritten by dq* in the forward and dq
+// in the backward sweep).  The sweeps form the spectral radii and the
+// diagonal from these (lu_radius, lu_diag), with the velocity q[1..3]/q[0]
+// as get_primitivevariables forms it, so the results are those of radii
+// and diagonal read from the preparation pass.  The grid's operands come
+// first, as whole field pairs: while the sweep layout holds the same block
+// and tiling and the metrics are unchanged, a gather can leave those pairs
+// in place and move only the other 12 fields.
+constexpr int kSlF = 22;
 constexpr int kSlQ = 10;
-constexpr int kSlOut = 19;
-static_assert(kSlOut % 2 == 1 && kSlOut + 5 == kSlF, "the add pass reads the results as pairs 9-11");
+constexpr int kSlC = 15;
+constexpr int kSlT = 16;
+constexpr int kSlOut = 17;
+static_assert(kSlOut % 2 == 1 && kSlOut + 5 == kSlF && kSlT == kSlOut - 1,
+              "the results are the upper half of the time term's pair and the two pairs after it");
 constexpr int kSlGeoPairs = 5;
 // exchange sentinel: a negative signalling NaN with a payload; arithmetic
 // only ever produces quiet NaNs
@@ -409,22 +421,23 @@ __device__ inline bool sl_line_ok(const SlGeom &g, int tile, int line)
     return (tile % g.nbj) * g.TJ + (line & (g.TJ - 1)) < g.Lj && (tile / g.nbj) * g.TK + (line >> g.lgTJ) < g.Lk;
 }
 
+// csnd: the sound speed (component 7 of pri_v), tterm: the time term
 __device__ inline const double *gather_src(int f, long N, const double *q, const double *dxidx,
-                                           const double *inv_j, const double *rr, const double *diag,
+                                           const double *inv_j, const double *csnd, const double *tterm,
                                            const double *rhs)
 {
-    return f < 9 ? dxidx + f * N : f == 9 ? inv_j : f < 15 ? q + (f - 10) * N
-         : f < 18 ? rr + (f - 15) * N : f == 18 ? diag : rhs + (f - 19) * N;
+    return f < 9 ? dxidx + f * N : f == 9 ? inv_j : f < kSlC ? q + (f - kSlQ) * N
+         : f == kSlC ? csnd : f == kSlT ? tterm : rhs + (f - kSlOut) * N;
 }
 
 // Gather the operands of every node into the sweep layout, a field pair at
 // a time (each lane then stores whole 16-byte pairs), from pair fp0 on (0:
-// all 24 fields, kSlGeoPairs: all but the grid's).  TT is a template
+// all kSlF fields, kSlGeoPairs: all but the grid's).  TT is a template
 // parameter so that a thread's CT loads are issued back to back.
 template <int TT>
 __device__ inline void sl_gather_tile(const Ext &e, const SlGeom &g, int tile, const double *__restrict__ q,
                                       const double *__restrict__ dxidx, const double *__restrict__ inv_j,
-                                      const double *__restrict__ rr, const double *__restrict__ diag,
+                                      const double *__restrict__ csnd, const double *__restrict__ tterm,
                                       const double *__restrict__ rhs, double *__restrict__ sl, int fp0,
                                       double *buf /* [2][CT][TT] LDS, swizzled */)
 {
@@ -448,7 +461,7 @@ __device__ inline void sl_gather_tile(const Ext &e, const SlGeom &g, int tile, c
         for (int fp = fp0; fp < kSlF / 2; fp++) {
 #pragma unroll
             for (int h = 0; h < 2; h++) {
-                const double *src = gather_src(2 * fp + h, N, q, dxidx, inv_j, rr, diag, rhs);
+                const double *src = gather_src(2 * fp + h, N, q, dxidx, inv_j, csnd, tterm, rhs);
                 double v[CT];
 #pragma unroll
                 for (int m = 0; m < CT; m++) v[m] = nat[m] >= 0 ? src[nat[m]] : 0.0;
@@ -477,12 +490,12 @@ template <int TT>
 __global__ void __launch_bounds__(TT) k_sl_gather(Ext e, SlGeom g, const double *__restrict__ q,
                                                   const double *__restrict__ dxidx,
                                                   const double *__restrict__ inv_j,
-                                                  const double *__restrict__ rr,
-                                                  const double *__restrict__ diag,
+                                                  const double *__restrict__ csnd,
+                                                  const double *__restrict__ tterm,
                                                   const double *__restrict__ rhs, double *__restrict__ sl, int fp0)
 {
     extern __shared__ double buf[];
-    sl_gather_tile<TT>(e, g, (int)blockIdx.x, q, dxidx, inv_j, rr, diag, rhs, sl, fp0, buf);
+    sl_gather_tile<TT>(e, g, (int)blockIdx.x, q, dxidx, inv_j, csnd, tterm, rhs, sl, fp0, buf);
 }
 
 // q += dq (:568-574): the backward sweep left q + dq in the result slots of
@@ -556,7 +569,7 @@ static void launch_layout(hipStream_t s, int which, const Ext &e, const SlGeom &
     const size_t lds = 2 * (size_t)kSlElems * sizeof(double);
     if (which == 0)
         hipLaunchKernelGGL(k_sl_gather<TT>, dim3((unsigned)g.ntiles), dim3(TT), lds, s, e, g, f.q, f.dxidx,
-                           f.inv_j, f.w_r, f.w_diag, f.rhs, ws.sl, fp0);
+                           f.inv_j, f.pri + 6 * e.n, f.w_diag, f.rhs, ws.sl, fp0);
     else
         hipLaunchKernelGGL(k_sl_add<TT>, dim3((unsigned)g.ntiles), dim3(TT), lds, s, e, g, ws.sl, f.q);
 }
@@ -576,16 +589,28 @@ __global__ void __launch_bounds__(256) k_fill_sentinel(double *p, long n)
     if (i < n) xstore_bits(p + i, kSentinel);
 }
 
-// One step of a lane: combine, solve, push.  op = the node's 24 operands.
-template <int DIR>
-__device__ inline void lu_node(const double *op, const double *ta, const double *tb, const double *tc,
-                               double gamma, double *res, double *pa, double *pb, double *pc)
+// The node's spectral radii and diagonal from its operands: velocity as
+// get_primitivevariables forms it (sub_getpri_v.f90:39-41)
+__device__ inline double lu_node_radii(const double *op, double *r)
 {
-    if (DIR > 0) lu_fwd_combine(op + kSlOut, ta, tb, tc, op[18], op[9], res);
-    else lu_bwd_combine(op + kSlOut, ta, tb, tc, op[18], op[9], res);
-    jac_mv_r<DIR>(op + kSlQ, op[0], op[1], op[2], op[9], op[15], gamma, res, pa);
-    jac_mv_r<DIR>(op + kSlQ, op[3], op[4], op[5], op[9], op[16], gamma, res, pb);
-    jac_mv_r<DIR>(op + kSlQ, op[6], op[7], op[8], op[9], op[17], gamma, res, pc);
+    const double d = op[kSlQ];
+    const double u = op[kSlQ + 1] / d, v = op[kSlQ + 2] / d, w = op[kSlQ + 3] / d;
+#pragma unroll
+    for (int x = 0; x < 3; x++) r[x] = lu_radius(op[3 * x], op[3 * x + 1], op[3 * x + 2], u, v, w, op[kSlC]);
+    return lu_diag(op[kSlT], r);
+}
+
+// One step of a lane: combine, solve, push.  op = the node's operands, dg
+// and r its diagonal and spectral radii.
+template <int DIR>
+__device__ inline void lu_node(const double *op, double dg, const double *r, const double *ta, const double *tb,
+                               const double *tc, double gamma, double *res, double *pa, double *pb, double *pc)
+{
+    if (DIR > 0) lu_fwd_combine(op + kSlOut, ta, tb, tc, dg, op[9], res);
+    else lu_bwd_combine(op + kSlOut, ta, tb, tc, dg, op[9], res);
+    jac_mv_r<DIR>(op + kSlQ, op[0], op[1], op[2], op[9], r[0], gamma, res, pa);
+    jac_mv_r<DIR>(op + kSlQ, op[3], op[4], op[5], op[9], r[1], gamma, res, pb);
+    jac_mv_r<DIR>(op + kSlQ, op[6], op[7], op[8], op[9], r[2], gamma, res, pc);
 }
 
 // Persistent tile sweep over the sweep layout.  The forward sweep numbers
@@ -673,6 +698,10 @@ __device__ inline void lu_tiles_body(const SlGeom &g, int tile, double *__restri
         const int p = s & 1;
         if (on_at(s)) {
             const int pos = s - sj - sk;
+            // from the node's own operands, ahead of the waits on its
+            // neighbours' values
+            double rad[3];
+            const double dg = lu_node_radii(op, rad);
             double tb[5], tc[5], res[5], pa[5], pb[5], pc[5];
             if (j_ring) {
 #pragma unroll
@@ -692,7 +721,7 @@ __device__ inline void lu_tiles_body(const SlGeom &g, int tile, double *__restri
 #pragma unroll
                 for (int x = 0; x < 5; x++) tc[x] = pk[p ^ 1][x][lpk];
             }
-            lu_node<DIR>(op, ta, tb, tc, gamma, res, pa, pb, pc);
+            lu_node<DIR>(op, dg, rad, ta, tb, tc, gamma, res, pa, pb, pc);
             if (fuse_add) {
                 const long n = nat0 + (g.Li - 1 - pos);
 #pragma unroll
@@ -705,9 +734,8 @@ __device__ inline void lu_tiles_body(const SlGeom &g, int tile, double *__restri
                 double o[5];
 #pragma unroll
                 for (int x = 0; x < 5; x++) o[x] = DIR < 0 ? op[kSlQ + x] + res[x] : res[x];
-                static_assert(kSlOut == 19 && kSlF == 24, "results are fields 19-23, after diag in field 18");
-                double2 *dst = reinterpret_cast<double2 *>(sl + g.slot(tile, row(s), kSlOut - 1, l));
-                dst[0] = make_double2(op[kSlOut - 1], o[0]);
+                double2 *dst = reinterpret_cast<double2 *>(sl + g.slot(tile, row(s), kSlT, l));
+                dst[0] = make_double2(op[kSlT], o[0]);
                 dst[TT] = make_double2(o[1], o[2]);
                 dst[2 * TT] = make_double2(o[3], o[4]);
             }
@@ -764,7 +792,7 @@ __global__ void __launch_bounds__(TJ *TK) k_lu_tiles(SlGeom g, double *__restric
 struct LuBatchEnt {
     Ext e;
     SlGeom g;
-    long q, dxidx, inv_j, rr, diag, rhs;
+    long q, dxidx, inv_j, csnd, tterm, rhs;
     long sl, ex;
     int tile0;
 };
@@ -779,8 +807,8 @@ __global__ void __launch_bounds__(TT) k_sl_gather_batch(const LuBatchEnt *__rest
     const LuBatchEnt &b = ents[tile_ent[flat]];
     const Ext e = b.e;
     const SlGeom g = b.g;
-    sl_gather_tile<TT>(e, g, flat - b.tile0, arena + b.q, arena + b.dxidx, arena + b.inv_j, arena + b.rr,
-                       arena + b.diag, arena + b.rhs, slb + b.sl, fp0, buf);
+    sl_gather_tile<TT>(e, g, flat - b.tile0, arena + b.q, arena + b.dxidx, arena + b.inv_j, arena + b.csnd,
+                       arena + b.tterm, arena + b.rhs, slb + b.sl, fp0, buf);
 }
 
 template <int TT>
@@ -1328,8 +1356,8 @@ int lusgs_batch_make(int nblk, const Ext *e, const Fields *f, double *arena, int
         b.q = f[m].q - arena;
         b.dxidx = f[m].dxidx - arena;
         b.inv_j = f[m].inv_j - arena;
-        b.rr = f[m].w_r - arena;
-        b.diag = f[m].w_diag - arena;
+        b.csnd = f[m].pri + 6 * e[m].n - arena;
+        b.tterm = f[m].w_diag - arena;
         b.rhs = f[m].rhs - arena;
         b.sl = so;
         b.ex = xo;
@@ -1660,7 +1688,8 @@ static void launch_tiles_shape(hipStream_t s, int shape, const SlGeom &g, const 
 }
 
 // Phases of one LU-SGS update:
-//   0 rhs, spectral radii, diag (hyperplanes also clear dq* and dq),
+//   0 rhs and, for hyperplanes, spectral radii and diag (and dq* and dq
+//     cleared), for tiles the time term phi/dt,
 //   1 tiles: gather the operands into the sweep layout (with geo_cached,
 //     which says that it already holds this block's grid operands in this
 //     tiling, all but those),
@@ -1676,7 +1705,7 @@ int launch_lusgs_phase(hipStream_t s, const Ext &e, const Fields &f, const Const
     const bool empty = Li <= 0 || Lj <= 0 || Lk <= 0;
     if (phase == 0) {
         hipLaunchKernelGGL(k_lu_prep, lu_prep_grid(e), dim3(64, 4, 1), 0, s, e, f.pri, f.dxidx, f.dt, f.rhsi,
-                           f.rhsv, f.rhs, f.w_r, f.w_diag, f.w_dqs, f.w_dq, cfl, tiles ? 0 : 1);
+                           f.rhsv, f.rhs, f.w_r, f.w_diag, f.w_dqs, f.w_dq, cfl, tiles ? 1 : 0);
     } else if (empty) {
         return 0;
     } else if (tiles) {

@@ -71,16 +71,16 @@ int launch_viscous_batch(hipStream_t s, const BlockRec *blks, const BatchGrid &b
                          int precision, int store_aux, int part = -1);
 // the viscous stage with the steady find_dt folded into its node pass
 // (statistics into stats from the last block; reset them first); with
-// prep, also implicit1st's spectral radii, diagonal (node pass) and rhs =
-// rhsi + rhsv (difference pass), which launch_lu_prep_batch then need not
-// form
+// prep, also what launch_lu_prep_batch forms for the tile sweeps: the time
+// term phi/dt (node pass) and rhs = rhsi + rhsv (difference pass)
 int launch_viscous_dt_batch(hipStream_t s, const BlockRec *blks, int nblk, const BatchGrid &bg, double *arena,
                             const Consts &c, int precision, int store_aux, const double *cfl,
                             unsigned long long *stats, int part = -1, int prep = 0);
 int launch_find_dt_batch(hipStream_t s, const BlockRec *blks, int nblk, const BatchGrid &bg, double *arena,
                          const Consts &c, double *cfl, double *cfl_blk, unsigned long long *stats);
+// rhs and the time term phi/dt of the batched tile sweeps
 int launch_lu_prep_batch(hipStream_t s, const BlockRec *blks, const BatchGrid &bg, double *arena, const double *cfl,
-                         int cfl_stride, int clear_dq);
+                         int cfl_stride);
 int launch_residual_batch(hipStream_t s, const BlockRec *blks, int nblk, double *arena, double *partial,
                           int with_sa);
 

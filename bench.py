@@ -47,14 +47,14 @@ BYTES_PER_NODE = {
     "getpri": 40 + 56,                              # q -> pri_v
     "inviscid": 40 + 72 + 8 + 40,                   # q, dxidx, inv_j -> rhsi
     "viscous": 32 + 8 + 72 + 8 + 16 + 40,           # u,v,w,T, mu_t, dxidx, inv_j -> mu, rhsv
-    # tile sweeps over the sweep layout: 24 operands in, 5 results out
-    "lusgs_forward": 24 * 8 + 40,                   # q, dxidx, inv_j, r_abc, diag, rhs -> dq*
-    "lusgs_backward": 24 * 8 + 40,                  # q, dxidx, inv_j, r_abc, diag, dq* -> dq
-    # 14 operands in (natural) and out (sweep layout): the 10 grid operands
+    # tile sweeps over the sweep layout: 22 operands in, 5 results out
+    "lusgs_forward": 22 * 8 + 40,                   # dxidx, inv_j, q, c, phi/dt, rhs -> dq*
+    "lusgs_backward": 22 * 8 + 40,                  # dxidx, inv_j, q, c, phi/dt, dq* -> q + dq
+    # 12 operands in (natural) and out (sweep layout): the 10 grid operands
     # stay in the sweep layout from one update of a block to the next
-    "lusgs_gather": 2 * 14 * 8,
+    "lusgs_gather": 2 * 12 * 8,
     "lusgs_add": 40 + 40,                           # q + dq (sweep layout) -> q
-    "lusgs_other": 32 + 72 + 8 + 80 + 40 + 32,      # u,v,w,c, dxidx, dt, rhsi+rhsv -> rhs, r_abc, diag
+    "lusgs_other": 8 + 80 + 40 + 8,                 # dt, rhsi+rhsv -> rhs, phi/dt (tile sweeps)
     # SA scalar sweeps: a, b, c, diag and rhs (forward) / dnut* (backward)
     # in, dnut* / dnut out
     "sa_forward": 5 * 8 + 8,
@@ -361,10 +361,10 @@ def roofline_of(args, tm, rr, sa):
         table["viscous_node"] = VISCOUS_NODE_SA
     # the viscous node pass with the steady find_dt: + rho, c in, dt out;
     # when it also forms implicit1st's preparation (batched sweeps, no
-    # separate preparation timed): + radii, diag out, and the difference
-    # pass + rhsi in, rhs out
+    # separate preparation timed): + the time term phi/dt out, and the
+    # difference pass + rhsi in, rhs out
     folded = args.lusgs == 2 and tm.get("lusgs_other", (0, 0))[1] == 0 and tm.get("viscous_node_dt", (0, 0))[1] > 0
-    table["viscous_node_dt"] = table["viscous_node"] + 16 + 8 + (32 if folded else 0)
+    table["viscous_node_dt"] = table["viscous_node"] + 16 + 8 + (8 if folded else 0)
     if folded:
         table["viscous_diff"] = BYTES_PER_NODE["viscous_diff"] + 80
     if args.lusgs == 0:
@@ -377,7 +377,7 @@ def roofline_of(args, tm, rr, sa):
         if (int(env) != 0) if env is not None else (args.lusgs != 2 and launch < FUSED_ADD_MAX_NODES):
             table["lusgs_backward"] = FUSED_BACKWARD_BYTES
         if os.environ.get("ACFD_SL_GEO_CACHE", "1") == "0":
-            table["lusgs_gather"] = 2 * 24 * 8
+            table["lusgs_gather"] = 2 * 22 * 8
 
     # sa_pre / sa_add are single kernels only on the batched SA path (the
     # per-block path times its launches of every block together)
