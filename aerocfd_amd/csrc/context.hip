@@ -2008,7 +2008,23 @@ int acfd_grid_metrics(acfd_ctx *ctx, double *const *xyz)
         std::vector<double> w(wx);  // kept until the stream is synchronised below
         w.insert(w.end(), wy.begin(), wy.end());
         w.insert(w.end(), wz.begin(), wz.end());
+        // bounding box of every chunk of kWallChunk consecutive wall points
+        const int nchunk = (int)((nw + kWallChunk - 1) / kWallChunk);
+        std::vector<double> cbox(6 * (size_t)nchunk);
+        for (int c = 0; c < nchunk; c++) {
+            double *bx = &cbox[6 * (size_t)c];
+            bx[0] = bx[1] = bx[2] = 1.0e300;
+            bx[3] = bx[4] = bx[5] = -1.0e300;
+            for (long p = (long)c * kWallChunk; p < std::min(nw, (long)(c + 1) * kWallChunk); p++) {
+                const double v[3] = {wx[p], wy[p], wz[p]};
+                for (int d = 0; d < 3; d++) {
+                    bx[d] = std::min(bx[d], v[d]);
+                    bx[3 + d] = std::max(bx[3 + d], v[d]);
+                }
+            }
+        }
         if (nw > 0) {
+            w.insert(w.end(), cbox.begin(), cbox.end());
             HIP_TRY(hipMalloc((void **)&dw, w.size() * sizeof(double)));
             HIP_TRY(hipMemcpyAsync(dw, w.data(), w.size() * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
         }
@@ -2016,7 +2032,8 @@ int acfd_grid_metrics(acfd_ctx *ctx, double *const *xyz)
         for (int m = 0; m < nb && !rc; m++) {
             const Block &b = ctx->blk[m];
             const long n = b.e.n;
-            rc = launch_walldist(ctx->stream, b.e, b.geo, b.geo + n, b.geo + 2 * n, dw, nw, b.f.dist);
+            rc = launch_walldist(ctx->stream, b.e, b.geo, b.geo + n, b.geo + 2 * n, dw, nw, dw ? dw + 3 * nw : nullptr,
+                                 nchunk, b.f.dist);
         }
         const hipError_t es = hipStreamSynchronize(ctx->stream);
         if (dw) (void)hipFree(dw);

@@ -218,12 +218,52 @@ __global__ void __launch_bounds__(256) k_grid_spacing(Ext e, const double *__res
 // minimum of the squared distances is taken and rooted once: the square
 // root is monotone and correctly rounded, so this equals the minimum of the
 // rooted distances the reference compares.
-constexpr int kWallTile = 256;
-__global__ void __launch_bounds__(256) k_walldist(Ext e, const double *__restrict__ x, const double *__restrict__ y,
-                                                  const double *__restrict__ z, const double *__restrict__ wall,
-                                                  long nw, double *__restrict__ dist)
+constexpr int kWallTile = kWallChunk;
+
+// workgroup-wide min (MAXOP false) or max of v, every thread gets the result
+template <bool MAXOP>
+__device__ inline double wg_reduce(double v, double *red)
 {
-    __shared__ double wx[kWallTile], wy[kWallTile], wz[kWallTile];
+    red[threadIdx.x] = v;
+    __syncthreads();
+    for (int o = kWallTile / 2; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) {
+            const double a = red[threadIdx.x], c = red[threadIdx.x + o];
+            red[threadIdx.x] = MAXOP ? (c > a ? c : a) : (c < a ? c : a);
+        }
+        __syncthreads();
+    }
+    const double r = red[0];
+    __syncthreads();
+    return r;
+}
+
+// squared distance between two axis-aligned boxes (0 where they overlap)
+__device__ inline double box_gap2(const double *a, const double *b)
+{
+    double s = 0.0;
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        const double g = fmax(0.0, fmax(b[d] - a[3 + d], a[d] - b[3 + d]));
+        s += g * g;
+    }
+    return s;
+}
+
+// Wall points are taken in chunks of kWallTile (cbox: each chunk's bounding
+// box, lo xyz then hi xyz).  A workgroup bounds its own nodes by a box,
+// takes first the chunk nearest that box, then every chunk whose box could
+// hold a point nearer than the largest distance its nodes have found; a
+// chunk it skips holds no point nearer to any of its nodes than what they
+// hold already (the test keeps a relative margin of 1e-12 over the rounding
+// of the box gap), so the minimum is the one over all points.
+__global__ void __launch_bounds__(kWallTile) k_walldist(Ext e, const double *__restrict__ x,
+                                                        const double *__restrict__ y, const double *__restrict__ z,
+                                                        const double *__restrict__ wall, long nw,
+                                                        const double *__restrict__ cbox, int nchunk,
+                                                        double *__restrict__ dist)
+{
+    __shared__ double wx[kWallTile], wy[kWallTile], wz[kWallTile], red[kWallTile];
     const long L = (long)(e.ie0 - e.is0 + 1) * (e.je0 - e.js0 + 1) * (e.ke0 - e.ks0 + 1);
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const bool ok = t < L;
@@ -238,21 +278,53 @@ __global__ void __launch_bounds__(256) k_walldist(Ext e, const double *__restric
         zz = z[n];
     }
     double best = 1.0e18;  // (1e9)^2: the reference's distance with no wall
-    for (long p0 = 0; p0 < nw; p0 += kWallTile) {
-        const long p = p0 + threadIdx.x;
-        if (p < nw) {
-            wx[threadIdx.x] = wall[p];
-            wy[threadIdx.x] = wall[nw + p];
-            wz[threadIdx.x] = wall[2 * nw + p];
+    if (nw > 0) {
+        // the workgroup's box
+        double wb[6];
+        wb[0] = wg_reduce<false>(ok ? xx : 1.0e300, red);
+        wb[1] = wg_reduce<false>(ok ? yy : 1.0e300, red);
+        wb[2] = wg_reduce<false>(ok ? zz : 1.0e300, red);
+        wb[3] = wg_reduce<true>(ok ? xx : -1.0e300, red);
+        wb[4] = wg_reduce<true>(ok ? yy : -1.0e300, red);
+        wb[5] = wg_reduce<true>(ok ? zz : -1.0e300, red);
+        // the nearest chunk (smallest box gap, lowest index among equals)
+        double g0 = 1.0e308;
+        int c0 = 0;
+        for (int c = threadIdx.x; c < nchunk; c += kWallTile) {
+            const double g = box_gap2(wb, cbox + 6L * c);
+            if (g < g0) {
+                g0 = g;
+                c0 = c;
+            }
         }
-        __syncthreads();
-        const int cnt = (int)min((long)kWallTile, nw - p0);
-        for (int q = 0; q < cnt; q++) {
-            const double dx = xx - wx[q], dy = yy - wy[q], dz = zz - wz[q];
-            const double d2 = dx * dx + dy * dy + dz * dz;
-            best = d2 < best ? d2 : best;
+        const double gmin = wg_reduce<false>(g0, red);
+        const double cand = g0 == gmin ? (double)c0 : 1.0e300;
+        const int first = (int)wg_reduce<false>(cand, red);
+        auto take = [&](int c) {
+            const long p0 = (long)c * kWallTile;
+            const long p = p0 + threadIdx.x;
+            if (p < nw) {
+                wx[threadIdx.x] = wall[p];
+                wy[threadIdx.x] = wall[nw + p];
+                wz[threadIdx.x] = wall[2 * nw + p];
+            }
+            __syncthreads();
+            const int cnt = (int)min((long)kWallTile, nw - p0);
+            for (int q = 0; q < cnt; q++) {
+                const double dx = xx - wx[q], dy = yy - wy[q], dz = zz - wz[q];
+                const double d2 = dx * dx + dy * dy + dz * dz;
+                best = d2 < best ? d2 : best;
+            }
+            __syncthreads();
+        };
+        take(first);
+        double bound = wg_reduce<true>(ok ? best : 0.0, red);
+        int taken = 0;
+        for (int c = 0; c < nchunk; c++) {
+            if (c == first || box_gap2(wb, cbox + 6L * c) > bound * (1.0 + 1.0e-12)) continue;
+            take(c);
+            if (++taken % 16 == 0) bound = wg_reduce<true>(ok ? best : 0.0, red);
         }
-        __syncthreads();
     }
     if (ok) dist[n] = nw > 0 ? sqrt(best) : 1.0e9;
 }
@@ -277,11 +349,12 @@ int launch_grid_spacing(hipStream_t s, const Ext &e, const double *x, const doub
 }
 
 int launch_walldist(hipStream_t s, const Ext &e, const double *x, const double *y, const double *z,
-                    const double *wall, long nw, double *dist)
+                    const double *wall, long nw, const double *cbox, int nchunk, double *dist)
 {
     const long L = (long)(e.ie0 - e.is0 + 1) * (e.je0 - e.js0 + 1) * (e.ke0 - e.ks0 + 1);
     if (L <= 0) return 0;
-    hipLaunchKernelGGL(k_walldist, dim3((unsigned)((L + 255) / 256)), dim3(256), 0, s, e, x, y, z, wall, nw, dist);
+    hipLaunchKernelGGL(k_walldist, dim3((unsigned)((L + kWallTile - 1) / kWallTile)), dim3(kWallTile), 0, s, e, x, y,
+                       z, wall, nw, cbox, nchunk, dist);
     ACFD_CHECK_LAUNCH();
     return 0;
 }

@@ -407,8 +407,15 @@ __device__ inline void lds_barrier()
 // (CT = min(32, 4096/TT)).
 // Sweep side: thread x is lane x.  LDS holds [CT][TT] doubles per field
 // with the line index XOR-swizzled by ii (no bank conflicts either way).
-constexpr int kSlElems = 4096;  // CT * TT at most
+// steps a layout kernel transposes at a time: CT * TT <= kSlElems
+constexpr int kSlElems = 4096;
 constexpr int sl_ct_of(int tt) { return kSlElems / tt < 32 ? kSlElems / tt : 32; }
+// the SA return pass (one value per node) takes at most 16: measured on
+// 64-lane tiles, 0.86 against 1.25 ms (the two gathers and the flow add are
+// faster at 32)
+constexpr int sa_ret_ct_of(int tt) { return sl_ct_of(tt) < 16 ? sl_ct_of(tt) : 16; }
+// dynamic LDS of the flow layout kernels: [2][CT][TT] doubles
+constexpr size_t sl_lds_bytes(int tt) { return 2 * (size_t)sl_ct_of(tt) * tt * sizeof(double); }
 
 // node index of (tile, line, li), li in range
 __device__ inline long sl_nat(const Ext &e, const SlGeom &g, int tile, int line, int li)
@@ -566,7 +573,7 @@ template <int TT>
 static void launch_layout(hipStream_t s, int which, const Ext &e, const SlGeom &g, const Fields &f,
                           const LuWs &ws, int fp0)
 {
-    const size_t lds = 2 * (size_t)kSlElems * sizeof(double);
+    const size_t lds = sl_lds_bytes(TT);
     if (which == 0)
         hipLaunchKernelGGL(k_sl_gather<TT>, dim3((unsigned)g.ntiles), dim3(TT), lds, s, e, g, f.q, f.dxidx,
                            f.inv_j, f.pri + 6 * e.n, f.w_diag, f.rhs, ws.sl, fp0);
@@ -1093,7 +1100,7 @@ __device__ inline void sa_layout_tile(const Ext &e, const SaGeom &g, int tile, c
                                       const double *__restrict__ rhs, double *__restrict__ nut, double *__restrict__ L,
                                       double *buf /* [CT][TT] */)
 {
-    constexpr int CT = sl_ct_of(TT), lstep = TT / CT;
+    constexpr int CT = TO_SL ? sl_ct_of(TT) : sa_ret_ct_of(TT), lstep = TT / CT;
     const int x = threadIdx.x;
     const int ii_n = x % CT;
     const int tj = x % g.TJ, tk = x / g.TJ;
@@ -1154,7 +1161,7 @@ __global__ void __launch_bounds__(TT) k_sa_layout_batch(const SaBatchEnt *__rest
                                                         const int *__restrict__ tile_ent, double *arena,
                                                         double *slbase)
 {
-    __shared__ double buf[sl_ct_of(TT) * TT];
+    __shared__ double buf[(TO_SL ? sl_ct_of(TT) : sa_ret_ct_of(TT)) * TT];
     const int flat = (int)xcd_slab(blockIdx.x, gridDim.x);
     const SaBatchEnt &b = ents[tile_ent[flat]];
     const Ext e = b.e;
@@ -1448,7 +1455,7 @@ static void launch_tiles_batch_shape(hipStream_t s, const LuBatch &b, double gam
 template <int TT>
 static void launch_layout_batch(hipStream_t s, int which, const LuBatch &b, int fp0)
 {
-    const size_t lds = 2 * (size_t)kSlElems * sizeof(double);
+    const size_t lds = sl_lds_bytes(TT);
     const LuBatchEnt *ents = (const LuBatchEnt *)b.ents;
     if (which == 0)
         hipLaunchKernelGGL(k_sl_gather_batch<TT>, dim3((unsigned)b.total), dim3(TT), lds, s, ents, b.tile_ent,

@@ -89,8 +89,11 @@ int launch_grid_metrics(hipStream_t s, const Ext &e, const double *x, const doub
                         double *inv_j, double *dxidx, double *aal, double *bbe, double *gga);
 int launch_grid_spacing(hipStream_t s, const Ext &e, const double *x, const double *y, const double *z, double *spc,
                         double *spacing);
+// wall points in chunks of kWallChunk; cbox: per chunk its bounding box
+// (lo x, y, z, hi x, y, z)
+constexpr int kWallChunk = 256;
 int launch_walldist(hipStream_t s, const Ext &e, const double *x, const double *y, const double *z,
-                    const double *wall, long nw, double *dist);
+                    const double *wall, long nw, const double *cbox, int nchunk, double *dist);
 // one cell of a wall subface: block index, face, wall-plane index and the
 // lower corner along the face's two tangential axes (in axis order)
 struct WallCell {
