@@ -112,31 +112,31 @@ HYPERPLANE_BYTES = {  # the hyperplane sweeps read the operands in place
 MEASURED_TRAFFIC_CHANNEL = {
     "getpri": 10.035e9,
     "inviscid_i": 11.015e9,
-    "inviscid_j": 17.100e9,
-    "inviscid_k": 16.498e9,
-    "viscous_node_dt": 39.294e9,
-    "viscous_diff": 25.357e9,
-    "lusgs_gather": 27.423e9,
-    "lusgs_forward": 30.329e9,
-    "lusgs_backward": 30.325e9,
+    "inviscid_j": 17.098e9,
+    "inviscid_k": 16.497e9,
+    "viscous_node_dt": 35.372e9,
+    "viscous_diff": 25.356e9,
+    "lusgs_gather": 23.504e9,
+    "lusgs_forward": 28.474e9,
+    "lusgs_backward": 28.453e9,
     "lusgs_add": 9.223e9,
-    "sa_pre": 26.744e9,
-    "sa_pre_b": 31.727e9,
+    "sa_pre": 26.731e9,
+    "sa_pre_b": 31.733e9,
     "sa_gather": 9.872e9,
-    "sa_forward": 6.170e9,
-    "sa_backward": 6.159e9,
-    "sa_add": 2.849e9,
+    "sa_forward": 6.168e9,
+    "sa_backward": 6.164e9,
+    "sa_add": 3.286e9,
 }
 # The TGV 256^3 block, FP64, tile sweeps (profiles/tgv256_fp64_hbm_traffic.md).
 
 MEASURED_TRAFFIC_256 = {
-    # each sweep plus the refill of the exchange region (132 MB written)
-    # that follows it inside the same timer
-    "lusgs_forward": 5.073e9,
-    "lusgs_backward": 5.075e9,
-    "lusgs_gather": 4.763e9,   # a later gather (the grid operands in place)
+    # the sweeps with the refill of the exchange region that shares their timer
+    "getpri": 1.687e9,
+    "lusgs_other": 2.322e9,
+    "lusgs_gather": 4.094e9,
+    "lusgs_forward": 4.456e9,
+    "lusgs_backward": 4.455e9,
     "lusgs_add": 1.572e9,
-    "lusgs_other": 4.537e9,
 }
 
 
