@@ -110,6 +110,21 @@ __device__ inline double lu_radii_diag(const double *pri, const double *dxidx, l
     return lu_diag(phi * 1.0 / dt, r3);
 }
 
+// Operands a pass reads once and results no later node of the pass reads:
+// nontemporal (streaming) loads and stores, so that the L2 keeps the lines
+// that neighbouring nodes' stencils read again.
+template <typename T>
+__device__ inline T ld_once(const T *p)
+{
+    return __builtin_nontemporal_load(p);
+}
+
+template <typename T>
+__device__ inline void st_out(T *p, T v)
+{
+    __builtin_nontemporal_store(v, p);
+}
+
 }  // namespace acfd
 
 #define ACFD_CHECK_LAUNCH()                                                    \

@@ -191,6 +191,12 @@ __global__ void __launch_bounds__(256) k_inviscid_i(Ext e, const double *__restr
 // over kInvRun nodes, keeping the four-value window and the previous face
 // flux in registers, so each face is evaluated once.  rhsi -= (F+ - F-).
 constexpr int kInvRun = 16;
+// Waves per SIMD asked of the k kernel: FP64 it needs 176 VGPRs (two
+// waves); held to 168 with a few values in scratch it runs three, and on
+// the 40-block channel takes 3.84 ms instead of 4.32.  The same bound on
+// the j kernel (four waves: 22 values in scratch) or four on the k kernel
+// is slower by a third or more.
+constexpr int kInvKWaves = 3;
 template <typename R, int AXIS>
 __device__ inline void inviscid_jk_body(const Ext &e, int bx, int by, int bz, const double *__restrict__ q,
                                         const double *__restrict__ dxidx, const double *__restrict__ inv_j,
@@ -258,7 +264,7 @@ __device__ inline void inviscid_jk_body(const Ext &e, int bx, int by, int bz, co
 }
 
 template <typename R, int AXIS>
-__global__ void __launch_bounds__(256) k_inviscid_jk(Ext e, const double *__restrict__ q,
+__global__ void __launch_bounds__(256, AXIS == 2 ? kInvKWaves : 1) k_inviscid_jk(Ext e, const double *__restrict__ q,
                                                     const double *__restrict__ dxidx,
                                                     const double *__restrict__ inv_j,
                                                     double *__restrict__ rhsi, double gamma)
@@ -266,9 +272,10 @@ __global__ void __launch_bounds__(256) k_inviscid_jk(Ext e, const double *__rest
     inviscid_jk_body<R, AXIS>(e, (int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z, q, dxidx, inv_j, rhsi, gamma);
 }
 
-// AXIS 0 = the i kernel, 1 / 2 = the j / k kernels
+// AXIS 0 = the i kernel, 1 / 2 = the j / k kernels.  The k kernel runs at
+// three waves per SIMD (kInvKWaves).
 template <typename R, int AXIS>
-__global__ void __launch_bounds__(256) k_inviscid_batch(const BlockRec *__restrict__ blks,
+__global__ void __launch_bounds__(256, AXIS == 2 ? kInvKWaves : 1) k_inviscid_batch(const BlockRec *__restrict__ blks,
                                                         const int *__restrict__ map,
                                                         const KGrid *__restrict__ kg, double *arena, double gamma)
 {
@@ -389,7 +396,7 @@ __device__ inline void viscous_node_eval(const Ext &e, int i, int j, int k, cons
     for (int x = 0; x < 9; x++) m[x] = (R)dxidx[x * N + n];
     R u0 = (R)pri[N + n], v0 = (R)pri[2 * N + n], w0 = (R)pri[3 * N + n], t0 = (R)pri[5 * N + n];
     am1 = (R)((1.0 + 124.0 / c.tinf) / ((double)t0 + 124.0 / c.tinf) * pow((double)t0, 1.5));
-    R am2 = (R)amu[N + n];
+    R am2 = (R)ld_once(amu + N + n);
     am3 = am1 + am2;
     R ki = (R)(c.gamma * c.cv) * (am1 / (R)c.prl + am2 / (R)c.prt);
     R tx = g[0][0] * m[0] + g[0][1] * m[3] + g[0][2] * m[6];
@@ -411,7 +418,7 @@ __device__ inline void viscous_node_eval(const Ext &e, int i, int j, int k, cons
     R bx = u0 * ta[0] + v0 * ta[1] + w0 * ta[2] + ki * tx;
     R by = u0 * ta[1] + v0 * ta[3] + w0 * ta[4] + ki * ty;
     R bz = u0 * ta[2] + v0 * ta[4] + w0 * ta[5] + ki * tz;
-    R ij = (R)inv_j[n];
+    R ij = (R)ld_once(inv_j + n);
 #pragma unroll
     for (int d = 0; d < 3; d++) {
         const R *r = m + 3 * d;
@@ -429,16 +436,16 @@ __device__ inline void viscous_node_store(long N, long n, R am1, R am3, const R 
                                           double *__restrict__ amu, double *__restrict__ vor,
                                           double *__restrict__ dudx, double *__restrict__ tao, int store_aux)
 {
-    amu[n] = (double)am1;
-    amu[2 * N + n] = (double)am3;
+    st_out(amu + n, (double)am1);
+    st_out(amu + 2 * N + n, (double)am3);
     if (store_aux) {
-        vor[n] = (double)sqrt((du[5] - du[7]) * (du[5] - du[7]) + (du[2] - du[6]) * (du[2] - du[6])
-                              + (du[1] - du[3]) * (du[1] - du[3]));
+        st_out(vor + n, (double)sqrt((du[5] - du[7]) * (du[5] - du[7]) + (du[2] - du[6]) * (du[2] - du[6])
+                              + (du[1] - du[3]) * (du[1] - du[3])));
         if (store_aux > 1) {
 #pragma unroll
-            for (int x = 0; x < 9; x++) dudx[x * N + n] = (double)du[x];
+            for (int x = 0; x < 9; x++) st_out(dudx + x * N + n, (double)du[x]);
 #pragma unroll
-            for (int x = 0; x < 6; x++) tao[x * N + n] = (double)ta[x];
+            for (int x = 0; x < 6; x++) st_out(tao + x * N + n, (double)ta[x]);
         }
     }
 }
@@ -492,17 +499,17 @@ __device__ inline void viscous_diff_body(const Ext &e, long wg, const S *__restr
     const long N = e.n;
     const long sj = e.ni, sk = (long)e.ni * e.nj;
     const double v0 = (0.0 + 0.0 + 0.0) / re;
-    rhsv[n] = v0;
+    st_out(rhsv + n, v0);
     const bool inner = rhs && i >= e.is1 && i <= e.ie1 && j >= e.js1 && j <= e.je1 && k >= e.ks1 && k <= e.ke1;
-    if (inner) rhs[n] = rhsi[n] + v0;
+    if (inner) st_out(rhs + n, ld_once(rhsi + n) + v0);
 #pragma unroll
     for (int c = 0; c < 4; c++) {
         double a = ndiff(w + (0 + c) * N, n, 1, i, e.is0, e.ie0);
         double bb = ndiff(w + (4 + c) * N, n, sj, j, e.js0, e.je0);
         double h = ndiff(w + (8 + c) * N, n, sk, k, e.ks0, e.ke0);
         const double v = (a + bb + h) / re;
-        rhsv[(c + 1) * N + n] = v;
-        if (inner) rhs[(c + 1) * N + n] = rhsi[(c + 1) * N + n] + v;
+        st_out(rhsv + (c + 1) * N + n, v);
+        if (inner) st_out(rhs + (c + 1) * N + n, ld_once(rhsi + (c + 1) * N + n) + v);
     }
 }
 
@@ -606,11 +613,12 @@ __device__ inline double local_rate_a3(const Ext &e, long n, const double *pri, 
     const long N = e.n;
     double m[9];
 #pragma unroll
-    for (int x = 0; x < 9; x++) m[x] = dxidx[x * N + n];
+    for (int x = 0; x < 9; x++) m[x] = ld_once(dxidx + x * N + n);
     double txi = sqrt(m[0] * m[0] + m[1] * m[1] + m[2] * m[2]);
     double teta = sqrt(m[3] * m[3] + m[4] * m[4] + m[5] * m[5]);
     double tzeta = sqrt(m[6] * m[6] + m[7] * m[7] + m[8] * m[8]);
-    double d0 = pri[n], u0 = pri[N + n], v0 = pri[2 * N + n], w0 = pri[3 * N + n], c0 = pri[6 * N + n];
+    double d0 = ld_once(pri + n), u0 = pri[N + n], v0 = pri[2 * N + n], w0 = pri[3 * N + n],
+           c0 = ld_once(pri + 6 * N + n);
     double uu = (m[0] * u0 + m[1] * v0 + m[2] * w0) / txi;
     double vv = (m[3] * u0 + m[4] * v0 + m[5] * w0) / teta;
     double ww = (m[6] * u0 + m[7] * v0 + m[8] * w0) / tzeta;
@@ -749,16 +757,16 @@ __global__ void __launch_bounds__(256) k_viscous_node_dt_batch(const BlockRec *_
                                   store_aux);
             R *w = reinterpret_cast<R *>(arena + b.o.w_flux);
 #pragma unroll
-            for (int x = 0; x < 12; x++) w[x * N + n] = (R)w12[x];
+            for (int x = 0; x < 12; x++) st_out(w + x * N + n, (R)w12[x]);
             a3 = (double)am3;
         } else {
             a3 = amu[2 * N + n];
         }
         v = *cfl / local_rate_a3(e, n, pri, a3, dxidx, c.re);
-        arena[b.o.dt + n] = v;
+        st_out(arena + b.o.dt + n, v);
         if (PREP) {
             const double phi = *cfl >= 1.0e3 ? 0.0 : 1.0;  // implicit1st :98-102
-            arena[b.o.w_diag + n] = phi * 1.0 / v;
+            st_out(arena + b.o.w_diag + n, phi * 1.0 / v);
         }
     }
     dt_stats(ok, v, m == nblk - 1 ? stats : nullptr);
