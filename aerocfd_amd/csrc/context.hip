@@ -166,6 +166,11 @@ struct acfd_ctx {
     // and diagonal (batched, steady NS): the next batched update skips its
     // preparation pass
     bool lu_prep_ready = false;
+    // the last residual stage formed rhs = rhsi + rhsv without storing rhsv
+    // (the same fold): rhsv_refresh forms it from the node fluxes, which
+    // stay in w_flux until the next inviscid stage or grid set-up; a
+    // captured step that does so sets it on every replay (graph_rhsv)
+    bool rhsv_stale = false, graph_rhsv = false;
     int *lu_ctr = nullptr;   // ticket counters of the batched sweeps
     // SA's update runs beside the flow's LU-SGS update on a second stream
     // (batched sweeps, SA): its own exchange slots and ticket counters
@@ -1297,6 +1302,17 @@ int st_viscous_dt_all(acfd_ctx *ctx, int store_aux)
                                         ctx->cfg.precision, store_aux, ctx->d_cfl, ctx->d_stats, -1, prep));
     }
     ctx->lu_prep_ready = prep == 1;
+    if (prep) ctx->rhsv_stale = true;
+    return 0;
+}
+
+// rhsv of the last residual stage where that stage did not store it
+int rhsv_refresh(acfd_ctx *ctx)
+{
+    if (!ctx->rhsv_stale) return 0;
+    ctx->rhsv_stale = false;
+    KRN_TRY(launch_viscous_batch(ctx->stream, ctx->d_blocks, ctx->bg_visc, ctx->arena, ctx->c, ctx->cfg.precision, 0,
+                                 2));
     return 0;
 }
 
@@ -1458,7 +1474,11 @@ int capture_step(acfd_ctx *ctx)
     ctx->graph_needs_pri = false;
     ctx->graph_sl_in[0] = ctx->sl_block;
     ctx->graph_sl_in[1] = ctx->sl_shape;
+    const bool stale = ctx->rhsv_stale;
+    ctx->rhsv_stale = false;
     const int rc = one_step(ctx);
+    ctx->graph_rhsv = ctx->rhsv_stale;
+    ctx->rhsv_stale = stale;  // nothing ran yet
     ctx->capturing = false;
     ctx->graph_sl_out[0] = ctx->sl_block;
     ctx->graph_sl_out[1] = ctx->sl_shape;
@@ -1765,6 +1785,10 @@ int acfd_upload(acfd_ctx *ctx, int m, int field, const double *host)
     if (!ctx || !ctx->topo || m < 1 || m > (int)ctx->blk.size() || !host || field < 1 || field > kLastField)
         return fail(ACFD_EINVAL, "acfd_upload: bad arguments");
     ctx->pri_current = false;
+    if (field == ACFD_RHSV) {
+        int rc = rhsv_refresh(ctx);  // the other blocks' rhsv
+        if (rc) return rc;
+    }
     if (field >= ACFD_X) {
         int rc = ensure_geo(ctx);
         if (rc) return rc;
@@ -1797,7 +1821,8 @@ int acfd_download(acfd_ctx *ctx, int m, int field, double *host)
     if (!src) return fail(ACFD_EINVAL, "acfd_download: field not allocated for this configuration");
     int nc = kNcomp[field];
     long tot = b.e.n * nc;
-    int rc = ensure_stage(ctx, tot);
+    int rc = field == ACFD_RHSV ? rhsv_refresh(ctx) : 0;
+    if (!rc) rc = ensure_stage(ctx, tot);
     if (rc) return rc;
     KRN_TRY(launch_soa_to_aos(ctx->stream, src, ctx->d_stage, b.e.n, nc));
     HIP_TRY(hipMemcpyAsync(host, ctx->d_stage, tot * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
@@ -1811,7 +1836,8 @@ int acfd_run_stage(acfd_ctx *ctx, int stage)
     ctx->pri_current = false;
     ctx->lu_prep_ready = false;
     const int nb = (int)ctx->blk.size();
-    int rc = 0;
+    int rc = rhsv_refresh(ctx);  // the stages read rhsv or reuse w_flux
+    if (rc) return rc;
     switch (stage) {
     case ACFD_STAGE_GETPRI:
         for (int m = 0; m < nb && !rc; m++) rc = st_getpri(ctx, m);
@@ -1891,6 +1917,7 @@ int acfd_step(acfd_ctx *ctx, int nsteps)
             ctx->sl_block = ctx->graph_sl_out[0];
             ctx->sl_shape = ctx->graph_sl_out[1];
             ctx->pri_current = true;
+            if (ctx->graph_rhsv) ctx->rhsv_stale = true;
             continue;
         }
         int rc = one_step(ctx);
@@ -1963,6 +1990,10 @@ int acfd_timing(acfd_ctx *ctx, int enable, double *ms, long *calls)
 int acfd_grid_metrics(acfd_ctx *ctx, double *const *xyz)
 {
     if (!ctx || !ctx->topo || !xyz) return fail(ACFD_EINVAL, "acfd_grid_metrics: bad arguments");
+    {
+        const int rc = rhsv_refresh(ctx);  // w_flux is scratch below
+        if (rc) return rc;
+    }
     const int nb = (int)ctx->blk.size();
     for (int m = 0; m < nb; m++)
         if (!xyz[m]) return fail(ACFD_EINVAL, "acfd_grid_metrics: missing coordinates of a block");

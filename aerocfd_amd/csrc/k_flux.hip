@@ -486,7 +486,8 @@ __global__ void __launch_bounds__(256) k_viscous_node(Ext e, Consts c, const dou
 
 // Stage 2 (sub_viscous.f90:296-365): rhsv = (d/dxi fv + d/deta gv + d/dzeta hv)/re;
 // with rhsi given, also rhs = rhsi + rhsv on the computed range (the
-// first line of implicit1st, sub_timeadvancement_1stimplicit.f90:72-78)
+// first line of implicit1st, sub_timeadvancement_1stimplicit.f90:72-78),
+// and with rhsv null only rhs
 template <typename S>
 __device__ inline void viscous_diff_body(const Ext &e, long wg, const S *__restrict__ w,
                                          double *__restrict__ rhsv, double re, const double *__restrict__ rhsi = nullptr,
@@ -498,9 +499,10 @@ __device__ inline void viscous_diff_body(const Ext &e, long wg, const S *__restr
     if (!lin_node_at(e, b, wg, i, j, k, n)) return;
     const long N = e.n;
     const long sj = e.ni, sk = (long)e.ni * e.nj;
-    const double v0 = (0.0 + 0.0 + 0.0) / re;
-    st_out(rhsv + n, v0);
     const bool inner = rhs && i >= e.is1 && i <= e.ie1 && j >= e.js1 && j <= e.je1 && k >= e.ks1 && k <= e.ke1;
+    if (!rhsv && !inner) return;
+    const double v0 = (0.0 + 0.0 + 0.0) / re;
+    if (rhsv) st_out(rhsv + n, v0);
     if (inner) st_out(rhs + n, ld_once(rhsi + n) + v0);
 #pragma unroll
     for (int c = 0; c < 4; c++) {
@@ -508,7 +510,7 @@ __device__ inline void viscous_diff_body(const Ext &e, long wg, const S *__restr
         double bb = ndiff(w + (4 + c) * N, n, sj, j, e.js0, e.je0);
         double h = ndiff(w + (8 + c) * N, n, sk, k, e.ks0, e.ke0);
         const double v = (a + bb + h) / re;
-        st_out(rhsv + (c + 1) * N + n, v);
+        if (rhsv) st_out(rhsv + (c + 1) * N + n, v);
         if (inner) st_out(rhs + (c + 1) * N + n, ld_once(rhsi + (c + 1) * N + n) + v);
     }
 }
@@ -535,8 +537,8 @@ __global__ void __launch_bounds__(256) k_viscous_node_batch(const BlockRec *__re
                          store_aux);
 }
 
-// with_rhs: also rhs = rhsi + rhsv on the computed range; S: the storage
-// type of the node fluxes
+// with_rhs 1: also rhs = rhsi + rhsv on the computed range; 2: rhs only
+// (rhsv not stored); S: the storage type of the node fluxes
 template <typename S>
 __global__ void __launch_bounds__(256) k_viscous_diff_batch(const BlockRec *__restrict__ blks,
                                                             const int *__restrict__ map,
@@ -547,7 +549,9 @@ __global__ void __launch_bounds__(256) k_viscous_diff_batch(const BlockRec *__re
     const BlockRec &b = blks[batch_wg(map, kg, bx, by, bz)];
     const Ext e = b.e;
     const S *w = reinterpret_cast<const S *>(arena + b.o.w_flux);
-    if (with_rhs) viscous_diff_body<S>(e, bx, w, arena + b.o.rhsv, re, arena + b.o.rhsi, arena + b.o.rhs);
+    if (with_rhs)
+        viscous_diff_body<S>(e, bx, w, with_rhs == 2 ? nullptr : arena + b.o.rhsv, re, arena + b.o.rhsi,
+                             arena + b.o.rhs);
     else viscous_diff_body<S>(e, bx, w, arena + b.o.rhsv, re);
 }
 
@@ -797,9 +801,9 @@ int launch_viscous_dt_batch(hipStream_t s, const BlockRec *blks, int nblk, const
     }
     if (part < 0 || part == 2) {
         if (precision == 1)
-            hipLaunchKernelGGL(k_viscous_diff_batch<float>, g, t, 0, s, blks, bg.map, bg.kg, arena, c.re, prep);
+            hipLaunchKernelGGL(k_viscous_diff_batch<float>, g, t, 0, s, blks, bg.map, bg.kg, arena, c.re, 2 * prep);
         else
-            hipLaunchKernelGGL(k_viscous_diff_batch<double>, g, t, 0, s, blks, bg.map, bg.kg, arena, c.re, prep);
+            hipLaunchKernelGGL(k_viscous_diff_batch<double>, g, t, 0, s, blks, bg.map, bg.kg, arena, c.re, 2 * prep);
     }
     ACFD_CHECK_LAUNCH();
     return 0;

@@ -72,7 +72,9 @@ int launch_viscous_batch(hipStream_t s, const BlockRec *blks, const BatchGrid &b
 // the viscous stage with the steady find_dt folded into its node pass
 // (statistics into stats from the last block; reset them first); with
 // prep, also what launch_lu_prep_batch forms for the tile sweeps: the time
-// term phi/dt (node pass) and rhs = rhsi + rhsv (difference pass)
+// term phi/dt (node pass) and rhs = rhsi + rhsv (difference pass, which
+// then does not store rhsv: launch_viscous_batch part 2 forms it again from
+// the node fluxes)
 int launch_viscous_dt_batch(hipStream_t s, const BlockRec *blks, int nblk, const BatchGrid &bg, double *arena,
                             const Consts &c, int precision, int store_aux, const double *cfl,
                             unsigned long long *stats, int part = -1, int prep = 0);

@@ -362,11 +362,11 @@ def roofline_of(args, tm, rr, sa):
     # the viscous node pass with the steady find_dt: + rho, c in, dt out;
     # when it also forms implicit1st's preparation (batched sweeps, no
     # separate preparation timed): + the time term phi/dt out, and the
-    # difference pass + rhsi in, rhs out
+    # difference pass + rhsi in, rhs out, rhsv not stored
     folded = args.lusgs == 2 and tm.get("lusgs_other", (0, 0))[1] == 0 and tm.get("viscous_node_dt", (0, 0))[1] > 0
     table["viscous_node_dt"] = table["viscous_node"] + 16 + 8 + (8 if folded else 0)
     if folded:
-        table["viscous_diff"] = BYTES_PER_NODE["viscous_diff"] + 80
+        table["viscous_diff"] = BYTES_PER_NODE["viscous_diff"] + 40
     if args.lusgs == 0:
         table.update(HYPERPLANE_BYTES)
     else:

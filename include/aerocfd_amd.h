@@ -114,7 +114,11 @@ int acfd_set_topology(acfd_ctx *ctx, int nblk, const int *dims, const int *nsub,
 int acfd_block_ranges(acfd_ctx *ctx, int m, int *ranges);
 
 /* Copy one field of block m between host (reference AoS layout, extended
- * extents) and the device-resident state. */
+ * extents) and the device-resident state.  The batched steady update forms
+ * rhs = rhsi + rhsv without storing rhsv; rhsv is then formed again from
+ * the viscous node fluxes by the first call that needs it (its download or
+ * upload, acfd_run_stage, acfd_grid_metrics), so every call sees the value
+ * of the last viscous evaluation. */
 int acfd_upload(acfd_ctx *ctx, int m, int field, const double *host);
 int acfd_download(acfd_ctx *ctx, int m, int field, double *host);
 
