@@ -110,6 +110,26 @@ __device__ inline double lu_radii_diag(const double *pri, const double *dxidx, l
     return lu_diag(phi * 1.0 / dt, r3);
 }
 
+// The SA diffusion's metric products alpha, beta, gamma at a node
+// (sub_init_gridtransformation.f90:809-866) from its nine dxidx values m
+// and 1/J: p[0..2] alpha, p[3..5] beta, p[6..8] gamma.  Never contracted:
+// the grid set-up stores them and the SA pass may form them again.
+__device__ inline void sa_metric_products(const double *m, double ij, double *p)
+{
+#pragma clang fp contract(off)
+    const double a0 = m[0], a1 = m[1], a2 = m[2], b0 = m[3], b1 = m[4], b2 = m[5];
+    const double c0 = m[6], c1 = m[7], c2 = m[8];
+    p[0] = (a0 * a0 + a1 * a1 + a2 * a2) * ij;
+    p[1] = (a0 * b0 + a1 * b1 + a2 * b2) * ij;
+    p[2] = (a0 * c0 + a1 * c1 + a2 * c2) * ij;
+    p[3] = (b0 * a0 + b1 * a1 + b2 * a2) * ij;
+    p[4] = (b0 * b0 + b1 * b1 + b2 * b2) * ij;
+    p[5] = (b0 * c0 + b1 * c1 + b2 * c2) * ij;
+    p[6] = (c0 * a0 + c1 * a1 + c2 * a2) * ij;
+    p[7] = (c0 * b0 + c1 * b1 + c2 * b2) * ij;
+    p[8] = (c0 * c0 + c1 * c1 + c2 * c2) * ij;
+}
+
 // Operands a pass reads once and results no later node of the pass reads:
 // nontemporal (streaming) loads and stores, so that the L2 keeps the lines
 // that neighbouring nodes' stencils read again.

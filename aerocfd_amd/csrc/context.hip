@@ -171,6 +171,10 @@ struct acfd_ctx {
     // stay in w_flux until the next inviscid stage or grid set-up; a
     // captured step that does so sets it on every replay (graph_rhsv)
     bool rhsv_stale = false, graph_rhsv = false;
+    // alpha, beta, gamma are the products of dxidx and 1/J that
+    // acfd_grid_metrics stored (an upload of any of the five clears it): the
+    // batched SA pass forms them again instead of reading them
+    bool abg_derived = false;
     int *lu_ctr = nullptr;   // ticket counters of the batched sweeps
     // SA's update runs beside the flow's LU-SGS update on a second stream
     // (batched sweeps, SA): its own exchange slots and ticket counters
@@ -1090,7 +1094,8 @@ int st_sa_all(acfd_ctx *ctx, F cfl_of)
         const int stride = nb > 1 && cfl_of(1) != c0 ? 1 : 0;
         for (int part = ctx->timing ? 1 : 0; part <= (ctx->timing ? 2 : 0); part++) {
             StageTimer ts(ctx, part == 2 ? kTimeSaPreB : kTimeSaPre);
-            KRN_TRY(launch_sa_pre_batch(ctx->stream, ctx->d_blocks, ctx->bg_sa, ctx->arena, ctx->c, c0, stride, part));
+            KRN_TRY(launch_sa_pre_batch(ctx->stream, ctx->d_blocks, ctx->bg_sa, ctx->arena, ctx->c, c0, stride, part,
+                                        ctx->abg_derived));
         }
         {
             StageTimer ts(ctx, kTimeSaGather);
@@ -1797,6 +1802,11 @@ int acfd_upload(acfd_ctx *ctx, int m, int field, const double *host)
     double *dst = field_ptr(b, field);
     if (!dst) return fail(ACFD_EINVAL, "acfd_upload: field not allocated for this configuration");
     if (field == ACFD_DXIDX || field == ACFD_INV_J) sl_invalidate(ctx);
+    if (field == ACFD_DXIDX || field == ACFD_INV_J || field == ACFD_AALPHA || field == ACFD_BBETA ||
+        field == ACFD_GGAMMA) {
+        ctx->abg_derived = false;
+        drop_graph(ctx);
+    }
     ctx->lu_prep_ready = false;
     if (field == ACFD_RHSI) {
         ctx->rhsi_clean = false;
@@ -2012,6 +2022,7 @@ int acfd_grid_metrics(acfd_ctx *ctx, double *const *xyz)
         KRN_TRY(launch_grid_metrics(ctx->stream, b.e, x, y, z, b.f.w_flux, b.f.inv_j, b.f.dxidx,
                                     sa ? b.f.aal : nullptr, sa ? b.f.bbe : nullptr, sa ? b.f.gga : nullptr));
     }
+    ctx->abg_derived = sa;
     if (sa) {
         // wall points: the face nodes of every wall subface of every block
         // (sub_init_readbcinfo.f90:294-400), SoA x | y | z

@@ -165,17 +165,14 @@ __global__ void __launch_bounds__(256) k_grid_d2(Ext e, const double *__restrict
 #pragma unroll
     for (int c = 0; c < 9; c++) dxidx[c * N + n] = m[c];
     if (aal) {
-        const double a0 = m[0], a1 = m[1], a2 = m[2], b0 = m[3], b1 = m[4], b2 = m[5];
-        const double c0 = m[6], c1 = m[7], c2 = m[8];
-        aal[n] = (a0 * a0 + a1 * a1 + a2 * a2) * ij;
-        aal[N + n] = (a0 * b0 + a1 * b1 + a2 * b2) * ij;
-        aal[2 * N + n] = (a0 * c0 + a1 * c1 + a2 * c2) * ij;
-        bbe[n] = (b0 * a0 + b1 * a1 + b2 * a2) * ij;
-        bbe[N + n] = (b0 * b0 + b1 * b1 + b2 * b2) * ij;
-        bbe[2 * N + n] = (b0 * c0 + b1 * c1 + b2 * c2) * ij;
-        gga[n] = (c0 * a0 + c1 * a1 + c2 * a2) * ij;
-        gga[N + n] = (c0 * b0 + c1 * b1 + c2 * b2) * ij;
-        gga[2 * N + n] = (c0 * c0 + c1 * c1 + c2 * c2) * ij;
+        double p[9];
+        sa_metric_products(m, ij, p);
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            aal[c * N + n] = p[c];
+            bbe[c * N + n] = p[3 + c];
+            gga[c * N + n] = p[6 + c];
+        }
     }
 }
 

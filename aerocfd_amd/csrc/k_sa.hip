@@ -87,13 +87,16 @@ __device__ inline double sa_d(const Ext &e, const double *a, long n, long st, in
 // stage A at node (i,j,k) = n of the computed range: convection
 // (sub_tur_sa.f90:152-208) into sa_rhs, nut gradient (:294-338) into w[0..2]
 // when W012, alpha/beta/gamma products (:340-356) into w[3..5] and
-// nu + (1+cb2) nut (:405-413) into w[6]
+// nu + (1+cb2) nut (:405-413) into w[6].  With inv_j given, alpha, beta and
+// gamma are formed from dxidx and 1/J as the grid set-up forms them
+// (sa_metric_products) instead of read.
 template <bool W012>
 __device__ inline void sa_a_at(const Ext &e, int i, int j, int k, long n, const double *__restrict__ pri,
                                const double *__restrict__ dxidx, const double *__restrict__ nut,
                                const double *__restrict__ amu, const double *__restrict__ aal,
                                const double *__restrict__ bbe, const double *__restrict__ gga,
-                               double *__restrict__ sa_rhs, double *__restrict__ w)
+                               double *__restrict__ sa_rhs, double *__restrict__ w,
+                               const double *__restrict__ inv_j = nullptr)
 {
     const long N = e.n;
     const long st[3] = {1, (long)e.ni, (long)e.ni * e.nj};
@@ -115,9 +118,23 @@ __device__ inline void sa_a_at(const Ext &e, int i, int j, int k, long n, const 
         w[N + n] = g1;
         w[2 * N + n] = g2;
     }
-    w[3 * N + n] = aal[n] * g0 + aal[N + n] * g1 + aal[2 * N + n] * g2;
-    w[4 * N + n] = bbe[n] * g0 + bbe[N + n] * g1 + bbe[2 * N + n] * g2;
-    w[5 * N + n] = gga[n] * g0 + gga[N + n] * g1 + gga[2 * N + n] * g2;
+    double p[9];
+    if (inv_j) {
+        double m[9];
+#pragma unroll
+        for (int x = 0; x < 9; x++) m[x] = dxidx[x * N + n];
+        sa_metric_products(m, ld_once(inv_j + n), p);
+    } else {
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            p[c] = aal[c * N + n];
+            p[3 + c] = bbe[c * N + n];
+            p[6 + c] = gga[c * N + n];
+        }
+    }
+    w[3 * N + n] = p[0] * g0 + p[1] * g1 + p[2] * g2;
+    w[4 * N + n] = p[3] * g0 + p[4] * g1 + p[5] * g2;
+    w[5 * N + n] = p[6] * g0 + p[7] * g1 + p[8] * g2;
     double nu0 = amu[n] / pri[n];
     w[6 * N + n] = nu0 + (1.0 + sa::cb2) * nut[n];
 }
@@ -348,6 +365,7 @@ __global__ void __launch_bounds__(256) k_add1(Ext e, Box b, double *__restrict__
 // Pass A over each block's whole storage (linear tiles of 256 nodes): the
 // length scale on the extended block, stage A on the computed range
 // (without the nut gradient, which pass B forms again).
+template <bool DERIVE>
 __global__ void __launch_bounds__(256) k_sa_a_batch(const BlockRec *__restrict__ blks, const int *__restrict__ map,
                                                     const KGrid *__restrict__ kg, double *arena, Consts c)
 {
@@ -363,7 +381,7 @@ __global__ void __launch_bounds__(256) k_sa_a_batch(const BlockRec *__restrict__
                                          arena + b.o.dudx);
     if (i < e.is1 || i > e.ie1 || j < e.js1 || j > e.je1 || k < e.ks1 || k > e.ke1) return;
     sa_a_at<false>(e, i, j, k, n, pri, arena + b.o.dxidx, nut, amu, arena + b.o.aal, arena + b.o.bbe,
-                   arena + b.o.gga, arena + b.o.sa_rhs, arena + b.o.w_sa);
+                   arena + b.o.gga, arena + b.o.sa_rhs, arena + b.o.w_sa, DERIVE ? arena + b.o.inv_j : nullptr);
 }
 
 // Pass B on the computed range: stage B (sa_rhs completed) and the sweep
@@ -396,11 +414,14 @@ __global__ void __launch_bounds__(256, 5) k_sa_b_batch(const BlockRec *__restric
 }
 
 int launch_sa_pre_batch(hipStream_t s, const BlockRec *blks, const BatchGrid &bg, double *arena, const Consts &c,
-                        const double *cfl, int cfl_stride, int part)
+                        const double *cfl, int cfl_stride, int part, bool derive_abg)
 {
     if (bg.total <= 0) return 0;
     const dim3 g((unsigned)bg.total), t(64, 4, 1);
-    if (part != 2) hipLaunchKernelGGL(k_sa_a_batch, g, t, 0, s, blks, bg.map, bg.kg, arena, c);
+    if (part != 2) {
+        if (derive_abg) hipLaunchKernelGGL(k_sa_a_batch<true>, g, t, 0, s, blks, bg.map, bg.kg, arena, c);
+        else hipLaunchKernelGGL(k_sa_a_batch<false>, g, t, 0, s, blks, bg.map, bg.kg, arena, c);
+    }
     if (part != 1) hipLaunchKernelGGL(k_sa_b_batch, g, t, 0, s, blks, bg.map, bg.kg, arena, c, cfl, cfl_stride);
     ACFD_CHECK_LAUNCH();
     return 0;

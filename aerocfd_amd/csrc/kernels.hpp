@@ -170,9 +170,11 @@ int launch_sa_pre(hipStream_t s, const Ext &e, const Fields &f, const Consts &c,
 int launch_sa_post(hipStream_t s, const Ext &e, const Fields &f);
 // length scale, right-hand side and the sweep operands of every block in two
 // launches (linear-tile grid); block m reads its cfl at cfl[m * cfl_stride];
-// part 1 / 2 launches only the first / second pass (per-kernel timing)
+// part 1 / 2 launches only the first / second pass (per-kernel timing);
+// derive_abg: alpha, beta, gamma are the products of dxidx and 1/J that the
+// grid set-up stores (sa_metric_products), formed again instead of read
 int launch_sa_pre_batch(hipStream_t s, const BlockRec *blks, const BatchGrid &bg, double *arena, const Consts &c,
-                        const double *cfl, int cfl_stride, int part = 0);
+                        const double *cfl, int cfl_stride, int part = 0, bool derive_abg = false);
 
 // k_bc.hip
 struct BcTask {       // one physical subface

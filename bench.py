@@ -62,6 +62,7 @@ BYTES_PER_NODE = {
     # the batched SA update's passes: stage A (length, convection, alpha /
     # beta / gamma products): dist, spacing, rho, u, v, w, nut, mu,
     # dxidx, alpha/beta/gamma in; length, sa_rhs, 4 work fields out
+    # (SA_PRE_DERIVED: alpha/beta/gamma formed from dxidx and 1/J)
     "sa_pre": 16 + 32 + 8 + 8 + 72 + 72 + 8 + 8 + 32,
     # stage B + sweep operands: 4 work fields, dxidx, inv_j, nut, sa_rhs,
     # rho, u, v, w, mu, vorticity, length, dt in; sa_rhs, a, b, c, diag out
@@ -78,6 +79,9 @@ BYTES_PER_NODE = {
 }
 # with SA the viscous node kernel also stores the vorticity magnitude
 VISCOUS_NODE_SA = BYTES_PER_NODE["viscous_node"] + 8
+# stage A when the metrics come from acfd_grid_metrics: 1/J in place of
+# alpha/beta/gamma
+SA_PRE_DERIVED = BYTES_PER_NODE["sa_pre"] - 72 + 8
 # the tile sweep backward pass adds dq to q itself on one block's launch
 # under 2^22 computed nodes (never in the batched sweeps): q in and q out
 # instead of dq out (the add pass is skipped)
@@ -347,7 +351,7 @@ def timed_run(args, case, fields, coords, precision):
     return t1 - t0, tm, tsteps, res, rr
 
 
-def roofline_of(args, tm, rr, sa):
+def roofline_of(args, tm, rr, sa, derived=False):
     """The dominant single-kernel timer (most GPU time per step) and its
     achieved algorithmic GB/s."""
     comp_all = sum((r[13] - r[12] + 1) * (r[15] - r[14] + 1) * (r[17] - r[16] + 1) for r in rr)
@@ -359,6 +363,8 @@ def roofline_of(args, tm, rr, sa):
     table = dict(BYTES_PER_NODE)
     if sa:
         table["viscous_node"] = VISCOUS_NODE_SA
+    if sa and derived:
+        table["sa_pre"] = SA_PRE_DERIVED
     # the viscous node pass with the steady find_dt: + rho, c in, dt out;
     # when it also forms implicit1st's preparation (batched sweeps, no
     # separate preparation timed): + the time term phi/dt out, and the
@@ -461,7 +467,7 @@ def main():
     # nodes of the single block
     nodes = case.totalnodes if fp else args.n ** 3
     value = nodes * args.steps / secs / 1e6
-    dom, avg_ms, achieved, bpn = roofline_of(args, tm, rr, sa)
+    dom, avg_ms, achieved, bpn = roofline_of(args, tm, rr, sa, derived=coords is not None)
     default_cfg = (not fp and args.n == 256 and args.lusgs == 1 and args.precision == 0 and args.cuts == 1
                    and args.scheme == "reference")
     traffic = MEASURED_TRAFFIC_256.get(dom) if default_cfg else None

@@ -146,3 +146,30 @@ def test_channel_bump_batched_graph_identical(precision):
             assert np.array_equal(na, nb), (m, np.abs(na - nb).max())
             assert np.array_equal(ra, rb), (m, np.abs(ra - rb).max())
     say("  identical over %d blocks" % len(out[0]))
+
+
+def test_channel_bump_uploaded_metric_products():
+    """After acfd_grid_metrics the batched SA pass forms alpha, beta and
+    gamma from dxidx and 1/J; an upload of them switches it back to reading
+    them: the same values uploaded give the same bits, scaled ones a
+    different nut."""
+    nsteps = 3
+    say("test_channel_bump_uploaded_metric_products steps", nsteps)
+    out = []
+    for mode in ("derived", "same", "scaled"):
+        case = cases.channel_bump(**SMALL)
+        sol = start(case, lusgs=2, batch=1)
+        try:
+            if mode != "derived":
+                for m in range(len(case.dims)):
+                    for k in ("aalpha", "bbeta", "ggamma"):
+                        v = sol.download(m + 1, k)
+                        sol.upload(m + 1, k, v * (1.5 if mode == "scaled" else 1.0))
+            sol.step(nsteps)
+            out.append([(sol.download(m + 1, "q"), sol.download(m + 1, "nut")) for m in range(len(case.dims))])
+        finally:
+            sol.close()
+    for (qa, na), (qb, nb) in zip(out[0], out[1]):
+        assert np.array_equal(qa, qb) and np.array_equal(na, nb)
+    assert any(not np.array_equal(na, nc) for (_, na), (_, nc) in zip(out[0], out[2]))
+    say("  derived == read; scaled products change nut")
