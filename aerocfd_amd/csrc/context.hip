@@ -1462,7 +1462,9 @@ int one_step(acfd_ctx *ctx)
     } else {
         return fail(ACFD_EINVAL, "iflag_timeadvance must be 2, 3 or 4");
     }
-    if (!rc) rc = launch_residual_all(ctx) ? ACFD_EDEVICE : 0;
+    // no residual here: the reference forms it at its output interval only
+    // (main.f90:575-603), acfd_residual when asked, from the rhs, dt and
+    // sa_rhs the step leaves
     if (!rc) ctx->pri_current = true;
     return rc;
 }

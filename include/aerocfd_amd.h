@@ -129,7 +129,9 @@ int acfd_prepare(acfd_ctx *ctx);
 /* nsteps time steps of main.f90:343-573 (iflag_timeadvance 2 or 3). */
 int acfd_step(acfd_ctx *ctx, int nsteps);
 /* RMS residual of sub_output_residual.f90:24-56: out[0..4] main equations,
- * out[5] SA, out[6] = 0. */
+ * out[5] SA, out[6] = 0; formed when called, from the rhs, dt and sa_rhs
+ * of the last step or stage (the reference forms it at its output
+ * interval, main.f90:575-603). */
 int acfd_residual(acfd_ctx *ctx, double *out);
 /* out = dtmax, dtmin, cflmax, cflmin of the last find_dt call and the
  * current cfl (main.f90:589-592). */
