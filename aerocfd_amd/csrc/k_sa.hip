@@ -245,9 +245,12 @@ __device__ inline double sa_b_at(const Ext &e, int i, int j, int k, long n, cons
     double rn = (s - sn * nut0) / div;
     double r5 = r * r * r * r * r;
     double gn = (1.0 + sa::cw2 * 6.0 * r5 - sa::cw2) * rn;
-    const double cw3m6 = pow(sa::cw3, -6.0);
-    double fwn = -(1.0 / 6.0) * pow((pow(g, -6.0) + cw3m6) / (1.0 + cw3m6), -7.0 / 6.0);
-    fwn = fwn / (1.0 + cw3m6) * (-6.0 * pow(g, -7.0)) * gn;
+    // the reference's integer powers cw3**(-6), g**(-6), g**(-7) as
+    // reciprocals of products (within a few ulps of pow, the same infinities
+    // at g = 0; two pow evaluations less: 6.58 -> 5.82 ms on the channel)
+    const double cw3m6 = 1.0 / cw36;
+    double fwn = -(1.0 / 6.0) * pow((1.0 / gg6 + cw3m6) / (1.0 + cw3m6), -7.0 / 6.0);
+    fwn = fwn / (1.0 + cw3m6) * (-6.0 * (1.0 / (gg6 * g))) * gn;
     double spn = sa::cb1 * (1.0 - ft2) * omega + sa::cb1 * (-ft2n) * omega * nut0;
     double sdn = (1.0 / re) * (sa::cb1 * (fv2 - ft2 * fv2 + ft2) / kar2 - sa::cw1 * fw) * 2.0 * nut0 / dist2;
     sdn = (1.0 / re) * (sa::cb1 * (fv2n - ft2n * fv2 - ft2 * fv2n + ft2n) / kar2 - sa::cw1 * fwn)
