@@ -395,7 +395,8 @@ __device__ inline void viscous_node_eval(const Ext &e, int i, int j, int k, cons
 #pragma unroll
     for (int x = 0; x < 9; x++) m[x] = (R)dxidx[x * N + n];
     R u0 = (R)pri[N + n], v0 = (R)pri[2 * N + n], w0 = (R)pri[3 * N + n], t0 = (R)pri[5 * N + n];
-    am1 = (R)((1.0 + 124.0 / c.tinf) / ((double)t0 + 124.0 / c.tinf) * pow((double)t0, 1.5));
+    // Sutherland; t^1.5 as t sqrt(t) (the reference's t**1.5 to an ulp)
+    am1 = (R)((1.0 + 124.0 / c.tinf) / ((double)t0 + 124.0 / c.tinf) * ((double)t0 * sqrt((double)t0)));
     R am2 = (R)ld_once(amu + N + n);
     am3 = am1 + am2;
     R ki = (R)(c.gamma * c.cv) * (am1 / (R)c.prl + am2 / (R)c.prt);
@@ -626,9 +627,11 @@ __device__ inline double local_rate_a3(const Ext &e, long n, const double *pri, 
     double uu = (m[0] * u0 + m[1] * v0 + m[2] * w0) / txi;
     double vv = (m[3] * u0 + m[4] * v0 + m[5] * w0) / teta;
     double ww = (m[6] * u0 + m[7] * v0 + m[8] * w0) / tzeta;
-    double t1 = fabs(uu) + c0 + 2.0 * txi * a3 * 4.0 / 3.0 / re / d0;
-    double t2 = fabs(vv) + c0 + 2.0 * teta * a3 * 4.0 / 3.0 / re / d0;
-    double t3 = fabs(ww) + c0 + 2.0 * tzeta * a3 * 4.0 / 3.0 / re / d0;
+    // 2 |grad xi| a3 4/3 / re / d0 of :88-90 with one division
+    const double kv = a3 * (8.0 / 3.0) / (re * d0);
+    double t1 = fabs(uu) + c0 + txi * kv;
+    double t2 = fabs(vv) + c0 + teta * kv;
+    double t3 = fabs(ww) + c0 + tzeta * kv;
     return txi * t1 + teta * t2 + tzeta * t3;
 }
 
