@@ -627,7 +627,8 @@ __device__ inline double local_rate_a3(const Ext &e, long n, const double *pri, 
     double uu = (m[0] * u0 + m[1] * v0 + m[2] * w0) / txi;
     double vv = (m[3] * u0 + m[4] * v0 + m[5] * w0) / teta;
     double ww = (m[6] * u0 + m[7] * v0 + m[8] * w0) / tzeta;
-    // 2 |grad xi| a3 4/3 / re / d0 of :88-90 with one division
+    // 2 |grad xi| a3 4/3 / re / d0 of sub_timeadvancement_finddt.f90:75-77
+    // with one division
     const double kv = a3 * (8.0 / 3.0) / (re * d0);
     double t1 = fabs(uu) + c0 + txi * kv;
     double t2 = fabs(vv) + c0 + teta * kv;
