@@ -114,33 +114,31 @@ HYPERPLANE_BYTES = {  # the hyperplane sweeps read the operands in place
 # (profiles/channel_bump_fp64_hbm_traffic.md); the sweeps' figures include
 # the refill of the exchange slots that shares their timer.
 MEASURED_TRAFFIC_CHANNEL = {
-    "getpri": 10.035e9,
+    "getpri": 10.034e9,
     "inviscid_i": 11.015e9,
-    "inviscid_j": 17.098e9,
-    "inviscid_k": 16.497e9,
-    "viscous_node_dt": 35.372e9,
-    "viscous_diff": 25.356e9,
-    "lusgs_gather": 23.504e9,
-    "lusgs_forward": 28.474e9,
-    "lusgs_backward": 28.453e9,
-    "lusgs_add": 9.223e9,
-    "sa_pre": 26.731e9,
-    "sa_pre_b": 31.733e9,
+    "inviscid_j": 17.102e9,
+    "inviscid_k": 18.419e9,
+    "viscous_node_dt": 30.727e9,
+    "viscous_diff": 20.762e9,
+    "lusgs_gather": 23.393e9,
+    "lusgs_forward": 28.464e9,
+    "lusgs_backward": 28.461e9,
+    "lusgs_add": 9.255e9,
+    "sa_pre": 19.906e9,
+    "sa_pre_b": 24.757e9,
     "sa_gather": 9.872e9,
-    "sa_forward": 6.168e9,
-    "sa_backward": 6.164e9,
+    "sa_forward": 6.162e9,
+    "sa_backward": 6.166e9,
     "sa_add": 3.286e9,
 }
 # The TGV 256^3 block, FP64, tile sweeps (profiles/tgv256_fp64_hbm_traffic.md).
-
 MEASURED_TRAFFIC_256 = {
-    # the sweeps with the refill of the exchange region that shares their timer
     "getpri": 1.687e9,
     "lusgs_other": 2.322e9,
-    "lusgs_gather": 4.094e9,
-    "lusgs_forward": 4.456e9,
-    "lusgs_backward": 4.455e9,
-    "lusgs_add": 1.572e9,
+    "lusgs_gather": 4.092e9,
+    "lusgs_forward": 4.457e9,
+    "lusgs_backward": 4.456e9,
+    "lusgs_add": 1.577e9,
 }
 
 
