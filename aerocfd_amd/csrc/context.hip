@@ -179,6 +179,10 @@ struct acfd_ctx {
     // SA's update runs beside the flow's LU-SGS update on a second stream
     // (batched sweeps, SA): its own exchange slots and ticket counters
     bool sa_concurrent = false;
+    // the residual's viscous node pass runs on the second stream beside the
+    // inviscid kernels (batched NS stages, reference scheme)
+    bool res_concurrent = false;
+    hipEvent_t ev_rfork = nullptr, ev_rjoin = nullptr;
     hipStream_t stream2 = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     double *sa_ex = nullptr;
@@ -638,9 +642,15 @@ int alloc_lu_ws(acfd_ctx *ctx)
             KRN_TRY(launch_fill_sentinel(ctx->stream, ctx->sa_ex, ex_n));
             HIP_TRY(hipMalloc((void **)&ctx->sa_ctr, 2 * sizeof(int)));
             HIP_TRY(hipMemsetAsync(ctx->sa_ctr, 0, 2 * sizeof(int), ctx->stream));
+        }
+        const char *rc_env = std::getenv("ACFD_RES_CONCURRENT");
+        ctx->res_concurrent = ctx->c.solver == 0 && !(rc_env && std::atoi(rc_env) == 0);
+        if (ctx->sa_concurrent || ctx->res_concurrent) {
             HIP_TRY(hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking));
             HIP_TRY(hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming));
             HIP_TRY(hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming));
+            HIP_TRY(hipEventCreateWithFlags(&ctx->ev_rfork, hipEventDisableTiming));
+            HIP_TRY(hipEventCreateWithFlags(&ctx->ev_rjoin, hipEventDisableTiming));
         }
         if (sa) {
             // the SA sweep layout, sized for the largest tile shape, and
@@ -1288,14 +1298,25 @@ int st_find_dt_all(acfd_ctx *ctx)
 // viscous_flux and the steady find_dt of every block, the find_dt folded
 // into the viscous node pass (the unsteady find_dt evaluates one node per
 // block, nothing to fold)
-int st_viscous_dt_all(acfd_ctx *ctx, int store_aux)
+// parts -1: both passes; 1: the node pass; 2: the difference pass
+int st_viscous_dt_all(acfd_ctx *ctx, int store_aux, int parts = -1)
 {
     StageTimer t(ctx, ACFD_STAGE_VISCOUS);
-    KRN_TRY(launch_reset_stats(ctx->stream, ctx->d_stats));
     const int nb = (int)ctx->blk.size();
     // the batched LU-SGS update's time term and rhs come from these two
     // passes too (implicit1st then skips its own preparation)
     const int prep = ctx->cfg.lusgs_kernel == 2 && ctx->cfg.iflag_timeadvance == 2 ? 1 : 0;
+    if (parts >= 0) {
+        if (parts == 1) KRN_TRY(launch_reset_stats(ctx->stream, ctx->d_stats));
+        KRN_TRY(launch_viscous_dt_batch(ctx->stream, ctx->d_blocks, nb, ctx->bg_visc, ctx->arena, ctx->c,
+                                        ctx->cfg.precision, store_aux, ctx->d_cfl, ctx->d_stats, parts, prep));
+        if (parts == 2) {
+            ctx->lu_prep_ready = prep == 1;
+            if (prep) ctx->rhsv_stale = true;
+        }
+        return 0;
+    }
+    KRN_TRY(launch_reset_stats(ctx->stream, ctx->d_stats));
     if (ctx->timing) {
         for (int part = 1; part <= 2; part++) {
             StageTimer ts(ctx, part == 1 ? kTimeViscNodeDt : kTimeViscDiff);
@@ -1324,9 +1345,24 @@ int rhsv_refresh(acfd_ctx *ctx)
 int residual_stage_all(acfd_ctx *ctx, int store_aux, bool pri_ok = false)
 {
     int rc = pri_ok ? 0 : st_getpri_all(ctx);
+    const bool fused = ctx->c.solver == 0 && ctx->c.itime == 0 && batched(ctx) && viscous_dt_fused_on();
+    if (!rc && fused && ctx->res_concurrent && ctx->cfg.inviscid_scheme == 0 && !ctx->timing) {
+        // the inviscid kernels read q and the metrics and write rhsi; the
+        // viscous node pass reads pri_v, amu and the metrics and writes
+        // neither: it runs on the second stream meanwhile, and the
+        // difference pass (rhs = rhsi + rhsv) after both
+        HIP_TRY(hipEventRecord(ctx->ev_rfork, ctx->stream));
+        HIP_TRY(hipStreamWaitEvent(ctx->stream2, ctx->ev_rfork, 0));
+        std::swap(ctx->stream, ctx->stream2);
+        rc = st_viscous_dt_all(ctx, store_aux, 1);
+        std::swap(ctx->stream, ctx->stream2);
+        if (!rc) rc = st_inviscid_all(ctx);
+        HIP_TRY(hipEventRecord(ctx->ev_rjoin, ctx->stream2));
+        HIP_TRY(hipStreamWaitEvent(ctx->stream, ctx->ev_rjoin, 0));
+        return rc ? rc : st_viscous_dt_all(ctx, store_aux, 2);
+    }
     if (!rc) rc = st_inviscid_all(ctx);
-    if (!rc && ctx->c.solver == 0 && ctx->c.itime == 0 && batched(ctx) && viscous_dt_fused_on())
-        return st_viscous_dt_all(ctx, store_aux);
+    if (!rc && fused) return st_viscous_dt_all(ctx, store_aux);
     if (!rc && ctx->c.solver == 0) rc = st_viscous_all(ctx, store_aux);
     if (!rc) rc = st_find_dt_all(ctx);
     return rc;
@@ -1603,6 +1639,8 @@ int acfd_destroy(acfd_ctx *ctx)
     for (auto &ev : ctx->ev_pool) (void)hipEventDestroy(ev);
     if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
     if (ctx->ev_join) (void)hipEventDestroy(ctx->ev_join);
+    if (ctx->ev_rfork) (void)hipEventDestroy(ctx->ev_rfork);
+    if (ctx->ev_rjoin) (void)hipEventDestroy(ctx->ev_rjoin);
     if (ctx->sa_ctr) (void)hipFree(ctx->sa_ctr);
     if (ctx->sa_ex) uc_release(ctx->sa_ex);
     if (ctx->sa_sl) (void)hipFree(ctx->sa_sl);
