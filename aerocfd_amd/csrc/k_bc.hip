@@ -184,7 +184,7 @@ __device__ inline void sa_clamp_node(const Ext &e, long n, double *__restrict__ 
     double l3 = leix * leix * leix;
     double fv1 = l3 / (l3 + cv1 * cv1 * cv1);
     double nn = fmin(nu, 1.0e6);
-    nut[n] = nn;
+    if (!(nn == nu)) nut[n] = nn;  // stored only where it changes (NaN included)
     amu[N + n] = nn * fv1 * d;
 }
 
