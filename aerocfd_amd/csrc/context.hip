@@ -175,6 +175,11 @@ struct acfd_ctx {
     // acfd_grid_metrics stored (an upload of any of the five clears it): the
     // batched SA pass forms them again instead of reading them
     bool abg_derived = false;
+    // the SA length scale holds min(d, C_des spacing) (DES97) or d (plain
+    // SA), which no step changes: the batched SA pass skips forming it
+    // again (an upload of dist, spacing or length, or acfd_grid_metrics,
+    // clears it)
+    bool sa_length_ready = false;
     int *lu_ctr = nullptr;   // ticket counters of the batched sweeps
     // SA's update runs beside the flow's LU-SGS update on a second stream
     // (batched sweeps, SA): its own exchange slots and ticket counters
@@ -1105,8 +1110,9 @@ int st_sa_all(acfd_ctx *ctx, F cfl_of)
         for (int part = ctx->timing ? 1 : 0; part <= (ctx->timing ? 2 : 0); part++) {
             StageTimer ts(ctx, part == 2 ? kTimeSaPreB : kTimeSaPre);
             KRN_TRY(launch_sa_pre_batch(ctx->stream, ctx->d_blocks, ctx->bg_sa, ctx->arena, ctx->c, c0, stride, part,
-                                        ctx->abg_derived));
+                                        ctx->abg_derived, !ctx->sa_length_ready));
         }
+        if (ctx->c.des != 2) ctx->sa_length_ready = true;  // DDES's depends on the flow
         {
             StageTimer ts(ctx, kTimeSaGather);
             KRN_TRY(launch_sa_sl_batch(ctx->stream, sb, ctx->lu.err, ctr, 0));
@@ -1847,6 +1853,10 @@ int acfd_upload(acfd_ctx *ctx, int m, int field, const double *host)
         ctx->abg_derived = false;
         drop_graph(ctx);
     }
+    if (field == ACFD_DIST || field == ACFD_SPACING || field == ACFD_LENGTH) {
+        ctx->sa_length_ready = false;
+        drop_graph(ctx);
+    }
     ctx->lu_prep_ready = false;
     if (field == ACFD_RHSI) {
         ctx->rhsi_clean = false;
@@ -2063,6 +2073,8 @@ int acfd_grid_metrics(acfd_ctx *ctx, double *const *xyz)
                                     sa ? b.f.aal : nullptr, sa ? b.f.bbe : nullptr, sa ? b.f.gga : nullptr));
     }
     ctx->abg_derived = sa;
+    ctx->sa_length_ready = false;  // new dist and spacing
+    drop_graph(ctx);
     if (sa) {
         // wall points: the face nodes of every wall subface of every block
         // (sub_init_readbcinfo.f90:294-400), SoA x | y | z

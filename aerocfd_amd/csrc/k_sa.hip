@@ -375,7 +375,8 @@ __global__ void __launch_bounds__(256) k_add1(Ext e, Box b, double *__restrict__
 // (without the nut gradient, which pass B forms again).
 template <bool DERIVE>
 __global__ void __launch_bounds__(256) k_sa_a_batch(const BlockRec *__restrict__ blks, const int *__restrict__ map,
-                                                    const KGrid *__restrict__ kg, double *arena, Consts c)
+                                                    const KGrid *__restrict__ kg, double *arena, Consts c,
+                                                    int length_on)
 {
     int bx, by, bz;
     const BlockRec &b = blks[batch_wg(map, kg, bx, by, bz)];
@@ -385,8 +386,9 @@ __global__ void __launch_bounds__(256) k_sa_a_batch(const BlockRec *__restrict__
     long n;
     if (!lin_node_at(e, all, bx, i, j, k, n)) return;
     const double *pri = arena + b.o.pri, *nut = arena + b.o.nut, *amu = arena + b.o.amu;
-    arena[b.o.length + n] = sa_length_at(e, c, n, arena + b.o.dist, arena + b.o.spacing, pri, nut, amu,
-                                         arena + b.o.dudx);
+    if (length_on)
+        arena[b.o.length + n] = sa_length_at(e, c, n, arena + b.o.dist, arena + b.o.spacing, pri, nut, amu,
+                                             arena + b.o.dudx);
     if (i < e.is1 || i > e.ie1 || j < e.js1 || j > e.je1 || k < e.ks1 || k > e.ke1) return;
     sa_a_at<false>(e, i, j, k, n, pri, arena + b.o.dxidx, nut, amu, arena + b.o.aal, arena + b.o.bbe,
                    arena + b.o.gga, arena + b.o.sa_rhs, arena + b.o.w_sa, DERIVE ? arena + b.o.inv_j : nullptr);
@@ -422,13 +424,14 @@ __global__ void __launch_bounds__(256, 5) k_sa_b_batch(const BlockRec *__restric
 }
 
 int launch_sa_pre_batch(hipStream_t s, const BlockRec *blks, const BatchGrid &bg, double *arena, const Consts &c,
-                        const double *cfl, int cfl_stride, int part, bool derive_abg)
+                        const double *cfl, int cfl_stride, int part, bool derive_abg, bool length_on)
 {
     if (bg.total <= 0) return 0;
     const dim3 g((unsigned)bg.total), t(64, 4, 1);
+    const int lo = length_on ? 1 : 0;
     if (part != 2) {
-        if (derive_abg) hipLaunchKernelGGL(k_sa_a_batch<true>, g, t, 0, s, blks, bg.map, bg.kg, arena, c);
-        else hipLaunchKernelGGL(k_sa_a_batch<false>, g, t, 0, s, blks, bg.map, bg.kg, arena, c);
+        if (derive_abg) hipLaunchKernelGGL(k_sa_a_batch<true>, g, t, 0, s, blks, bg.map, bg.kg, arena, c, lo);
+        else hipLaunchKernelGGL(k_sa_a_batch<false>, g, t, 0, s, blks, bg.map, bg.kg, arena, c, lo);
     }
     if (part != 1) hipLaunchKernelGGL(k_sa_b_batch, g, t, 0, s, blks, bg.map, bg.kg, arena, c, cfl, cfl_stride);
     ACFD_CHECK_LAUNCH();

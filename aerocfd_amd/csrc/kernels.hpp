@@ -172,9 +172,12 @@ int launch_sa_post(hipStream_t s, const Ext &e, const Fields &f);
 // launches (linear-tile grid); block m reads its cfl at cfl[m * cfl_stride];
 // part 1 / 2 launches only the first / second pass (per-kernel timing);
 // derive_abg: alpha, beta, gamma are the products of dxidx and 1/J that the
-// grid set-up stores (sa_metric_products), formed again instead of read
+// grid set-up stores (sa_metric_products), formed again instead of read;
+// length_on: the first pass forms the length scale (off while it holds the
+// flow-independent length of SA or DES97 already)
 int launch_sa_pre_batch(hipStream_t s, const BlockRec *blks, const BatchGrid &bg, double *arena, const Consts &c,
-                        const double *cfl, int cfl_stride, int part = 0, bool derive_abg = false);
+                        const double *cfl, int cfl_stride, int part = 0, bool derive_abg = false,
+                        bool length_on = true);
 
 // k_bc.hip
 struct BcTask {       // one physical subface

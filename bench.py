@@ -82,6 +82,9 @@ VISCOUS_NODE_SA = BYTES_PER_NODE["viscous_node"] + 8
 # stage A when the metrics come from acfd_grid_metrics: 1/J in place of
 # alpha/beta/gamma
 SA_PRE_DERIVED = BYTES_PER_NODE["sa_pre"] - 72 + 8
+# stage A once the flow-independent length scale (plain SA, DES97) is
+# formed: no dist, spacing in, no length out
+SA_PRE_LENGTH_KEPT = 24
 # the tile sweep backward pass adds dq to q itself on one block's launch
 # under 2^22 computed nodes (never in the batched sweeps): q in and q out
 # instead of dq out (the add pass is skipped)
@@ -349,7 +352,7 @@ def timed_run(args, case, fields, coords, precision):
     return t1 - t0, tm, tsteps, res, rr
 
 
-def roofline_of(args, tm, rr, sa, derived=False):
+def roofline_of(args, tm, rr, sa, derived=False, length_kept=False):
     """The dominant single-kernel timer (most GPU time per step) and its
     achieved algorithmic GB/s."""
     comp_all = sum((r[13] - r[12] + 1) * (r[15] - r[14] + 1) * (r[17] - r[16] + 1) for r in rr)
@@ -363,6 +366,8 @@ def roofline_of(args, tm, rr, sa, derived=False):
         table["viscous_node"] = VISCOUS_NODE_SA
     if sa and derived:
         table["sa_pre"] = SA_PRE_DERIVED
+    if sa and length_kept and args.lusgs == 2:
+        table["sa_pre"] -= SA_PRE_LENGTH_KEPT
     # the viscous node pass with the steady find_dt: + rho, c in, dt out;
     # when it also forms implicit1st's preparation (batched sweeps, no
     # separate preparation timed): + the time term phi/dt out, and the
@@ -465,7 +470,8 @@ def main():
     # nodes of the single block
     nodes = case.totalnodes if fp else args.n ** 3
     value = nodes * args.steps / secs / 1e6
-    dom, avg_ms, achieved, bpn = roofline_of(args, tm, rr, sa, derived=coords is not None)
+    dom, avg_ms, achieved, bpn = roofline_of(args, tm, rr, sa, derived=coords is not None,
+                                             length_kept=case.cfg.iflag_des != 2)
     default_cfg = (not fp and args.n == 256 and args.lusgs == 1 and args.precision == 0 and args.cuts == 1
                    and args.scheme == "reference")
     traffic = MEASURED_TRAFFIC_256.get(dom) if default_cfg else None

@@ -173,3 +173,28 @@ def test_channel_bump_uploaded_metric_products():
         assert np.array_equal(qa, qb) and np.array_equal(na, nb)
     assert any(not np.array_equal(na, nc) for (_, na), (_, nc) in zip(out[0], out[2]))
     say("  derived == read; scaled products change nut")
+
+
+def test_channel_bump_sa_length_after_dist_upload():
+    """The batched SA update forms the DES97 length scale once and keeps it;
+    an upload of the wall distance between steps makes it form the length
+    again: batched and block-by-block runs stay bit-identical across it."""
+    say("test_channel_bump_sa_length_after_dist_upload")
+    out = []
+    for lusgs, batch, graph in ((1, 0, 0), (2, 1, 1)):
+        case = cases.channel_bump(**SMALL)
+        sol = start(case, lusgs=lusgs, batch=batch, graph=graph)
+        try:
+            sol.step(3)
+            for m in range(len(case.dims)):
+                sol.upload(m + 1, "dist", sol.download(m + 1, "dist") * 1.1)
+            sol.step(3)
+            out.append([(sol.download(m + 1, "q"), sol.download(m + 1, "nut"), sol.download(m + 1, "length"))
+                        for m in range(len(case.dims))])
+        finally:
+            sol.close()
+    for m, ((qa, na, la), (qb, nb, lb)) in enumerate(zip(*out)):
+        assert np.array_equal(la, lb), (m, np.abs(la - lb).max())
+        assert np.array_equal(qa, qb), (m, np.abs(qa - qb).max())
+        assert np.array_equal(na, nb), (m, np.abs(na - nb).max())
+    say("  identical over %d blocks" % len(out[0]))
