@@ -119,19 +119,19 @@ HYPERPLANE_BYTES = {  # the hyperplane sweeps read the operands in place
 MEASURED_TRAFFIC_CHANNEL = {
     "getpri": 10.034e9,
     "inviscid_i": 11.015e9,
-    "inviscid_j": 17.102e9,
-    "inviscid_k": 18.419e9,
-    "viscous_node_dt": 30.727e9,
-    "viscous_diff": 20.762e9,
-    "lusgs_gather": 23.393e9,
-    "lusgs_forward": 28.464e9,
-    "lusgs_backward": 28.461e9,
-    "lusgs_add": 9.255e9,
-    "sa_pre": 19.906e9,
-    "sa_pre_b": 24.757e9,
+    "inviscid_j": 17.094e9,
+    "inviscid_k": 18.420e9,
+    "viscous_node_dt": 30.728e9,
+    "viscous_diff": 20.764e9,
+    "lusgs_gather": 23.391e9,
+    "lusgs_forward": 28.450e9,
+    "lusgs_backward": 28.455e9,
+    "lusgs_add": 9.256e9,
+    "sa_pre": 17.182e9,
+    "sa_pre_b": 24.793e9,
     "sa_gather": 9.872e9,
-    "sa_forward": 6.162e9,
-    "sa_backward": 6.166e9,
+    "sa_forward": 6.173e9,
+    "sa_backward": 6.163e9,
     "sa_add": 3.286e9,
 }
 # The TGV 256^3 block, FP64, tile sweeps (profiles/tgv256_fp64_hbm_traffic.md).
@@ -139,9 +139,9 @@ MEASURED_TRAFFIC_256 = {
     "getpri": 1.687e9,
     "lusgs_other": 2.322e9,
     "lusgs_gather": 4.092e9,
-    "lusgs_forward": 4.457e9,
+    "lusgs_forward": 4.456e9,
     "lusgs_backward": 4.456e9,
-    "lusgs_add": 1.577e9,
+    "lusgs_add": 1.576e9,
 }
 
 
