@@ -446,8 +446,8 @@ def main():
         args.graph = True
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; every block of the workload is resident on it. "
-              "Sharding blocks over several GPUs is described in DESIGN.md but not built, so --gpus %d "
-              "is not supported." % args.gpus, file=sys.stderr)
+              "Sharding blocks over several GPUs is not built, so --gpus %d is not supported." % args.gpus,
+              file=sys.stderr)
         sys.exit(2)
     if args.scheme != "reference" and fp:
         print("bench.py: --scheme applies to the TGV workload", file=sys.stderr)

@@ -13,7 +13,7 @@
  * first (AoS): q(5,is:ie,js:je,ks:ke) is q[5*IX(i,j,k) + (v-1)].
  *
  * Pinning (what each function is checked against) is listed in the header
- * comment of each source file and summarised per function in DESIGN.md.
+ * comment of each source file.
  */
 #ifndef ACFD_ORACLE_H
 #define ACFD_ORACLE_H

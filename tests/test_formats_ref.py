@@ -12,7 +12,8 @@ sub_init_allocatememory.f90:36-71,388-417, sub_init_readbcinfo.f90:45-104,
 sub_init_flowfield.f90:126-262) and writers (sub_output_flowfield.f90:30-81,
 sub_output_residual.f90:62-64) are thereby pinned to formats.py.
 
-Skipped where the compiled reference is not built (the GPU box).
+Skipped where the compiled reference is not built (a machine without the
+reference sources).
 """
 import numpy as np
 import pytest
