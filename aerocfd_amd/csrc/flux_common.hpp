@@ -15,25 +15,6 @@ namespace acfd {
 // sqrt(rho_l rho_r)/rho_l.  Results agree with the reference's operation order to
 // rounding.
 //
-// 1/x for x of a positive normal range (WENO3's weight denominators,
-// densities, 1 + sqrt(rho_r/rho_l), gamma - 1, 2 c^2): the hardware's reciprocal estimate and
-// two Newton-Raphson steps r <- r + r(1 - x r), within an ulp of the
-// correctly rounded quotient (the inviscid kernels 6-7 % faster than with
-// the division sequence); float keeps its division
-template <typename R>
-__device__ inline R rcp_pos(R x)
-{
-    return (R)1.0 / x;
-}
-template <>
-__device__ inline double rcp_pos<double>(double x)
-{
-    double r = __builtin_amdgcn_rcp(x);
-    double e = fma(-x, r, 1.0);
-    r = fma(r, e, r);
-    e = fma(-x, r, 1.0);
-    return fma(r, e, r);
-}
 
 // WENO3 state at a face from three values; sub_inviscid_upwind.f90:32-41
 // (right state: u1,u2,u3 = f(a),f(a+1),f(a+2)) and :57-66 (left state:

@@ -209,16 +209,16 @@ __device__ inline double sa_b_at(const Ext &e, int i, int j, int k, long n, cons
     // one division per distinct denominator, multiplications by its
     // reciprocal elsewhere (the reference's expressions to a few ulps; pass
     // B 5.81 -> 5.68 ms on the channel)
-    const double rkar2 = 1.0 / kar2, rre = 1.0 / re;
+    const double rkar2 = 1.0 / kar2, rre = rcp_pos(re);
     double d = ld_once(pri + n);
-    const double rnu0 = d / ld_once(amu + n);  // 1 / nu0
+    const double rnu0 = d * rcp_pos(ld_once(amu + n));  // 1 / nu0
     double omega = fabs(ld_once(vor + n));
     double dist0 = ld_once(length + n);
     double leix = fmax(nut0 * rnu0, 0.001);
     double leix2 = leix * leix, leix3 = leix * leix * leix;
-    const double den = leix3 + cv13, rden = 1.0 / den;
+    const double den = leix3 + cv13, rden = rcp_pos(den);
     double fv1 = leix3 * rden;
-    const double den2 = 1.0 + leix * fv1, rden2 = 1.0 / den2;
+    const double den2 = 1.0 + leix * fv1, rden2 = rcp_pos(den2);
     double fv2 = 1.0 - leix * rden2;
     double dist2 = dist0 * dist0;
     const double rkd = 1.0 / (re * kar2 * dist2), rd2 = 1.0 / dist2;
@@ -231,7 +231,7 @@ __device__ inline double sa_b_at(const Ext &e, int i, int j, int k, long n, cons
     double r6 = r * r * r * r * r * r;
     double g = r + sa::cw2 * (r6 - r);
     double gg6 = g * g * g * g * g * g;
-    double fw = g * pow((1.0 + cw36) / (gg6 + cw36), 1.0 / 6.0);
+    double fw = g * pow((1.0 + cw36) * rcp_pos(gg6 + cw36), 1.0 / 6.0);
     double ft2 = sa::ct3 * exp(-sa::ct4 * leix2);
     double aaa = sa::cb1 * ((1.0 - ft2) * fv2 + ft2) * rkar2;
     double bbb = sa::cw1 * fw;
@@ -247,7 +247,7 @@ __device__ inline double sa_b_at(const Ext &e, int i, int j, int k, long n, cons
     double sn = rkd * (fv2 + fv2n * nut0);
     div = (s * s) * (re * kar2 * dist2);
     div = fmax(div, 1.0e-6);
-    double rn = (s - sn * nut0) / div;
+    double rn = (s - sn * nut0) * rcp_pos(div);
     double r5 = r * r * r * r * r;
     double gn = (1.0 + sa::cw2 * 6.0 * r5 - sa::cw2) * rn;
     // the reference's integer powers cw3**(-6), g**(-6), g**(-7) as
