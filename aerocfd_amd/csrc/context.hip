@@ -180,6 +180,9 @@ struct acfd_ctx {
     // again (an upload of dist, spacing or length, or acfd_grid_metrics,
     // clears it)
     bool sa_length_ready = false;
+    // d_res holds the norms of the current rhs, dt and sa_rhs (set by
+    // acfd_step, cleared by every call that writes fields)
+    bool res_current = false;
     int *lu_ctr = nullptr;   // ticket counters of the batched sweeps
     // SA's update runs beside the flow's LU-SGS update on a second stream
     // (batched sweeps, SA): its own exchange slots and ticket counters
@@ -1504,9 +1507,10 @@ int one_step(acfd_ctx *ctx)
     } else {
         return fail(ACFD_EINVAL, "iflag_timeadvance must be 2, 3 or 4");
     }
-    // no residual here: the reference forms it at its output interval only
-    // (main.f90:575-603), acfd_residual when asked, from the rhs, dt and
-    // sa_rhs the step leaves
+    // the residual norms of every step, as the reference's loop forms them
+    // (its output interval is 1, main.f90:341,575-603); acfd_residual reads
+    // them back
+    if (!rc) rc = launch_residual_all(ctx) ? ACFD_EDEVICE : 0;
     if (!rc) ctx->pri_current = true;
     return rc;
 }
@@ -1836,6 +1840,7 @@ int acfd_upload(acfd_ctx *ctx, int m, int field, const double *host)
     if (!ctx || !ctx->topo || m < 1 || m > (int)ctx->blk.size() || !host || field < 1 || field > kLastField)
         return fail(ACFD_EINVAL, "acfd_upload: bad arguments");
     ctx->pri_current = false;
+    ctx->res_current = false;
     if (field == ACFD_RHSV) {
         int rc = rhsv_refresh(ctx);  // the other blocks' rhsv
         if (rc) return rc;
@@ -1894,6 +1899,7 @@ int acfd_run_stage(acfd_ctx *ctx, int stage)
 {
     if (!ctx || !ctx->topo) return fail(ACFD_ESTATE, "acfd_run_stage: no topology");
     ctx->pri_current = false;
+    ctx->res_current = false;
     ctx->lu_prep_ready = false;
     const int nb = (int)ctx->blk.size();
     int rc = rhsv_refresh(ctx);  // the stages read rhsv or reuse w_flux
@@ -1941,6 +1947,7 @@ int acfd_prepare(acfd_ctx *ctx)
 {
     if (!ctx || !ctx->topo) return fail(ACFD_ESTATE, "acfd_prepare: no topology");
     ctx->pri_current = false;
+    ctx->res_current = false;
     int rc = refresh(ctx);
     if (rc) return rc;
     HIP_TRY(hipStreamSynchronize(ctx->stream));
@@ -1950,6 +1957,7 @@ int acfd_prepare(acfd_ctx *ctx)
 int acfd_step(acfd_ctx *ctx, int nsteps)
 {
     if (!ctx || !ctx->topo) return fail(ACFD_ESTATE, "acfd_step: no topology");
+    if (nsteps > 0) ctx->res_current = false;
     int s = 0;
     // graph mode: the first step runs eagerly (it also tunes the tile
     // sweeps), the next one is captured, and every later step replays it;
@@ -1984,6 +1992,7 @@ int acfd_step(acfd_ctx *ctx, int nsteps)
         if (rc) return rc;
     }
     HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (nsteps > 0) ctx->res_current = true;
     int rc = tile_error(ctx);
     if (rc) return rc;
     return check_nneg(ctx);
@@ -1992,7 +2001,9 @@ int acfd_step(acfd_ctx *ctx, int nsteps)
 int acfd_residual(acfd_ctx *ctx, double *out)
 {
     if (!ctx || !ctx->topo || !out) return fail(ACFD_EINVAL, "acfd_residual: bad arguments");
-    int rc = launch_residual_all(ctx);
+    // the last step's norms stand until a call changes the fields they
+    // come from; otherwise formed now
+    int rc = ctx->res_current ? 0 : launch_residual_all(ctx);
     if (rc) return rc;
     double s[6];
     HIP_TRY(hipMemcpyAsync(s, ctx->d_res, sizeof(s), hipMemcpyDeviceToHost, ctx->stream));
@@ -2050,6 +2061,7 @@ int acfd_timing(acfd_ctx *ctx, int enable, double *ms, long *calls)
 int acfd_grid_metrics(acfd_ctx *ctx, double *const *xyz)
 {
     if (!ctx || !ctx->topo || !xyz) return fail(ACFD_EINVAL, "acfd_grid_metrics: bad arguments");
+    ctx->res_current = false;
     {
         const int rc = rhsv_refresh(ctx);  // w_flux is scratch below
         if (rc) return rc;

@@ -117,29 +117,29 @@ HYPERPLANE_BYTES = {  # the hyperplane sweeps read the operands in place
 # (profiles/channel_bump_fp64_hbm_traffic.md); the sweeps' figures include
 # the refill of the exchange slots that shares their timer.
 MEASURED_TRAFFIC_CHANNEL = {
-    "getpri": 10.034e9,
-    "inviscid_i": 11.020e9,
-    "inviscid_j": 17.101e9,
-    "inviscid_k": 17.824e9,
-    "viscous_node_dt": 30.729e9,
+    "getpri": 10.033e9,
+    "inviscid_i": 11.021e9,
+    "inviscid_j": 17.097e9,
+    "inviscid_k": 17.817e9,
+    "viscous_node_dt": 30.728e9,
     "viscous_diff": 20.763e9,
-    "lusgs_gather": 23.393e9,
-    "lusgs_forward": 28.465e9,
-    "lusgs_backward": 28.461e9,
+    "lusgs_gather": 23.392e9,
+    "lusgs_forward": 28.469e9,
+    "lusgs_backward": 28.466e9,
     "lusgs_add": 9.256e9,
-    "sa_pre": 17.184e9,
-    "sa_pre_b": 24.719e9,
+    "sa_pre": 17.185e9,
+    "sa_pre_b": 24.725e9,
     "sa_gather": 9.872e9,
-    "sa_forward": 6.163e9,
-    "sa_backward": 6.171e9,
+    "sa_forward": 6.167e9,
+    "sa_backward": 6.158e9,
     "sa_add": 3.286e9,
 }
 # The TGV 256^3 block, FP64, tile sweeps (profiles/tgv256_fp64_hbm_traffic.md).
 MEASURED_TRAFFIC_256 = {
     "getpri": 1.687e9,
     "lusgs_other": 2.322e9,
-    "lusgs_gather": 4.093e9,
-    "lusgs_forward": 4.457e9,
+    "lusgs_gather": 4.092e9,
+    "lusgs_forward": 4.456e9,
     "lusgs_backward": 4.456e9,
     "lusgs_add": 1.576e9,
 }

@@ -129,9 +129,10 @@ int acfd_prepare(acfd_ctx *ctx);
 /* nsteps time steps of main.f90:343-573 (iflag_timeadvance 2 or 3). */
 int acfd_step(acfd_ctx *ctx, int nsteps);
 /* RMS residual of sub_output_residual.f90:24-56: out[0..4] main equations,
- * out[5] SA, out[6] = 0; formed when called, from the rhs, dt and sa_rhs
- * of the last step or stage (the reference forms it at its output
- * interval, main.f90:575-603). */
+ * out[5] SA, out[6] = 0.  Every step forms it, as the reference's loop does
+ * (output interval 1, main.f90:341,575-603), and this call reads it back;
+ * after a call that writes fields it is formed from the current rhs, dt and
+ * sa_rhs. */
 int acfd_residual(acfd_ctx *ctx, double *out);
 /* out = dtmax, dtmin, cflmax, cflmin of the last find_dt call and the
  * current cfl (main.f90:589-592). */
